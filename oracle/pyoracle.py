@@ -1,0 +1,160 @@
+"""ctypes wrapper around the oracle C restatement (oracle/swe_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker.  The product (samoa_amd/) never
+imports it.
+
+Also holds the readers for the reference's VTU dumps (SWE_xml_output.f90:220-241,
+LIB_VTK_IO raw appended big-endian data) used by tests/golden/make_golden.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libswe_oracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+BC_OUTFLOW = 0
+BC_WALL = 1
+SOLVER_AUG = 0
+SOLVER_FWAVE = 1
+G = 9.80665
+
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        L.swo_leg_sizes.argtypes = [_dp]
+        L.swo_riemanntype.argtypes = [C.c_double] * 6 + [C.POINTER(C.c_double)] * 3 + [
+            C.POINTER(C.c_int)] * 2
+        L.swo_geoclaw.argtypes = [C.c_int] + [C.c_double] * 10 + [_dp, _dp, C.POINTER(C.c_double)]
+        L.swo_flux.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, _dp, _dp]
+        L.swo_normals.argtypes = [C.c_int, _dp]
+        L.swo_mesh_uniform.argtypes = [C.c_int, _i8p, _dp, _i32p]
+        L.swo_mesh_uniform.restype = C.c_int64
+        step_args = [C.c_int64, _i8p, _i8p, _i32p, C.c_double, C.c_double, C.c_double,
+                     C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                     _dp, _dp, _dp, _dp, _dp, _dp, _dp, _i8p, C.POINTER(C.c_int64)]
+        L.swo_step.argtypes = step_args
+        L.swo_step.restype = C.c_double
+        L.swo_step_omp.argtypes = step_args
+        L.swo_step_omp.restype = C.c_double
+        L.swo_initial_dt.argtypes = [C.c_int64, _i8p, _dp, _dp, _dp, _dp, C.c_double, C.c_double]
+        L.swo_initial_dt.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def leg_sizes() -> np.ndarray:
+    """l(k) for k = -1..62 at index k+1."""
+    out = np.zeros(64)
+    lib().swo_leg_sizes(out)
+    return out
+
+
+def riemanntype(hL, hR, uL, uR, drytol, g=G):
+    hm, s1, s2 = C.c_double(), C.c_double(), C.c_double()
+    r1, r2 = C.c_int(), C.c_int()
+    lib().swo_riemanntype(hL, hR, uL, uR, drytol, g, C.byref(hm), C.byref(s1), C.byref(s2),
+                          C.byref(r1), C.byref(r2))
+    return hm.value, s1.value, s2.value, r1.value, r2.value
+
+
+def geoclaw(hL, hR, huL, huR, hvL, hvR, bL, bR, drytol, g=G, solver=SOLVER_AUG):
+    nl, nr, s = np.zeros(3), np.zeros(3), C.c_double()
+    lib().swo_geoclaw(solver, hL, hR, huL, huR, hvL, hvR, bL, bR, drytol, g, nl, nr, C.byref(s))
+    return nl, nr, s.value
+
+
+def flux(n, QL, QR, drytol, g=G, solver=SOLVER_AUG):
+    fl, fr = np.zeros(4), np.zeros(4)
+    lib().swo_flux(solver, np.ascontiguousarray(n, dtype=np.float64),
+                   np.ascontiguousarray(QL, dtype=np.float64),
+                   np.ascontiguousarray(QR, dtype=np.float64), drytol, g, fl, fr)
+    return fl, fr
+
+
+def normals(p: int) -> np.ndarray:
+    out = np.zeros(6)
+    lib().swo_normals(p, out)
+    return out.reshape(3, 2)
+
+
+def mesh_uniform(depth: int):
+    """Returns (plotter[N] int8, coords[N,3,2] float64, nbr[N,3] int32) in forward SFC order."""
+    n = 1 << (depth + 1)
+    plotter = np.zeros(n, np.int8)
+    coords = np.zeros(6 * n)
+    nbr = np.zeros(3 * n, np.int32)
+    got = lib().swo_mesh_uniform(depth, plotter, coords, nbr)
+    assert got == n
+    return plotter, coords.reshape(n, 3, 2), nbr.reshape(n, 3)
+
+
+def step(depth, plotter, nbr, state, dt, *, scaling=1.0, drytol=0.01, dmin=None, dmax=None,
+         bc=BC_OUTFLOW, solver=SOLVER_AUG, forward=True, g=G, threads=False):
+    """One reference Euler traversal on (eta, hu, hv, b).  Returns
+    (eta', hu', hv', refine, dt_candidate_min, n_refine)."""
+    eta, hu, hv, b = (np.ascontiguousarray(a, dtype=np.float64) for a in state)
+    n = eta.shape[0]
+    depth = np.ascontiguousarray(depth, dtype=np.int8)
+    d = int(depth.max()) if n else 0
+    dmin = d if dmin is None else dmin
+    dmax = d if dmax is None else dmax
+    eo, ho, vo = np.empty_like(eta), np.empty_like(eta), np.empty_like(eta)
+    rf = np.zeros(n, np.int8)
+    nref = C.c_int64()
+    fn = lib().swo_step_omp if threads else lib().swo_step
+    cand = fn(n, depth, np.ascontiguousarray(plotter, dtype=np.int8),
+              np.ascontiguousarray(nbr, dtype=np.int32).reshape(-1), scaling, g, drytol,
+              dmin, dmax, bc, solver, int(bool(forward)), dt, eta, hu, hv, b, eo, ho, vo, rf,
+              C.byref(nref))
+    return eo, ho, vo, rf, cand, nref.value
+
+
+def initial_dt(depth, state, courant=0.45, g=G):
+    eta, hu, hv, b = (np.ascontiguousarray(a, dtype=np.float64) for a in state)
+    return lib().swo_initial_dt(eta.shape[0], np.ascontiguousarray(depth, dtype=np.int8),
+                                eta, hu, hv, b, g, courant)
+
+
+# --------------------------------------------------------------------------
+# Reference VTU reader (SWE_xml_output.f90:220-241; LIB_VTK_IO 'binary' =
+# raw appended, big-endian, each block prefixed by an Int32 byte count).
+
+_VTK_TYPES = {"Float64": ">f8", "Int32": ">i4", "Int8": ">i1"}
+
+
+def read_vtu(path: str) -> dict:
+    raw = open(path, "rb").read()
+    head_end = raw.index(b"<AppendedData")
+    head = raw[:head_end].decode("latin-1")
+    start = raw.index(b"_", head_end) + 1
+    out = {}
+    for m in re.finditer(r'<DataArray type="(\w+)"([^>]*)>', head):
+        typ, attrs = m.group(1), m.group(2)
+        name = re.search(r'Name="([^"]+)"', attrs).group(1)
+        off = int(re.search(r'offset="(\d+)"', attrs).group(1))
+        nb = int(np.frombuffer(raw[start + off:start + off + 4], ">i4")[0])
+        arr = np.frombuffer(raw[start + off + 4:start + off + 4 + nb], _VTK_TYPES[typ])
+        out[name] = arr.astype(arr.dtype.newbyteorder("="))
+    return out

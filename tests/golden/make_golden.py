@@ -1,0 +1,215 @@
+"""Writes the golden fixtures under tests/golden/ by executing the reference.
+
+Runs only in a container that holds the reference tree and the binaries that
+`make -C oracle ref` compiles from it (oracle/_ref/).  The fixtures are data
+(inputs and the reference's outputs); the tests compare the oracle and the HIP
+library against them, so neither the reference nor its binaries are needed at
+test time.
+
+    python tests/golden/make_golden.py
+
+Fixtures:
+  kat_geoclaw.npz      inputs (n,10) + reference outputs of c_bind_geoclaw_solver
+                       (c_bind_riemannsolvers.f90:16-176) for the augmented
+                       (GEOCLAW_AUG_RIEMANN) and f-wave (GEOCLAW_FWAVE) solvers
+  kat_riemanntype.npz  inputs (n,6) + outputs of riemanntype (riemannsolvers.f:507-618)
+  tables.npz           bit patterns of r_leg_sizes / r_volumes (SFC_data_types.f90:524,557)
+  run_*.npz            full SWE runs of the reference binary: mesh point order,
+                       per-output state (forward SFC order), time, refinement flags
+"""
+from __future__ import annotations
+
+import glob
+import json
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import pyoracle as po  # noqa: E402
+
+G = 9.80665
+F32_COURANT = float(np.float32(0.45))  # Config.f90:243 rget() is default REAL
+F32_DRYTOL = float(np.float32(0.01))   # Config.f90:284
+
+
+def kat_inputs(seed: int = 20261202) -> np.ndarray:
+    """Designed Riemann states: wet/wet shocks, rarefactions, transcritical and
+    sonic states, wet/dry with and without a wall, dry/dry, lake at rest,
+    identical states, the drytol boundary itself, and random states."""
+    rng = np.random.default_rng(seed)
+    dry = F32_DRYTOL
+    rows = []
+
+    def add(hL, hR, huL, huR, hvL, hvR, bL, bR):
+        rows.append([hL, hR, huL, huR, hvL, hvR, bL, bR, dry, G])
+
+    for _ in range(3000):  # random wet/wet
+        hL, hR = rng.uniform(0.05, 20.0, 2)
+        uL, uR, vL, vR = rng.uniform(-8, 8, 4)
+        bL, bR = rng.uniform(-3, 3, 2)
+        add(hL, hR, hL * uL, hR * uR, hL * vL, hR * vR, bL, bR)
+    for _ in range(800):  # strong shocks / rarefactions, flat bed
+        hL, hR = 10 ** rng.uniform(-1.5, 1.5, 2)
+        s = rng.choice([-1.0, 1.0])
+        uL, uR = s * rng.uniform(0, 12), -s * rng.uniform(0, 12)
+        add(hL, hR, hL * uL, hR * uR, 0.0, 0.0, 0.0, 0.0)
+    for _ in range(800):  # transcritical / sonic: |u| close to sqrt(g h)
+        hL, hR = rng.uniform(0.1, 5.0, 2)
+        cL, cR = np.sqrt(G * hL), np.sqrt(G * hR)
+        uL = rng.choice([-1, 1]) * cL * (1 + rng.normal(0, 1e-3))
+        uR = rng.choice([-1, 1]) * cR * (1 + rng.normal(0, 1e-3))
+        bL, bR = rng.uniform(-0.5, 0.5, 2)
+        add(hL, hR, hL * uL, hR * uR, hL * rng.normal(), hR * rng.normal(), bL, bR)
+    for _ in range(800):  # wet/dry, either side, with and without a wall
+        hW = rng.uniform(0.02, 5.0)
+        hD = rng.choice([0.0, dry * rng.uniform(0, 1), dry, -rng.uniform(0, 1)])
+        uW, vW = rng.uniform(-5, 5, 2)
+        bW = rng.uniform(-3, 0)
+        bD = bW + rng.choice([rng.uniform(0, 2 * hW), rng.uniform(2 * hW, 10), -rng.uniform(0, 2)])
+        huD = rng.normal() * abs(hD)
+        if rng.random() < 0.5:
+            add(hW, hD, hW * uW, huD, hW * vW, 0.3 * huD, bW, bD)
+        else:
+            add(hD, hW, huD, hW * uW, 0.3 * huD, hW * vW, bD, bW)
+    for _ in range(200):  # dry/dry
+        hL, hR = rng.uniform(-1, dry, 2)
+        add(hL, hR, rng.normal(), rng.normal(), rng.normal(), rng.normal(), rng.normal(),
+            rng.normal())
+    for _ in range(600):  # lake at rest: eta equal, p = 0, varying b
+        eta = rng.uniform(-1, 1)
+        bL, bR = eta - rng.uniform(0.02, 10, 2)
+        add(eta - bL, eta - bR, 0.0, 0.0, 0.0, 0.0, bL, bR)
+    for _ in range(400):  # identical states (outflow ghost)
+        h = rng.uniform(0.0, 10.0)
+        u, v, b = rng.normal(0, 3, 3)
+        add(h, h, h * u, h * u, h * v, h * v, b, b)
+    for _ in range(200):  # mirrored states (wall ghost)
+        h = rng.uniform(0.02, 10.0)
+        u, v, b = rng.normal(0, 3, 3)
+        add(h, h, h * u, -h * u, h * v, h * v, b, b)
+    for _ in range(200):  # zero velocities, flat bed, depth jump
+        hL, hR = rng.uniform(0.0, 10.0, 2)
+        add(hL, hR, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    return np.asarray(rows, dtype=np.float64)
+
+
+def rt_inputs(seed: int = 7) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    dry = F32_DRYTOL
+    rows = []
+    for _ in range(3000):
+        hL, hR = 10 ** rng.uniform(-3, 1.5, 2)
+        uL, uR = rng.uniform(-15, 15, 2)
+        rows.append([hL, hR, uL, uR, dry, G])
+    for _ in range(300):
+        h = rng.uniform(0, 3 * dry)
+        rows.append([h, rng.uniform(0, 5), rng.normal(0, 3), rng.normal(0, 3), dry, G])
+        rows.append([rng.uniform(0, 5), h, rng.normal(0, 3), rng.normal(0, 3), dry, G])
+    for _ in range(300):  # the wall test call of c_bind: (hL, hL, uL, -uL)
+        h = rng.uniform(dry, 10)
+        u = rng.normal(0, 5)
+        rows.append([h, h, u, -u, dry, G])
+    return np.asarray(rows, dtype=np.float64)
+
+
+def run_kat(mode: str, inputs: np.ndarray, *extra: str, nout: int) -> np.ndarray:
+    with tempfile.TemporaryDirectory() as td:
+        fin, fout = os.path.join(td, "in.bin"), os.path.join(td, "out.bin")
+        inputs.astype("<f8").tofile(fin)
+        subprocess.run([os.path.join(REF, "ref_kat"), mode, *extra, fin, fout], check=True)
+        return np.fromfile(fout, "<f8").reshape(-1, nout)
+
+
+def run_reference(binary: str, depth: int, steps: int, nout: int) -> list[dict]:
+    od = tempfile.mkdtemp(prefix="swe_golden_")
+    try:
+        cmd = [os.path.join(REF, binary), "-threads", "1", "-sections", "1",
+               "-dstart", str(depth), "-dmin", str(depth), "-dmax", str(depth),
+               "-nadapt", "-1", "-tmax", "-1", "-nmax", str(steps), "-xmloutput", ".true.",
+               "-nout", str(nout), "-output_dir", od]
+        subprocess.run(cmd, check=True, capture_output=True)
+        files = sorted(glob.glob(os.path.join(od, "*_r0_s1.vtu")),
+                       key=lambda f: int(f.split("_")[-3]))
+        return [po.read_vtu(f) for f in files]
+    finally:
+        shutil.rmtree(od, ignore_errors=True)
+
+
+def write_run(name: str, binary: str, depth: int, steps: int, nout: int, bc: int) -> None:
+    outs = run_reference(binary, depth, steps, nout)
+    S, off = 10.0, -5.0  # SWE.f90:169-170 (non-ASAGI domain [-5,5]^2)
+    n = 1 << (depth + 1)
+    pts0 = outs[0]["Point"].reshape(n, 3, 3)[:, :, :2]
+    # map reference output order -> forward SFC index via the oracle mesh
+    _, co, _ = po.mesh_uniform(depth)
+    cen = (S * co + off).mean(axis=1)
+    index = {tuple(np.round(c, 9)): i for i, c in enumerate(cen)}
+    states, times, refine, order_fwd = [], [], [], []
+    for v in outs:
+        pts = v["Point"].reshape(n, 3, 3)[:, :, :2]
+        perm = np.array([index[tuple(np.round(c, 9))] for c in pts.mean(axis=1)])
+        st = np.zeros((4, n))
+        st[0, perm] = v["water height"]
+        st[3, perm] = v["bathymetry"]
+        m = v["momentum"].reshape(n, 3)
+        st[1, perm] = m[:, 0]
+        st[2, perm] = m[:, 1]
+        rf = np.zeros(n, np.int8)
+        rf[perm] = v["refinement flag"]
+        states.append(st)
+        times.append(v["time"][0])
+        refine.append(rf)
+        order_fwd.append(bool(np.array_equal(perm, np.arange(n))))
+        assert order_fwd[-1] or np.array_equal(perm, np.arange(n)[::-1])
+    meta = dict(binary=binary, depth=depth, steps=steps, nout=nout, bc=bc, scaling=S,
+                offset=off, courant=F32_COURANT, drytol=F32_DRYTOL,
+                cmd="-threads 1 -sections 1 -dstart d -dmin d -dmax d -nadapt -1 -tmax -1 "
+                    f"-nmax {steps} -xmloutput .true. -nout {nout}")
+    np.savez_compressed(os.path.join(HERE, f"run_{name}.npz"), states=np.asarray(states),
+                        times=np.asarray(times), refine=np.asarray(refine),
+                        order_fwd=np.asarray(order_fwd), points0=pts0,
+                        meta=json.dumps(meta))
+    print(f"run_{name}: {len(outs)} outputs, n={n}")
+
+
+def main() -> None:
+    if not os.path.exists(os.path.join(REF, "ref_kat")):
+        sys.exit("build the reference first: make -C oracle ref")
+    kin = kat_inputs()
+    aug = run_kat("geoclaw", kin, "3", nout=7)
+    fwave = run_kat("geoclaw", kin, "1", nout=7)
+    np.savez_compressed(os.path.join(HERE, "kat_geoclaw.npz"), inputs=kin, aug=aug, fwave=fwave)
+    rin = rt_inputs()
+    rt = run_kat("riemanntype", rin, nout=5)
+    np.savez_compressed(os.path.join(HERE, "kat_riemanntype.npz"), inputs=rin, outputs=rt)
+    print(f"kat: {len(kin)} geoclaw records, {len(rin)} riemanntype records")
+
+    lines = subprocess.run([os.path.join(REF, "ref_tables")], check=True, capture_output=True,
+                           text=True).stdout.split("\n")
+    tab = {"leg": {}, "vol": {}, "scl": {}}
+    for line in lines:
+        if line.strip():
+            k, i, h = line.split()
+            tab[k][int(i)] = int(h, 16)
+    np.savez_compressed(os.path.join(HERE, "tables.npz"),
+                        leg=np.array([tab["leg"][i] for i in range(-1, 61)], np.uint64),
+                        vol=np.array([tab["vol"][i] for i in range(0, 61)], np.uint64))
+
+    write_run("plain_d6_n1", "samoa_swe", 6, 20, 1, 0)
+    write_run("plain_d8_n3", "samoa_swe", 8, 30, 3, 0)
+    write_run("plain_d10_n5", "samoa_swe", 10, 25, 5, 0)
+    write_run("bath_d6_n1", "samoa_swe_bath", 6, 30, 1, 1)
+    write_run("bath_d7_n2", "samoa_swe_bath", 7, 40, 2, 1)
+    write_run("bath_d9_n7", "samoa_swe_bath", 9, 35, 7, 1)
+
+
+if __name__ == "__main__":
+    main()
