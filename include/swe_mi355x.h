@@ -1,0 +1,131 @@
+/*
+ * swe_mi355x.h -- C ABI of the MI355X shallow-water (SWE) Euler step.
+ *
+ * A drop-in for the per-step edge/cell traversal of sam(oa)^2's SWE scenario:
+ * one call replaces `swe%euler%traverse(grid)` (SWE.f90:393), i.e. the
+ * operators of SWE_euler_timestep.f90:36-54 run over every cell and edge of a
+ * frozen mesh (flux = GeoClaw augmented Riemann solver on every edge, cell
+ * update with the wet/dry limiter, CFL reduction).  Plain pointers and sizes
+ * only; every entry point returns SWE_OK (0) or a negative status and never
+ * aborts.  The text of the last error of the calling thread is available from
+ * swe_last_error().
+ *
+ * Data conventions (all arrays in forward Sierpinski SFC order, as
+ * SFC_node_traversal.f90:90-145 generates them):
+ *   eta[n]     surface elevation (the reference's Q%h), metres
+ *   hu[n],hv[n] momentum in world coordinates (Q%p)
+ *   b[n]       bathymetry (Q%b)
+ *   depth[n]   refinement depth d of each cell (fine_triangle%i_depth)
+ *   plotter[n] plotter-grammar type in forward orientation, -8..8 without 0
+ *              (fine_triangle%i_plotter_type)
+ *   nbr[3n]    neighbour across edge k = 0,1,2 (reference edge index k+1,
+ *              edge i opposite node i, SFC_data_types.f90:660-666), -1 on the
+ *              domain boundary
+ *
+ * Ownership: the caller owns host arrays and may free them when a call
+ * returns; the library owns its device buffers.  A handle is not re-entrant:
+ * calls on one handle must not overlap.
+ */
+#ifndef SWE_MI355X_H
+#define SWE_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWE_OK 0
+#define SWE_ERR_ARG (-1)      /* invalid argument (null pointer, bad size, bad mesh) */
+#define SWE_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define SWE_ERR_NOMEM (-3)    /* device or host allocation failed */
+#define SWE_ERR_STATE (-4)    /* call not valid in the handle's state */
+
+#define SWE_SOLVER_AUG_RIEMANN 0 /* riemannsolvers.f:2-258 (flux_solver=aug_riemann) */
+#define SWE_SOLVER_FWAVE 1       /* riemannsolvers.f:446-500 (flux_solver=fwave) */
+
+#define SWE_BC_OUTFLOW 0 /* ghost = copy of the cell (SWE_euler_timestep.f90:248) */
+#define SWE_BC_WALL 1    /* ghost momentum mirrored, p - 2 (p.n) n */
+
+typedef struct swe_handle swe_handle;
+
+/* Mesh and scenario constants; the fields a `t_swe` run takes from `cfg`
+ * (Config.f90:243,284, SWE.f90:169-170) and from the grid. */
+typedef struct swe_mesh_desc {
+    int64_t n_cells;
+    const int8_t *depth;    /* [n_cells] */
+    const int8_t *plotter;  /* [n_cells] */
+    const int32_t *nbr;     /* [3*n_cells] */
+    double scaling;         /* cfg%scaling: edge length of the unit square in metres */
+    double g;               /* gravitational constant (SWE_data_types.f90:34: 9.80665) */
+    double dry_tolerance;   /* cfg%dry_tolerance */
+    double courant;         /* cfg%courant_number (applied by swe_run only) */
+    int32_t min_depth;      /* cfg%i_min_depth, for the refinement flags */
+    int32_t max_depth;      /* cfg%i_max_depth */
+    int32_t solver;         /* SWE_SOLVER_* */
+    int32_t boundary;       /* SWE_BC_* */
+} swe_mesh_desc;
+
+/* Creates a handle: validates and uploads the mesh topology.  Replaces the
+ * grid set-up the traversal template performs per scenario
+ * (SFC_generic_traversal_ringbuffer.f90:252-285). */
+int swe_create(const swe_mesh_desc *desc, swe_handle **out);
+
+/* Copies a host state in.  b is constant during steps. */
+int swe_upload(swe_handle *h, const double *eta, const double *hu, const double *hv,
+               const double *b);
+
+/* Copies the current state out (any pointer may be NULL to skip it). */
+int swe_download(swe_handle *h, double *eta, double *hu, double *hv, double *b);
+
+/* One Euler traversal: replaces swe%euler%traverse(grid) (SWE.f90:393).
+ *   dt       grid%r_dt after pre_traversal_grid_op (SWE_euler_timestep.f90:89-109)
+ *   forward  1 if the traversal visits cells in forward SFC order, 0 if backward
+ *            (the reference flips the direction after every traversal,
+ *            SFC_generic_traversal_ringbuffer.f90:420); decides which cell is
+ *            the left state of each edge and the order of the update sum.
+ *   dt_cfl   out: min over cells of vol / sum(len_i * s_i) (the reduced
+ *            grid%r_dt_new before the courant factor, :118)
+ *   n_refine out (nullable): number of cells flagged for refinement (:117)
+ *   refine   out (nullable): refinement flag per cell, -1/0/1 (:336-344)
+ * The host keeps t += dt and dt = courant * dt_cfl, as post_traversal_grid_op
+ * (:111-131) does.  Synchronous. */
+int swe_step(swe_handle *h, double dt, int forward, double *dt_cfl, int64_t *n_refine,
+             int8_t *refine);
+
+/* Runs n_steps Euler traversals on the device without host round trips: the
+ * dt chain of post_traversal_grid_op / pre_traversal_grid_op is evaluated on
+ * the device (dt_{k+1} = min(dt_clamp, courant * dt_cfl_k); dt_clamp <= 0
+ * disables the clamp).  The direction starts at `forward` and flips after
+ * every step.  Returns the simulated time added and the dt for the next step.
+ * The steps are captured once into a HIP graph and replayed. */
+int swe_run(swe_handle *h, int64_t n_steps, double dt0, int forward, double dt_clamp,
+            double *time_added, double *dt_next);
+
+/* Device-pointer step for callers that own device memory (the PyTorch
+ * binding): reads eta/hu/hv/b at the given device addresses, writes
+ * eta_out/hu_out/hv_out (must not alias the inputs), on `stream`
+ * (a hipStream_t, NULL = the library's stream).  dt_cfl_dev and
+ * n_refine_dev (nullable) are device scalars that receive the step's CFL
+ * minimum and refinement count; refine_dev (nullable) receives the flags.
+ * Asynchronous. */
+int swe_step_device(swe_handle *h, const double *eta, const double *hu, const double *hv,
+                    const double *b, double *eta_out, double *hu_out, double *hv_out,
+                    double dt, int forward, double *dt_cfl_dev, int64_t *n_refine_dev,
+                    int8_t *refine_dev, void *stream);
+
+/* Releases the handle and its device memory. NULL is accepted. */
+int swe_destroy(swe_handle *h);
+
+/* Copies the last error message of the calling thread into buf (always NUL
+ * terminated when size > 0); returns its full length. */
+size_t swe_last_error(char *buf, size_t size);
+
+/* Number of cells a workgroup processes (the LDS chunk), for tests/tuning. */
+int swe_chunk_cells(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,178 @@
+"""Python mirror of the SWE Euler-step operator behind the C ABI.
+
+`EulerTimestep` plays the role of the reference's
+`t_swe_euler_timestep_traversal` (SWE_euler_timestep.f90:9-54): it is created
+once per frozen mesh and `traverse()` replaces `swe%euler%traverse(grid)`
+(SWE.f90:393).  `Grid` keeps the scenario bookkeeping the reference keeps in
+`t_grid` (r_time, r_dt, traversal direction) and applies
+pre/post_traversal_grid_op (SWE_euler_timestep.f90:89-131) around each
+traversal exactly as the reference does.  Every call goes through the HIP
+library; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional
+
+import numpy as np
+
+from . import _native
+
+SOLVER_AUG_RIEMANN = 0
+SOLVER_FWAVE = 1
+BC_OUTFLOW = 0
+BC_WALL = 1
+G = 9.80665  # SWE_data_types.f90:34
+
+
+@dataclass
+class Mesh:
+    """A frozen triangle mesh in forward Sierpinski order.
+
+    depth[n] int8, plotter[n] int8 (forward plotter type), nbr[n,3] int32
+    (neighbour across reference edge k+1, -1 on the boundary), coords[n,3,2]
+    node positions in unit-square coordinates (optional)."""
+
+    depth: np.ndarray
+    plotter: np.ndarray
+    nbr: np.ndarray
+    coords: Optional[np.ndarray] = None
+
+    @property
+    def n(self) -> int:
+        return int(self.depth.shape[0])
+
+    def centroids(self) -> np.ndarray:
+        return self.coords.mean(axis=1)
+
+
+@dataclass
+class StepResult:
+    dt_cfl: float          # min over cells of vol / sum(len * s)
+    n_refine: int          # cells flagged for refinement
+    refine: Optional[np.ndarray] = None
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+class EulerTimestep:
+    """One frozen mesh on the GPU; `traverse` performs one Euler traversal."""
+
+    def __init__(self, mesh: Mesh, *, scaling: float = 1.0, g: float = G,
+                 dry_tolerance: float = 0.01, courant: float = 0.45,
+                 min_depth: Optional[int] = None, max_depth: Optional[int] = None,
+                 solver: int = SOLVER_AUG_RIEMANN, boundary: int = BC_OUTFLOW):
+        self._lib = _native.load()
+        self.mesh = mesh
+        self._depth = np.ascontiguousarray(mesh.depth, dtype=np.int8)
+        self._plotter = np.ascontiguousarray(mesh.plotter, dtype=np.int8)
+        self._nbr = np.ascontiguousarray(mesh.nbr, dtype=np.int32).reshape(-1)
+        dmax = int(self._depth.max()) if max_depth is None else int(max_depth)
+        dmin = int(self._depth.min()) if min_depth is None else int(min_depth)
+        self.desc = _native.MeshDesc(
+            n_cells=mesh.n, depth=_ptr(self._depth), plotter=_ptr(self._plotter),
+            nbr=_ptr(self._nbr), scaling=float(scaling), g=float(g),
+            dry_tolerance=float(dry_tolerance), courant=float(courant), min_depth=dmin,
+            max_depth=dmax, solver=int(solver), boundary=int(boundary))
+        h = C.c_void_p()
+        _native.check("swe_create", self._lib.swe_create(C.byref(self.desc), C.byref(h)))
+        self._h = h
+        self.n = mesh.n
+        self.courant = float(courant)
+
+    # -- state -----------------------------------------------------------
+    def upload(self, eta, hu, hv, b) -> None:
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (eta, hu, hv, b)]
+        for a in arrs:
+            if a.shape != (self.n,):
+                raise ValueError(f"state arrays must have shape ({self.n},), got {a.shape}")
+        _native.check("swe_upload", self._lib.swe_upload(self._h, *[_ptr(a) for a in arrs]))
+
+    def download(self):
+        out = [np.empty(self.n) for _ in range(4)]
+        _native.check("swe_download", self._lib.swe_download(self._h, *[_ptr(a) for a in out]))
+        return tuple(out)
+
+    # -- traversals --------------------------------------------------------
+    def traverse(self, dt: float, forward: bool, flags: bool = False) -> StepResult:
+        dtc = C.c_double()
+        nref = C.c_int64()
+        rf = np.empty(self.n, np.int8) if flags else None
+        _native.check("swe_step", self._lib.swe_step(
+            self._h, float(dt), int(bool(forward)), C.byref(dtc), C.byref(nref),
+            _ptr(rf) if flags else None))
+        return StepResult(dtc.value, nref.value, rf)
+
+    def run(self, n_steps: int, dt0: float, forward: bool, dt_clamp: float = -1.0):
+        """n_steps traversals with the dt chain on the device; returns
+        (time added, dt of the next step)."""
+        t = C.c_double()
+        dtn = C.c_double()
+        _native.check("swe_run", self._lib.swe_run(self._h, int(n_steps), float(dt0),
+                                                   int(bool(forward)), float(dt_clamp),
+                                                   C.byref(t), C.byref(dtn)))
+        return t.value, dtn.value
+
+    def step_device(self, eta, hu, hv, b, eta_out, hu_out, hv_out, dt, forward, dt_cfl,
+                    n_refine=0, refine=0, stream=0) -> None:
+        """Device-pointer traversal (integers are device addresses)."""
+        _native.check("swe_step_device", self._lib.swe_step_device(
+            self._h, eta, hu, hv, b, eta_out, hu_out, hv_out, float(dt), int(bool(forward)),
+            dt_cfl, n_refine or None, refine or None, stream or None))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.swe_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+@dataclass
+class Grid:
+    """Scenario time bookkeeping of t_grid (r_time, r_dt) and the traversal
+    direction, driven exactly as swe_run (SWE.f90:380-429) drives it."""
+
+    op: EulerTimestep
+    dt: float
+    time: float = 0.0
+    forward: bool = False
+    max_time: float = -1.0          # cfg%r_max_time (-tmax)
+    output_time_step: float = -1.0  # cfg%r_output_time_step (-tout)
+    history: list = field(default_factory=list)
+
+    def clamp(self) -> float:
+        """pre_traversal_grid_op, SWE_euler_timestep.f90:93-99."""
+        dt = self.dt
+        if self.max_time > 0.0:
+            dt = min(self.max_time, dt)
+        if self.output_time_step > 0.0:
+            dt = min(self.output_time_step, dt)
+        return dt
+
+    def euler_step(self, flags: bool = False) -> StepResult:
+        self.dt = self.clamp()
+        res = self.op.traverse(self.dt, self.forward, flags=flags)
+        # post_traversal_grid_op, :115-128
+        self.time = self.time + self.dt
+        self.dt = self.op.courant * res.dt_cfl
+        self.forward = not self.forward  # grid%reverse(), ringbuffer :420
+        self.history.append(res)
+        return res
+
+    def other_traversal(self) -> None:
+        """An output or remeshing traversal: only flips the direction."""
+        self.forward = not self.forward

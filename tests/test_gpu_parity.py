@@ -1,0 +1,172 @@
+"""HIP path vs the oracle and the reference's golden runs, through the C ABI.
+
+Integer/flag outputs and FP64 fields are compared bit for bit: the library
+and the oracle round every operation as the reference does."""
+import numpy as np
+import pytest
+
+import golden_io
+import pyoracle as po
+from samoa_amd import swe
+
+pytestmark = pytest.mark.gpu
+
+
+def make_mesh(d):
+    plotter, co, nbr = po.mesh_uniform(d)
+    return swe.Mesh(np.full(plotter.shape[0], d, np.int8), plotter, nbr, co)
+
+
+def random_state(mesh, seed, S, dry_fraction=0.3):
+    rng = np.random.default_rng(seed)
+    cen = mesh.centroids()
+    x, y = cen[:, 0], cen[:, 1]
+    b = -1.0 + 1.5 * x + 0.3 * np.sin(7 * y) + 0.05 * rng.standard_normal(mesh.n)
+    eta = np.where(np.hypot(x - 0.3, y - 0.5) < 0.15, 0.8, 0.1) + 0.01 * rng.standard_normal(mesh.n)
+    dry = b > eta
+    eta = np.where(dry, np.minimum(b, 0.0), eta)
+    h = np.maximum(eta - b, 0)
+    hu = h * rng.normal(0, 0.5, mesh.n)
+    hv = h * rng.normal(0, 0.5, mesh.n)
+    hu[dry] = 0
+    hv[dry] = 0
+    return eta * S / 10, hu * S / 10, hv * S / 10, b * S / 10
+
+
+@pytest.mark.parametrize("name", [r[0] for r in golden_io.runs()])
+def test_golden_run_bit_exact(name):
+    _, z, meta = {r[0]: r for r in golden_io.runs()}[name]
+    d = meta["depth"]
+    mesh = make_mesh(d)
+    op = swe.EulerTimestep(mesh, scaling=meta["scaling"], dry_tolerance=meta["drytol"],
+                           courant=meta["courant"], boundary=meta["bc"])
+    st = z["states"][0]
+    op.upload(*st)
+    depth = mesh.depth
+    dt = po.initial_dt(depth, st, meta["courant"])
+    t = 0.0
+    sched = golden_io.direction_schedule(meta["steps"], meta["nout"])
+    oi = 1
+    for k in range(1, meta["steps"] + 1):
+        res = op.traverse(dt, sched[k - 1], flags=True)
+        t += dt
+        dt = meta["courant"] * res.dt_cfl
+        if k % meta["nout"] == 0:
+            eta, hu, hv, b = op.download()
+            assert z["times"][oi] == t
+            assert np.array_equal(np.stack([eta, hu, hv, b]), z["states"][oi]), f"step {k}"
+            assert np.array_equal(res.refine, z["refine"][oi])
+            oi += 1
+    op.close()
+
+
+@pytest.mark.parametrize("d", [1, 4, 9, 12, 15])
+@pytest.mark.parametrize("solver", [swe.SOLVER_AUG_RIEMANN, swe.SOLVER_FWAVE])
+@pytest.mark.parametrize("bc", [swe.BC_OUTFLOW, swe.BC_WALL])
+def test_random_wetdry_vs_oracle(d, solver, bc):
+    mesh = make_mesh(d)
+    S = 1000.0
+    st = random_state(mesh, seed=d * 7 + solver + 3 * bc, S=S)
+    drytol = 1e-3
+    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=drytol, courant=0.45,
+                           min_depth=max(d - 1, 0), max_depth=d + 1, solver=solver, boundary=bc)
+    op.upload(*st)
+    ref = [a.copy() for a in st]
+    dt = po.initial_dt(mesh.depth, ref, 0.45) * S
+    for k, fwd in enumerate([True, False, False, True, True]):
+        res = op.traverse(dt, fwd, flags=True)
+        e, u, v, rf, cand, nref = po.step(mesh.depth, mesh.plotter, mesh.nbr, ref, dt, scaling=S,
+                                          drytol=drytol, dmin=max(d - 1, 0), dmax=d + 1, bc=bc,
+                                          solver=solver, forward=fwd)
+        ref = [e, u, v, ref[3]]
+        got = op.download()
+        assert res.dt_cfl == cand, f"step {k}: dt_cfl {res.dt_cfl!r} != {cand!r}"
+        assert res.n_refine == nref
+        assert np.array_equal(res.refine, rf)
+        for name, a, b_ in zip("eta hu hv b".split(), got, ref):
+            bad = np.flatnonzero(a != b_)
+            assert bad.size == 0, f"step {k} {name}: {bad.size} cells differ, first {bad[:5]}"
+        dt = 0.45 * cand
+    op.close()
+
+
+def test_device_run_matches_host_chain():
+    d, S = 13, 1.0e5
+    mesh = make_mesh(d)
+    st = random_state(mesh, seed=11, S=S)
+    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3, courant=0.45)
+    op.upload(*st)
+    dt0 = po.initial_dt(mesh.depth, st, 0.45) * S
+    t_added, dt_next = op.run(7, dt0, forward=False)
+    ref = [a.copy() for a in st]
+    dt, t, fwd = dt0, 0.0, False
+    for _ in range(7):
+        e, u, v, _, cand, _ = po.step(mesh.depth, mesh.plotter, mesh.nbr, ref, dt, scaling=S,
+                                      drytol=1e-3, forward=fwd)
+        ref = [e, u, v, ref[3]]
+        t += dt
+        dt = 0.45 * cand
+        fwd = not fwd
+    got = op.download()
+    assert t_added == t and dt_next == dt
+    for a, b_ in zip(got, ref):
+        assert np.array_equal(a, b_)
+    # a second run continues from the same buffers and graph
+    t2, dt2 = op.run(4, dt_next, forward=fwd)
+    for _ in range(4):
+        e, u, v, _, cand, _ = po.step(mesh.depth, mesh.plotter, mesh.nbr, ref, dt, scaling=S,
+                                      drytol=1e-3, forward=fwd)
+        ref = [e, u, v, ref[3]]
+        dt = 0.45 * cand
+        fwd = not fwd
+    got = op.download()
+    assert dt2 == dt and all(np.array_equal(a, b_) for a, b_ in zip(got, ref))
+    op.close()
+
+
+def test_step_device_pointers_torch():
+    torch = pytest.importorskip("torch")
+    d, S = 10, 50.0
+    mesh = make_mesh(d)
+    st = random_state(mesh, seed=5, S=S)
+    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3)
+    dev = torch.device("cuda:0")
+    q = [torch.tensor(a, dtype=torch.float64, device=dev) for a in st]
+    out = [torch.empty_like(q[0]) for _ in range(3)]
+    dtc = torch.empty(1, dtype=torch.float64, device=dev)
+    dt = po.initial_dt(mesh.depth, st, 0.45) * S
+    stream = torch.cuda.current_stream().cuda_stream
+    op.step_device(*[t.data_ptr() for t in q], *[t.data_ptr() for t in out], dt, True,
+                   dtc.data_ptr(), stream=stream)
+    torch.cuda.synchronize()
+    e, u, v, _, cand, _ = po.step(mesh.depth, mesh.plotter, mesh.nbr, st, dt, scaling=S,
+                                  drytol=1e-3, forward=True)
+    assert dtc.item() == cand
+    for t_, r in zip(out, (e, u, v)):
+        assert np.array_equal(t_.cpu().numpy(), r)
+    op.close()
+
+
+def test_large_mesh_lake_at_rest_and_mass():
+    """d=20 (2M cells): lake at rest stays at rest to round-off; with outflow
+    boundaries and an all-wet interior wave, mass changes only through the
+    boundary (none while the wave is inside)."""
+    d, S = 20, 1.0e6
+    mesh = make_mesh(d)
+    cen = mesh.centroids()
+    b = -4000.0 + 800.0 * np.sin(9 * cen[:, 0]) * np.cos(7 * cen[:, 1])
+    eta = np.zeros(mesh.n)
+    z = np.zeros(mesh.n)
+    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3)
+    op.upload(eta, z, z, b)
+    op.run(6, 1.0, forward=True)
+    e, u, v, _ = op.download()
+    assert np.abs(e).max() < 1e-9 and np.abs(u).max() < 1e-6 and np.abs(v).max() < 1e-6
+    eta0 = 1.0 * np.exp(-((cen[:, 0] - 0.5) ** 2 + (cen[:, 1] - 0.5) ** 2) / 0.01)
+    op.upload(eta0, z, z, b)
+    dt0 = po.initial_dt(mesh.depth, [eta0, z, z, b], 0.45) * S
+    op.run(10, dt0, forward=False)
+    e, _, _, _ = op.download()
+    vol = (S * S) * 0.5 ** (d + 1)
+    assert abs((e - b).sum() * vol - (eta0 - b).sum() * vol) <= 1e-12 * abs((eta0 - b).sum() * vol)
+    op.close()
