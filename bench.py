@@ -1,0 +1,171 @@
+"""Benchmark of the MI355X SWE Euler step.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload tsunami]
+                    [--no-cpu] [--calibrate-copy]
+
+Workload (one GPU): the adaptive single-tsunami configuration on a frozen
+uniform Sierpinski mesh of depth 21 (4,194,304 cells, inside the 2-5M cell
+range of that configuration; remeshing is not part of the timed step):
+1000 km square, b = -4000 m + Gaussian random field (std 800 m, l = 50 km)
++ a 150 km shelf rising to +200 m, elliptical Gaussian source, dry tolerance
+1e-3 m, courant 0.45, augmented Riemann solver, FP64, 1000 steps by default.
+
+A step is one Euler traversal (flux on every edge, cell update with the
+wet/dry limiter, CFL reduction and the device-side dt update).  The K timed
+steps run back to back from a captured HIP graph, inputs resident in HBM.
+Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "cell-updates/s/GPU plain vs batched ensemble; adjoint/forward cost ratio"
+HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
+BYTES_PER_CELL = 56         # read eta, hu, hv, b; write eta, hu, hv (FP64)
+PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--workload", default="tsunami", choices=["tsunami"])
+    p.add_argument("--depth", type=int, default=21)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="also run a 1 GiB device copy (counter calibration under rocprofv3)")
+    return p.parse_args()
+
+
+def cpu_baseline(mesh, state, dt, seconds_target=15.0):
+    """The oracle's OpenMP restatement of the same step on this host's cores,
+    on the same mesh and state, for a bounded number of steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+
+    po.build()
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    cores = int(os.environ["OMP_NUM_THREADS"])
+    st = [a.copy() for a in state]
+    steps, t_used, fwd = 0, 0.0, False
+    while t_used < seconds_target and steps < 50:
+        t0 = time.perf_counter()
+        e, u, v, _, cand, _ = po.step(mesh.depth, mesh.plotter, mesh.nbr, st, dt, scaling=1.0e6,
+                                      drytol=1e-3, forward=fwd, threads=True)
+        t_used += time.perf_counter() - t0
+        st = [e, u, v, st[3]]
+        dt = 0.45 * cand
+        fwd = not fwd
+        steps += 1
+    return {"value": mesh.n * steps / t_used, "unit": "cell-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{steps} steps of the same {mesh.n}-cell mesh and state "
+                      f"(oracle/swe_oracle.c swo_step_omp, {t_used:.1f} s)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not run "
+              "(the multi-GPU sharding is a design note, see DESIGN.md)", file=sys.stderr)
+        sys.exit(1)
+
+    import numpy as np
+    import torch
+
+    from samoa_amd import host, swe
+
+    d = args.depth
+    mesh = host.mesh_uniform(d)
+    state = host.tsunami(mesh, seed=0, drytol=1e-3)
+    S, courant = 1.0e6, 0.45
+    dt0 = host.initial_dt(mesh, state, courant)  # unit-square units (reference quirk)
+
+    dev = torch.device("cuda:0")
+    torch.cuda.init()
+    stream = torch.cuda.Stream(device=dev)
+    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3, courant=courant,
+                           solver=swe.SOLVER_AUG_RIEMANN, boundary=swe.BC_OUTFLOW)
+    op.upload(*state)
+    op.set_stream(stream.cuda_stream)
+
+    # warmup: first steps of the run (also captures the graph)
+    t_added, dt = op.run(args.warmup, dt0, forward=False)
+    fwd = args.warmup % 2 == 1
+
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(stream):
+        start.record(stream)
+        t_added2, dt_next = op.run(args.steps, dt, forward=fwd)
+        end.record(stream)
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(end)
+    ms_per_step = ms / args.steps
+    cells = mesh.n
+    value = cells * args.steps / (ms * 1e-3)
+
+    eta, hu, hv, b = op.download()
+    if not (np.isfinite(eta).all() and np.isfinite(hu).all()):
+        raise RuntimeError("non-finite state after the timed steps")
+
+    achieved = BYTES_PER_CELL * cells / (ms_per_step * 1e-3)
+    traffic = None
+    if os.path.exists(PROFILE_TRAFFIC):
+        prof = json.load(open(PROFILE_TRAFFIC))
+        key = f"euler_step_tsunami_d{d}"
+        if key in prof:
+            traffic = prof[key]["hbm_bytes_per_launch"]
+
+    if args.calibrate_copy:
+        a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        bb = torch.empty_like(a)
+        for _ in range(3):
+            bb.copy_(a)
+        torch.cuda.synchronize()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"tsunami_frozen_uniform_d{d}",
+            "cells": cells, "depth": d, "scaling_m": S, "solver": "aug_riemann",
+            "boundary": "outflow", "dry_tolerance": 1e-3, "courant": courant,
+            "simulated_time_s": t_added + t_added2,
+        },
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK, "traffic": traffic,
+            "kernel": "swe::euler_step_kernel<0>",
+            "bytes_per_cell_update": BYTES_PER_CELL,
+        },
+    }
+    out["roofline"]["achieved"] = achieved / 1e9
+    out["roofline"]["peak"] = HBM_PEAK / 1e9
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mesh, state, dt0 * S)
+    op.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

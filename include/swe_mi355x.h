@@ -109,11 +109,16 @@ int swe_run(swe_handle *h, int64_t n_steps, double dt0, int forward, double dt_c
  * (a hipStream_t, NULL = the library's stream).  dt_cfl_dev and
  * n_refine_dev (nullable) are device scalars that receive the step's CFL
  * minimum and refinement count; refine_dev (nullable) receives the flags.
- * Asynchronous. */
+ * Asynchronous; calls on one handle must not overlap, whatever their streams. */
 int swe_step_device(swe_handle *h, const double *eta, const double *hu, const double *hv,
                     const double *b, double *eta_out, double *hu_out, double *hv_out,
                     double dt, int forward, double *dt_cfl_dev, int64_t *n_refine_dev,
                     int8_t *refine_dev, void *stream);
+
+/* Launch subsequent work of this handle on `stream` (a hipStream_t; NULL
+ * restores the library's own stream), e.g. to time it with events the caller
+ * records on that stream.  Waits for work already queued. */
+int swe_set_stream(swe_handle *h, void *stream);
 
 /* Releases the handle and its device memory. NULL is accepted. */
 int swe_destroy(swe_handle *h);

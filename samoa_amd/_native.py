@@ -54,6 +54,7 @@ SIGNATURES = {
     "swe_step": (_I, [_P, _D, _I, C.POINTER(_D), C.POINTER(_I64), _P]),
     "swe_run": (_I, [_P, _I64, _D, _I, _D, C.POINTER(_D), C.POINTER(_D)]),
     "swe_step_device": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P]),
+    "swe_set_stream": (_I, [_P, _P]),
     "swe_destroy": (_I, [_P]),
     "swe_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
     "swe_chunk_cells": (_I, []),

@@ -3,6 +3,9 @@
     python samoa_amd/build.py
 
   samoa_amd/lib/libswe_mi355x.so  HIP kernels + C ABI (hipcc, gfx950)
+  samoa_amd/lib/libswe_host.so    Fortran host: ISO_C_BINDING layer, SFC mesh,
+                                  scenarios, time loop (amdflang)
+  samoa_amd/bin/swe_mi355x        Fortran command-line driver
 """
 from __future__ import annotations
 
@@ -41,8 +44,40 @@ def build_hip(verbose: bool = False) -> str:
     return out
 
 
+FORTRAN_SOURCES = ["swe_mi355x.f90", "swe_sfc_mesh.f90", "swe_scenarios.f90", "swe_driver.f90"]
+FFLAGS = ["-O2", "-fPIC"]
+
+
+def build_fortran() -> None:
+    import tempfile
+
+    fdir = os.path.join(HERE, "fortran")
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    with tempfile.TemporaryDirectory() as mod:
+        objs = []
+        for src in FORTRAN_SOURCES:
+            obj = os.path.join(mod, src.replace(".f90", ".o"))
+            subprocess.run(["amdflang", *FFLAGS, "-module-dir", mod, "-c",
+                            os.path.join(fdir, src), "-o", obj], check=True)
+            objs.append(obj)
+        rpath = "-Wl,-rpath,$ORIGIN"
+        lib = os.path.join(LIB, "libswe_host.so")
+        subprocess.run(["amdflang", "-shared", *objs, "-o", lib + ".tmp", f"-L{LIB}",
+                        "-lswe_mi355x", rpath], check=True)
+        os.replace(lib + ".tmp", lib)
+        main_obj = os.path.join(mod, "main.o")
+        subprocess.run(["amdflang", *FFLAGS, "-module-dir", mod, "-c",
+                        os.path.join(fdir, "swe_mi355x_main.f90"), "-o", main_obj], check=True)
+        exe = os.path.join(bindir, "swe_mi355x")
+        subprocess.run(["amdflang", main_obj, *objs, "-o", exe + ".tmp", f"-L{LIB}",
+                        "-lswe_mi355x", "-Wl,-rpath,$ORIGIN/../lib"], check=True)
+        os.replace(exe + ".tmp", exe)
+
+
 def build_all(verbose: bool = False) -> None:
     build_hip(verbose)
+    build_fortran()
 
 
 if __name__ == "__main__":

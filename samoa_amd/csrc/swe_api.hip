@@ -123,13 +123,13 @@ struct swe_handle {
     int64_t n = 0;
     swe_mesh_desc desc{};
     double refine_thr = 0.0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;         // where work is launched (caller may replace it)
+    hipStream_t own_stream = nullptr;     // created by the library; used for graph capture
     double *eta[2] = {nullptr, nullptr}, *hu[2] = {nullptr, nullptr}, *hv[2] = {nullptr, nullptr};
     double *b = nullptr;
     int32_t *nbr = nullptr;
     int8_t *plotter = nullptr, *depth = nullptr, *refine = nullptr;
-    // device scalars: [0] dtmin bits, [1] nref, [2] dt, [3] time
-    unsigned long long *scalars = nullptr;
+    swe::Slots *slots = nullptr;  // reduction slots and device dt chain
     int cur = 0;
     bool has_state = false;
     GraphCache graph;
@@ -151,8 +151,8 @@ void release(swe_handle *h) {
     (void)hipFree(h->plotter);
     (void)hipFree(h->depth);
     (void)hipFree(h->refine);
-    (void)hipFree(h->scalars);
-    if (h->stream) (void)hipStreamDestroy(h->stream);
+    (void)hipFree(h->slots);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     delete h;
 }
 
@@ -180,6 +180,8 @@ swe::StepArgs make_args(const swe_handle *h, const double *eta, const double *hu
     a.dmax = h->desc.max_depth;
     a.bc = h->desc.boundary;
     a.forward = forward ? 1 : 0;
+    a.courant = h->desc.courant;
+    a.slots = h->slots;
     return a;
 }
 
@@ -266,7 +268,8 @@ int swe_create(const swe_mesh_desc *desc, swe_handle **out) {
         return e != hipSuccess ? fail(SWE_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e)) : rc;
     }
     const size_t bytes = sizeof(double) * (size_t)n;
-    bool ok = hipStreamCreate(&h->stream) == hipSuccess;
+    bool ok = hipStreamCreate(&h->own_stream) == hipSuccess;
+    h->stream = h->own_stream;
     for (int i = 0; i < 2 && ok; ++i)
         ok = hipMalloc(&h->eta[i], bytes) == hipSuccess && hipMalloc(&h->hu[i], bytes) == hipSuccess &&
              hipMalloc(&h->hv[i], bytes) == hipSuccess;
@@ -274,7 +277,7 @@ int swe_create(const swe_mesh_desc *desc, swe_handle **out) {
          hipMalloc(&h->nbr, sizeof(int32_t) * 3 * (size_t)n) == hipSuccess &&
          hipMalloc(&h->plotter, (size_t)n) == hipSuccess && hipMalloc(&h->depth, (size_t)n) == hipSuccess &&
          hipMalloc(&h->refine, (size_t)n) == hipSuccess &&
-         hipMalloc(&h->scalars, 4 * sizeof(unsigned long long)) == hipSuccess;
+         hipMalloc(&h->slots, sizeof(swe::Slots)) == hipSuccess;
     if (!ok) {
         release(h);
         return fail(SWE_ERR_NOMEM, "swe_create: device allocation of %lld cells failed", (long long)n);
@@ -282,6 +285,10 @@ int swe_create(const swe_mesh_desc *desc, swe_handle **out) {
     ok = hipMemcpy(h->nbr, packed.data(), sizeof(int32_t) * 3 * (size_t)n, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(h->plotter, desc->plotter, (size_t)n, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(h->depth, desc->depth, (size_t)n, hipMemcpyHostToDevice) == hipSuccess;
+    swe::Slots init{};
+    const double huge_sr = 1.7976931348623157e308;
+    std::memcpy(&init.dtmin, &huge_sr, sizeof huge_sr);
+    ok = ok && hipMemcpy(h->slots, &init, sizeof init, hipMemcpyHostToDevice) == hipSuccess;
     if (!ok) {
         release(h);
         return fail(SWE_ERR_HIP, "swe_create: topology upload failed");
@@ -330,19 +337,14 @@ int swe_step(swe_handle *h, double dt, int forward, double *dt_cfl, int64_t *n_r
     swe::StepArgs a = make_args(h, h->eta[c], h->hu[c], h->hv[c], h->b, h->eta[o], h->hu[o],
                                 h->hv[o], dt, forward);
     a.refine = refine ? h->refine : nullptr;
-    a.dtmin = &h->scalars[0];
-    a.nref = &h->scalars[1];
-    HIP_TRY(swe::launch_reset(a.dtmin, a.nref, h->stream));
     HIP_TRY(swe::launch_step(a, h->desc.solver, h->stream));
-    unsigned long long res[2];
-    HIP_TRY(hipMemcpyAsync(res, h->scalars, sizeof res, hipMemcpyDeviceToHost, h->stream));
+    swe::Slots res;
+    HIP_TRY(hipMemcpyAsync(&res, h->slots, sizeof res, hipMemcpyDeviceToHost, h->stream));
     if (refine)
         HIP_TRY(hipMemcpyAsync(refine, h->refine, (size_t)h->n, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    double m;
-    std::memcpy(&m, &res[0], sizeof m);
-    *dt_cfl = m;
-    if (n_refine) *n_refine = (int64_t)res[1];
+    *dt_cfl = res.res_dtcfl;
+    if (n_refine) *n_refine = (int64_t)res.res_nref;
     h->cur = o;
     return SWE_OK;
 }
@@ -351,44 +353,41 @@ int swe_run(swe_handle *h, int64_t n_steps, double dt0, int forward, double dt_c
             double *time_added, double *dt_next) {
     if (!h || n_steps < 0) return fail(SWE_ERR_ARG, "swe_run: invalid argument");
     if (!h->has_state) return fail(SWE_ERR_STATE, "swe_run: no state uploaded");
-    unsigned long long *dtmin = &h->scalars[0];
-    double *dt_dev = reinterpret_cast<double *>(&h->scalars[2]);
-    double *time_dev = reinterpret_cast<double *>(&h->scalars[3]);
-    const double init[2] = {dt0, 0.0};
-    HIP_TRY(hipMemcpyAsync(dt_dev, init, sizeof init, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(swe::launch_reset(dtmin, nullptr, h->stream));
+    // the chain starts from dt0 and zero time; kernels then carry it
+    double init[2] = {dt0, 0.0};
+    if (dt_clamp > 0.0 && dt_clamp < dt0) init[0] = dt_clamp;
+    HIP_TRY(hipMemcpyAsync(&h->slots->dt, init, sizeof init, hipMemcpyHostToDevice, h->stream));
 
-    swe::FinalizeArgs f{dt_dev, time_dev, dtmin, h->desc.courant, dt_clamp};
-    auto one_step = [&](int from, int fwd) -> hipError_t {
+    auto one_step = [&](hipStream_t s, int from, int fwd) -> hipError_t {
         const int to = 1 - from;
         swe::StepArgs a = make_args(h, h->eta[from], h->hu[from], h->hv[from], h->b, h->eta[to],
                                     h->hu[to], h->hv[to], 0.0, fwd);
-        a.dt_dev = dt_dev;
-        a.dtmin = dtmin;
-        hipError_t e = swe::launch_step(a, h->desc.solver, h->stream);
-        if (e != hipSuccess) return e;
-        return swe::launch_finalize(f, h->stream);
+        a.device_dt = 1;
+        a.dt_clamp = dt_clamp;
+        return swe::launch_step(a, h->desc.solver, s);
     };
 
     int fwd = forward ? 1 : 0;
-    int64_t pairs = n_steps / 2;
+    const int64_t pairs = n_steps / 2;
     if (pairs > 0) {
         GraphCache &gc = h->graph;
         if (!gc.exec || gc.cur != h->cur || gc.forward != fwd || gc.dt_clamp != dt_clamp) {
             if (gc.exec) {
-                hipGraphExecDestroy(gc.exec);
+                (void)hipGraphExecDestroy(gc.exec);
                 gc.exec = nullptr;
             }
+            // capture on the library's own stream, replay on h->stream
+            HIP_TRY(hipStreamSynchronize(h->stream));
             hipGraph_t graph;
-            HIP_TRY(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e1 = one_step(h->cur, fwd);
-            hipError_t e2 = e1 == hipSuccess ? one_step(1 - h->cur, 1 - fwd) : e1;
-            hipError_t e3 = hipStreamEndCapture(h->stream, &graph);
+            HIP_TRY(hipStreamBeginCapture(h->own_stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e1 = one_step(h->own_stream, h->cur, fwd);
+            hipError_t e2 = e1 == hipSuccess ? one_step(h->own_stream, 1 - h->cur, 1 - fwd) : e1;
+            hipError_t e3 = hipStreamEndCapture(h->own_stream, &graph);
             if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
                 return fail(SWE_ERR_HIP, "swe_run: graph capture failed: %s",
                             hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3)));
             hipError_t e4 = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
-            hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             if (e4 != hipSuccess) return fail(SWE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e4));
             gc.cur = h->cur;
             gc.forward = fwd;
@@ -397,14 +396,21 @@ int swe_run(swe_handle *h, int64_t n_steps, double dt0, int forward, double dt_c
         for (int64_t i = 0; i < pairs; ++i) HIP_TRY(hipGraphLaunch(gc.exec, h->stream));
     }
     if (n_steps % 2) {
-        HIP_TRY(one_step(h->cur, fwd));
+        HIP_TRY(one_step(h->stream, h->cur, fwd));
         h->cur = 1 - h->cur;
     }
     double res[2];
-    HIP_TRY(hipMemcpyAsync(res, dt_dev, sizeof res, hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(res, &h->slots->dt, sizeof res, hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
     if (dt_next) *dt_next = res[0];
     if (time_added) *time_added = res[1];
+    return SWE_OK;
+}
+
+int swe_set_stream(swe_handle *h, void *stream) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_set_stream: null handle");
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->stream = stream ? (hipStream_t)stream : h->own_stream;
     return SWE_OK;
 }
 
@@ -419,9 +425,8 @@ int swe_step_device(swe_handle *h, const double *eta, const double *hu, const do
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
     swe::StepArgs a = make_args(h, eta, hu, hv, b, eta_out, hu_out, hv_out, dt, forward);
     a.refine = refine_dev;
-    a.dtmin = reinterpret_cast<unsigned long long *>(dt_cfl_dev);
-    a.nref = reinterpret_cast<unsigned long long *>(n_refine_dev);
-    HIP_TRY(swe::launch_reset(a.dtmin, a.nref, s));
+    a.out_dtcfl = dt_cfl_dev;
+    a.out_nref = n_refine_dev;
     HIP_TRY(swe::launch_step(a, h->desc.solver, s));
     return SWE_OK;
 }

@@ -9,6 +9,20 @@ namespace swe {
 
 constexpr int CHUNK = 256;  // cells per workgroup (four wave64s)
 
+// Reduction slots of one step.  Invariant between launches: dtmin holds the
+// bits of huge(1.0), counter and nref are 0.  The last workgroup of a step
+// restores the invariant after publishing the results.
+struct Slots {
+    unsigned long long dtmin;    // running min of the CFL candidates (double bits)
+    unsigned long long nref;     // running count of refinement flags
+    unsigned int counter;        // workgroups done
+    unsigned int pad;
+    double dt;                   // dt of the next step (device dt chain)
+    double time;                 // accumulated simulated time (device dt chain)
+    double res_dtcfl;            // published CFL minimum of the last step
+    unsigned long long res_nref; // published refinement count of the last step
+};
+
 struct StepArgs {
     const double *eta, *hu, *hv, *b;       // state at t_n, SoA, SFC order
     double *eta_out, *hu_out, *hv_out;     // state at t_{n+1}
@@ -16,25 +30,18 @@ struct StepArgs {
     const int8_t *plotter;                 // [n] forward plotter type
     const int8_t *depth;                   // [n]
     int8_t *refine;                        // [n] or null
-    unsigned long long *dtmin;             // device min slot (bit pattern of a double)
-    unsigned long long *nref;              // device refinement counter or null
-    const double *dt_dev;                  // if set, dt is read from here
+    Slots *slots;
+    double *out_dtcfl;                     // extra copy of the CFL minimum, or null
+    int64_t *out_nref;                     // extra copy of the refinement count, or null
     int64_t n;
-    double dt;
+    double dt;                             // used when device_dt == 0
     double scaling, g, drytol, refine_thr;
+    double courant, dt_clamp;              // device dt chain: dt' = min(clamp, courant * min)
     int dmin, dmax, bc, forward;
-};
-
-struct FinalizeArgs {
-    double *dt;                  // in: dt used by the step; out: dt of the next step
-    double *time;                // accumulated simulated time
-    unsigned long long *dtmin;   // reduced CFL minimum; reset afterwards
-    double courant, dt_clamp;
+    int device_dt;                         // 1: read dt from slots->dt and write the next one
 };
 
 hipError_t upload_tables(const double *normal, const double *leg);
 hipError_t launch_step(const StepArgs &a, int solver, hipStream_t s);
-hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t s);
-hipError_t launch_reset(unsigned long long *dtmin, unsigned long long *nref, hipStream_t s);
 
 }  // namespace swe

@@ -71,7 +71,7 @@ __global__ __launch_bounds__(CHUNK) void euler_step_kernel(StepArgs a) {
     const int cnt = (int)((a.n - base) < CHUNK ? (a.n - base) : CHUNK);
     const bool active = t < cnt;
     const int64_t c = base + t;
-    const double dt = a.dt_dev ? *a.dt_dev : a.dt;
+    const double dt = a.device_dt ? a.slots->dt : a.dt;
 
     int32_t nb[3] = {-1, -1, -1};
     int8_t plot = 1;
@@ -256,26 +256,35 @@ __global__ __launch_bounds__(CHUNK) void euler_step_kernel(StepArgs a) {
             m = dmin2(m, s_wmin[w]);
             rsum += s_wref[w];
         }
+        Slots *sl = a.slots;
         // non-negative doubles order like their bit patterns
-        atomicMin((unsigned long long *)a.dtmin, (unsigned long long)__double_as_longlong(m));
-        if (rsum && a.nref) atomicAdd((unsigned long long *)a.nref, (unsigned long long)rsum);
+        atomicMin(&sl->dtmin, (unsigned long long)__double_as_longlong(m));
+        if (rsum) atomicAdd(&sl->nref, (unsigned long long)rsum);
+        __threadfence();  // publish this workgroup's contribution before counting it
+        const unsigned int done = atomicAdd(&sl->counter, 1u);
+        if (done == (unsigned int)(nchunks - 1)) {
+            // last workgroup of the step: every contribution is visible
+            __threadfence();
+            const unsigned long long bits = atomicAdd(&sl->dtmin, 0ull);
+            const unsigned long long nr = atomicAdd(&sl->nref, 0ull);
+            const double mn = __longlong_as_double((long long)bits);
+            sl->res_dtcfl = mn;
+            sl->res_nref = nr;
+            if (a.out_dtcfl) *a.out_dtcfl = mn;
+            if (a.out_nref) *a.out_nref = (int64_t)nr;
+            if (a.device_dt) {
+                // post_traversal_grid_op :115-128 and the next pre_traversal_grid_op :93-99
+                sl->time = sl->time + dt;
+                double next = a.courant * mn;
+                if (a.dt_clamp > 0.0) next = dmin2(a.dt_clamp, next);
+                sl->dt = next;
+            }
+            atomicExch(&sl->dtmin, (unsigned long long)__double_as_longlong(1.7976931348623157e308));
+            atomicExch(&sl->nref, 0ull);
+            atomicExch(&sl->counter, 0u);
+            __threadfence();
+        }
     }
-}
-
-// dt chain of post_traversal_grid_op / pre_traversal_grid_op on the device.
-__global__ void finalize_dt_kernel(FinalizeArgs f) {
-    const double used = *f.dt;
-    const double m = __longlong_as_double((long long)*f.dtmin);
-    double next = f.courant * m;
-    if (f.dt_clamp > 0.0) next = dmin2(f.dt_clamp, next);
-    *f.time = *f.time + used;
-    *f.dt = next;
-    *f.dtmin = (unsigned long long)__double_as_longlong(1.7976931348623157e308);
-}
-
-__global__ void reset_slots_kernel(unsigned long long *dtmin, unsigned long long *nref) {
-    *dtmin = (unsigned long long)__double_as_longlong(1.7976931348623157e308);
-    if (nref) *nref = 0ull;
 }
 
 hipError_t upload_tables(const double *normal, const double *leg) {
@@ -291,16 +300,6 @@ hipError_t launch_step(const StepArgs &a, int solver, hipStream_t s) {
         hipLaunchKernelGGL(euler_step_kernel<1>, dim3((unsigned)blocks), dim3(CHUNK), 0, s, a);
     else
         hipLaunchKernelGGL(euler_step_kernel<0>, dim3((unsigned)blocks), dim3(CHUNK), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_finalize(const FinalizeArgs &f, hipStream_t s) {
-    hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(1), 0, s, f);
-    return hipGetLastError();
-}
-
-hipError_t launch_reset(unsigned long long *dtmin, unsigned long long *nref, hipStream_t s) {
-    hipLaunchKernelGGL(reset_slots_kernel, dim3(1), dim3(1), 0, s, dtmin, nref);
     return hipGetLastError();
 }
 

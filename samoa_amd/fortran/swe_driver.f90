@@ -1,0 +1,197 @@
+! swe_driver.f90 -- the Fortran host: scenario set-up and the SWE time loop
+! over the HIP C ABI.
+!
+! `swe_time_loop` is the main loop of swe_run (SWE.f90:380-429) for a frozen
+! mesh: it applies pre_traversal_grid_op's clamps (SWE_euler_timestep.f90:93-99),
+! calls `swe_step` where the reference calls swe%euler%traverse(grid), then
+! post_traversal_grid_op's bookkeeping (:115-128: t += dt, dt = courant *
+! dt_new) and flips the traversal direction (SFC_generic_traversal_ringbuffer
+! .f90:420).  Without per-step host work the loop is handed to `swe_run`,
+! which evaluates the same dt chain on the device.
+!
+! The bind(c) wrappers export the host routines to the Python harness.
+module swe_driver
+    use iso_c_binding
+    use swe_mi355x
+    use swe_sfc_mesh
+    use swe_scenarios
+    implicit none
+    private
+    public :: swe_time_loop, initial_dt, leg_size
+
+    real(c_double), parameter :: HUGE_SR = huge(1.0_c_double)
+
+contains
+
+    !> sqrt(0.5)**k as the reference's parameter tables hold it
+    !> (SFC_data_types.f90:557).
+    pure real(c_double) function leg_size(k)
+        integer, intent(in) :: k
+        integer :: e
+        real(c_double) :: r, base
+        e = abs(k)
+        r = 1.0_c_double
+        base = sqrt(0.5_c_double)
+        do while (e > 0)
+            if (iand(e, 1) == 1) r = r * base
+            base = base * base
+            e = ishft(e, -1)
+        end do
+        if (k < 0) r = 1.0_c_double / r
+        leg_size = r
+    end function
+
+    !> Initial time step of the init traversal (SWE_initialize.f90:327-335,
+    !> :240): courant * min over cells of vol / (sum(len) * max wave speed),
+    !> in unit-square units (the reference does not apply cfg%scaling here).
+    real(c_double) function initial_dt(n, depth, eta, hu, hv, b, g, courant)
+        integer(c_int64_t), intent(in) :: n
+        integer(c_int8_t), intent(in)  :: depth(n)
+        real(c_double), intent(in)     :: eta(n), hu(n), hv(n), b(n), g, courant
+        integer(c_int64_t) :: i
+        real(c_double) :: h, s, dmin_, vol, sumlen
+        integer :: d
+        dmin_ = HUGE_SR
+        do i = 1, n
+            h = eta(i) - b(i)
+            if (h > 0.0_c_double) then
+                s = sqrt(g * h) + sqrt((hu(i) * hu(i) + hv(i) * hv(i)) / (h * h))
+            else
+                s = 0.0_c_double
+            end if
+            d = depth(i)
+            vol = 0.5_c_double ** (d + 1)
+            sumlen = (leg_size(d) + leg_size(d - 1)) + leg_size(d)
+            dmin_ = min(dmin_, vol / (sumlen * s))
+        end do
+        initial_dt = courant * dmin_
+    end function
+
+    !> SWE.f90:380-429 on a frozen mesh.  Runs until time >= tmax (tmax >= 0)
+    !> or nmax steps (nmax >= 0); dt, time and forward are updated in place.
+    !> If host_steps is false and no time limit is set, the steps run on the
+    !> device through swe_run.  Returns a SWE_* status.
+    integer(c_int) function swe_time_loop(handle, courant, tmax, tout, nmax, host_steps, &
+                                          dt, time, forward, steps) result(rc)
+        type(c_ptr), intent(in)            :: handle
+        real(c_double), intent(in)         :: courant, tmax, tout
+        integer(c_int64_t), intent(in)     :: nmax
+        logical, intent(in)                :: host_steps
+        real(c_double), intent(inout)      :: dt, time
+        logical, intent(inout)             :: forward
+        integer(c_int64_t), intent(out)    :: steps
+        real(c_double) :: dt_cfl, added, clamp
+        integer(c_int64_t) :: nref
+        integer(c_int) :: fw
+
+        steps = 0
+        rc = SWE_OK
+        if (.not. host_steps .and. tmax < 0.0_c_double .and. nmax >= 0) then
+            clamp = -1.0_c_double
+            if (tout > 0.0_c_double) clamp = tout
+            if (tout > 0.0_c_double) dt = min(tout, dt)
+            fw = merge(1_c_int, 0_c_int, forward)
+            rc = swe_run(handle, nmax, dt, fw, clamp, added, dt_cfl)
+            if (rc /= SWE_OK) return
+            time = time + added
+            dt = dt_cfl
+            steps = nmax
+            if (mod(nmax, 2_c_int64_t) == 1) forward = .not. forward
+            return
+        end if
+        do
+            if ((tmax >= 0.0_c_double .and. time >= tmax) .or. (nmax >= 0 .and. steps >= nmax)) exit
+            steps = steps + 1
+            if (tmax > 0.0_c_double) dt = min(tmax, dt)
+            if (tout > 0.0_c_double) dt = min(tout, dt)
+            fw = merge(1_c_int, 0_c_int, forward)
+            rc = swe_step(handle, dt, fw, dt_cfl, nref, c_null_ptr)
+            if (rc /= SWE_OK) return
+            time = time + dt
+            dt = courant * dt_cfl
+            forward = .not. forward
+        end do
+    end function
+
+    ! ------------------------------------------------------------------
+    ! C entry points for the Python harness
+
+    real(c_double) function swe_host_initial_dt(n, depth, eta, hu, hv, b, g, courant) &
+            bind(c, name='swe_host_initial_dt')
+        integer(c_int64_t), value     :: n
+        integer(c_int8_t), intent(in) :: depth(n)
+        real(c_double), intent(in)    :: eta(n), hu(n), hv(n), b(n)
+        real(c_double), value         :: g, courant
+        swe_host_initial_dt = initial_dt(n, depth, eta, hu, hv, b, g, courant)
+    end function
+
+    subroutine swe_host_reference_dam_break(n, coords, eta, hu, hv, b) &
+            bind(c, name='swe_host_reference_dam_break')
+        integer(c_int64_t), value   :: n
+        real(c_double), intent(in)  :: coords(2, 3, n)
+        real(c_double), intent(out) :: eta(n), hu(n), hv(n), b(n)
+        integer(c_int64_t) :: i
+        real(c_double) :: x(2)
+        do i = 1, n
+            x = 10.0_c_double * centroid(coords(:, :, i)) - 5.0_c_double
+            eta(i) = merge(1.0_c_double, 0.0_c_double, x(1) < 0.0_c_double)
+        end do
+        hu = 0.0_c_double
+        hv = 0.0_c_double
+        b = 0.0_c_double
+    end subroutine
+
+    subroutine swe_host_radial_dam_break(n, coords, eta, hu, hv, b) &
+            bind(c, name='swe_host_radial_dam_break')
+        integer(c_int64_t), value   :: n
+        real(c_double), intent(in)  :: coords(2, 3, n)
+        real(c_double), intent(out) :: eta(n), hu(n), hv(n), b(n)
+        integer(c_int64_t) :: i
+        real(c_double) :: x(2)
+        do i = 1, n
+            x = centroid(coords(:, :, i))
+            eta(i) = merge(10.0_c_double, 5.0_c_double, norm2(x - 0.5_c_double) < 0.1_c_double)
+        end do
+        hu = 0.0_c_double
+        hv = 0.0_c_double
+        b = 0.0_c_double
+    end subroutine
+
+    subroutine swe_host_tsunami_draw(seed, dlo, dhi, amp, ratio, x0, y0, depth) &
+            bind(c, name='swe_host_tsunami_draw')
+        integer(c_int64_t), value   :: seed
+        integer(c_int), value       :: dlo, dhi
+        real(c_double), intent(out) :: amp, ratio, x0, y0
+        integer(c_int), intent(out) :: depth
+        integer :: d
+        call tsunami_draw(seed, amp, ratio, x0, y0, d, int(dlo), int(dhi))
+        depth = int(d, c_int)
+    end subroutine
+
+    subroutine swe_host_tsunami(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b) &
+            bind(c, name='swe_host_tsunami')
+        integer(c_int64_t), value   :: n, seed
+        real(c_double), intent(in)  :: coords(2, 3, n)
+        real(c_double), value       :: amp, ratio, x0, y0, drytol
+        real(c_double), intent(out) :: eta(n), hu(n), hv(n), b(n)
+        call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+    end subroutine
+
+    !> The time loop for a handle created by the caller.  forward is in/out
+    !> (0/1).  Returns a SWE_* status.
+    integer(c_int) function swe_host_time_loop(handle, courant, tmax, tout, nmax, host_steps, &
+                                               dt, time, forward, steps) bind(c, name='swe_host_time_loop')
+        type(c_ptr), value                :: handle
+        real(c_double), value             :: courant, tmax, tout
+        integer(c_int64_t), value         :: nmax
+        integer(c_int), value             :: host_steps
+        real(c_double), intent(inout)     :: dt, time
+        integer(c_int), intent(inout)     :: forward
+        integer(c_int64_t), intent(out)   :: steps
+        logical :: fw
+        fw = forward /= 0
+        swe_host_time_loop = swe_time_loop(handle, courant, tmax, tout, nmax, host_steps /= 0, &
+                                           dt, time, fw, steps)
+        forward = merge(1_c_int, 0_c_int, fw)
+    end function
+end module swe_driver

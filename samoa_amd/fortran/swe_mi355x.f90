@@ -1,0 +1,97 @@
+! swe_mi355x.f90 -- ISO_C_BINDING interface to the HIP SWE library.
+!
+! The thin layer through which a Fortran host (the reference's own language)
+! calls the MI355X Euler step.  Each interface mirrors one declaration of
+! include/swe_mi355x.h; `swe_step` replaces `swe%euler%traverse(grid)`
+! (SWE.f90:393).
+module swe_mi355x
+    use iso_c_binding
+    implicit none
+
+    integer(c_int), parameter :: SWE_OK = 0
+    integer(c_int), parameter :: SWE_ERR_ARG = -1, SWE_ERR_HIP = -2, SWE_ERR_NOMEM = -3, SWE_ERR_STATE = -4
+    integer(c_int32_t), parameter :: SWE_SOLVER_AUG_RIEMANN = 0, SWE_SOLVER_FWAVE = 1
+    integer(c_int32_t), parameter :: SWE_BC_OUTFLOW = 0, SWE_BC_WALL = 1
+
+    type, bind(c) :: swe_mesh_desc
+        integer(c_int64_t) :: n_cells
+        type(c_ptr)        :: depth
+        type(c_ptr)        :: plotter
+        type(c_ptr)        :: nbr
+        real(c_double)     :: scaling
+        real(c_double)     :: g
+        real(c_double)     :: dry_tolerance
+        real(c_double)     :: courant
+        integer(c_int32_t) :: min_depth
+        integer(c_int32_t) :: max_depth
+        integer(c_int32_t) :: solver
+        integer(c_int32_t) :: boundary
+    end type
+
+    interface
+        integer(c_int) function swe_create(desc, handle) bind(c, name='swe_create')
+            import :: c_int, c_ptr, swe_mesh_desc
+            type(swe_mesh_desc), intent(in) :: desc
+            type(c_ptr), intent(out)        :: handle
+        end function
+
+        integer(c_int) function swe_upload(handle, eta, hu, hv, b) bind(c, name='swe_upload')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value        :: handle
+            real(c_double), intent(in) :: eta(*), hu(*), hv(*), b(*)
+        end function
+
+        integer(c_int) function swe_download(handle, eta, hu, hv, b) bind(c, name='swe_download')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            type(c_ptr), value :: eta, hu, hv, b
+        end function
+
+        integer(c_int) function swe_step(handle, dt, forward, dt_cfl, n_refine, refine) bind(c, name='swe_step')
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value          :: handle
+            real(c_double), value       :: dt
+            integer(c_int), value       :: forward
+            real(c_double), intent(out) :: dt_cfl
+            integer(c_int64_t), intent(out) :: n_refine
+            type(c_ptr), value          :: refine
+        end function
+
+        integer(c_int) function swe_run(handle, n_steps, dt0, forward, dt_clamp, time_added, dt_next) bind(c, name='swe_run')
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value          :: handle
+            integer(c_int64_t), value   :: n_steps
+            real(c_double), value       :: dt0
+            integer(c_int), value       :: forward
+            real(c_double), value       :: dt_clamp
+            real(c_double), intent(out) :: time_added, dt_next
+        end function
+
+        integer(c_int) function swe_destroy(handle) bind(c, name='swe_destroy')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+        end function
+
+        integer(c_size_t) function swe_last_error(buf, size) bind(c, name='swe_last_error')
+            import :: c_size_t, c_char
+            character(kind=c_char), intent(out) :: buf(*)
+            integer(c_size_t), value            :: size
+        end function
+    end interface
+
+contains
+
+    !> The library's last error message as a Fortran string.
+    function swe_error_message() result(msg)
+        character(len=512) :: msg
+        character(kind=c_char) :: buf(512)
+        integer(c_size_t) :: n
+        integer :: i
+        n = swe_last_error(buf, int(size(buf), c_size_t))
+        msg = ' '
+        do i = 1, min(int(n), 511)
+            if (buf(i) == c_null_char) exit
+            msg(i:i) = buf(i)
+        end do
+    end function
+end module swe_mi355x

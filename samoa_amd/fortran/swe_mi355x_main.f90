@@ -1,0 +1,199 @@
+! swe_mi355x_main.f90 -- command-line SWE run on the MI355X, the counterpart
+! of the reference's `samoa` binary built with scenario=swe (SFC_main.f90,
+! SWE.f90:205-444) for frozen uniform meshes.
+!
+!   swe_mi355x -dmax 14 -nmax 100 [-dmin d] [-tmax t] [-tout t] [-nout k]
+!              [-courant c] [-drytolerance d] [-scenario dambreak|radial|tsunami]
+!              [-seed s] [-boundary outflow|wall] [-solver aug_riemann|fwave]
+!              [-dump prefix] [-hoststeps]
+!
+! Like the reference CLI (Config.f90:213-296, M_kracken rget), -courant,
+! -drytolerance, -tmax and -tout are read as default REAL before they are
+! stored in double precision.  With -nout k, every k-th step is followed by an
+! output traversal: the direction flips once more and, with -dump, the state
+! is written to <prefix>_<i>.bin (int64 n, float64 time, eta, hu, hv, b).
+program swe_mi355x_main
+    use iso_c_binding
+    use swe_mi355x
+    use swe_sfc_mesh
+    use swe_scenarios
+    use swe_driver
+    implicit none
+
+    integer :: dmax, dmin, nout, i, narg, ierr, depth_i
+    integer(c_int64_t) :: nmax, n, seed, steps, k, nsteps_done, chunk
+    real :: r_courant, r_drytol, r_tmax, r_tout
+    real(c_double) :: courant, drytol, tmax, tout, scaling, offset, dt, time, amp, ratio, x0, y0
+    character(len=256) :: arg, val, scenario, dump, bcname, solvername
+    logical :: host_steps, forward
+    integer(c_int8_t), allocatable, target :: plotter(:), depth(:)
+    integer(c_int32_t), allocatable, target :: nbr(:, :)
+    real(c_double), allocatable, target :: coords(:, :, :), eta(:), hu(:), hv(:), b(:)
+    type(swe_mesh_desc) :: desc
+    type(c_ptr) :: h
+    integer(c_int) :: rc
+    integer(c_int64_t) :: t0, t1, rate
+    integer :: iout
+
+    dmax = 14; dmin = -1; nmax = -1; nout = -1
+    r_courant = 0.45; r_drytol = 0.01; r_tmax = 3600.0; r_tout = -1.0
+    scenario = 'dambreak'; dump = ''; bcname = 'outflow'; solvername = 'aug_riemann'
+    seed = 0; host_steps = .false.
+    narg = command_argument_count()
+    i = 1
+    do while (i <= narg)
+        call get_command_argument(i, arg)
+        val = ''
+        if (i < narg) call get_command_argument(i + 1, val)
+        select case (trim(arg))
+        case ('-dmax', '-dstart'); read(val, *) dmax; i = i + 1
+        case ('-dmin'); read(val, *) dmin; i = i + 1
+        case ('-nmax'); read(val, *) nmax; i = i + 1
+        case ('-nout'); read(val, *) nout; i = i + 1
+        case ('-tmax'); read(val, *) r_tmax; i = i + 1
+        case ('-tout'); read(val, *) r_tout; i = i + 1
+        case ('-courant'); read(val, *) r_courant; i = i + 1
+        case ('-drytolerance'); read(val, *) r_drytol; i = i + 1
+        case ('-scenario'); scenario = val; i = i + 1
+        case ('-seed'); read(val, *) seed; i = i + 1
+        case ('-boundary'); bcname = val; i = i + 1
+        case ('-solver'); solvername = val; i = i + 1
+        case ('-dump'); dump = val; i = i + 1
+        case ('-hoststeps'); host_steps = .true.
+        case default
+            print '(A, A)', 'swe_mi355x: unknown argument ', trim(arg)
+            stop 2
+        end select
+        i = i + 1
+    end do
+    if (dmin < 0) dmin = dmax
+    courant = r_courant; drytol = r_drytol; tmax = r_tmax; tout = r_tout
+
+    n = ishft(1_c_int64_t, dmax + 1)
+    allocate(plotter(n), depth(n), nbr(3, n), coords(2, 3, n), eta(n), hu(n), hv(n), b(n))
+    call sfc_uniform_mesh(dmax, plotter, nbr, coords, ierr)
+    if (ierr /= 0) then
+        print '(A, I0)', 'swe_mi355x: mesh generation failed: ', ierr
+        stop 1
+    end if
+    depth = int(dmax, c_int8_t)
+
+    select case (trim(scenario))
+    case ('dambreak')
+        scaling = 10.0_c_double; offset = -5.0_c_double
+        call swe_host_reference_dam_break_f(n, coords, eta, hu, hv, b)
+    case ('radial')
+        scaling = 1.0_c_double; offset = 0.0_c_double
+        do k = 1, n
+            eta(k) = merge(10.0_c_double, 5.0_c_double, norm2(centroid(coords(:, :, k)) - 0.5_c_double) < 0.1_c_double)
+        end do
+        hu = 0; hv = 0; b = 0
+    case ('tsunami')
+        scaling = 1.0e6_c_double; offset = 0.0_c_double
+        call tsunami_draw(seed, amp, ratio, x0, y0, depth_i, dmax, dmax)
+        call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+    case default
+        print '(A, A)', 'swe_mi355x: unknown scenario ', trim(scenario)
+        stop 2
+    end select
+
+    desc%n_cells = n
+    desc%depth = c_loc(depth)
+    desc%plotter = c_loc(plotter)
+    desc%nbr = c_loc(nbr)
+    desc%scaling = scaling
+    desc%g = 9.80665_c_double
+    desc%dry_tolerance = drytol
+    desc%courant = courant
+    desc%min_depth = dmin
+    desc%max_depth = dmax
+    desc%solver = merge(SWE_SOLVER_FWAVE, SWE_SOLVER_AUG_RIEMANN, trim(solvername) == 'fwave')
+    desc%boundary = merge(SWE_BC_WALL, SWE_BC_OUTFLOW, trim(bcname) == 'wall')
+    rc = swe_create(desc, h)
+    call check(rc, 'swe_create')
+    rc = swe_upload(h, eta, hu, hv, b)
+    call check(rc, 'swe_upload')
+
+    ! init_grid leaves the grid reversed; init_b runs backward, init_dofs
+    ! forward (SWE.f90:230-234); the initial output traversal flips once more.
+    forward = .false.
+    time = 0.0_c_double
+    dt = initial_dt(n, depth, eta, hu, hv, b, 9.80665_c_double, courant)
+    iout = 0
+    if (nout > 0) then
+        if (len_trim(dump) > 0) call write_dump(iout)
+        iout = iout + 1
+        forward = .not. forward
+    end if
+
+    call system_clock(t0, rate)
+    nsteps_done = 0
+    do
+        if ((tmax >= 0.0_c_double .and. time >= tmax) .or. (nmax >= 0 .and. nsteps_done >= nmax)) exit
+        chunk = -1
+        if (nout > 0) chunk = int(nout, c_int64_t)
+        if (nmax >= 0) then
+            if (chunk < 0) then
+                chunk = nmax - nsteps_done
+            else
+                chunk = min(chunk, nmax - nsteps_done)
+            end if
+        end if
+        rc = swe_time_loop(h, courant, tmax, tout, chunk, host_steps .or. tmax >= 0.0_c_double, &
+                           dt, time, forward, steps)
+        call check(rc, 'swe_time_loop')
+        nsteps_done = nsteps_done + steps
+        if (nout > 0 .and. mod(nsteps_done, int(nout, c_int64_t)) == 0) then
+            if (len_trim(dump) > 0) then
+                rc = swe_download(h, c_loc(eta), c_loc(hu), c_loc(hv), c_loc(b))
+                call check(rc, 'swe_download')
+                call write_dump(iout)
+            end if
+            iout = iout + 1
+            forward = .not. forward
+        end if
+        if (steps == 0) exit
+    end do
+    call system_clock(t1)
+    rc = swe_download(h, c_loc(eta), c_loc(hu), c_loc(hv), c_loc(b))
+    call check(rc, 'swe_download')
+    if (len_trim(dump) > 0 .and. nout <= 0) call write_dump(0)
+    print '(A, I0, A, I0, A, ES22.15)', ' SWE: done. cells: ', n, ' steps: ', nsteps_done, ' time: ', time
+    if (t1 > t0) print '(A, F14.4, A)', ' Cell update throughput: ', &
+        1.0d-6 * real(n * nsteps_done, c_double) / (real(t1 - t0, c_double) / real(rate, c_double)), ' M/s'
+    rc = swe_destroy(h)
+
+contains
+
+    subroutine check(code, what)
+        integer(c_int), intent(in) :: code
+        character(*), intent(in)   :: what
+        if (code /= SWE_OK) then
+            print '(A, A, A, I0, A, A)', 'swe_mi355x: ', what, ' failed (', code, '): ', trim(swe_error_message())
+            stop 1
+        end if
+    end subroutine
+
+    subroutine write_dump(idx)
+        integer, intent(in) :: idx
+        character(len=512) :: fname
+        integer :: u
+        write(fname, '(A, A, I0, A)') trim(dump), '_', idx, '.bin'
+        open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
+        write(u) n, time, eta, hu, hv, b
+        close(u)
+    end subroutine
+
+    subroutine swe_host_reference_dam_break_f(nc, x, e, u, v, bb)
+        integer(c_int64_t), intent(in) :: nc
+        real(c_double), intent(in)     :: x(2, 3, nc)
+        real(c_double), intent(out)    :: e(nc), u(nc), v(nc), bb(nc)
+        integer(c_int64_t) :: j
+        real(c_double) :: p(2)
+        do j = 1, nc
+            p = 10.0_c_double * centroid(x(:, :, j)) - 5.0_c_double
+            e(j) = merge(1.0_c_double, 0.0_c_double, p(1) < 0.0_c_double)
+        end do
+        u = 0; v = 0; bb = 0
+    end subroutine
+end program swe_mi355x_main

@@ -1,0 +1,121 @@
+"""ctypes binding of the Fortran host library (samoa_amd/lib/libswe_host.so).
+
+The host side of the drop-in is Fortran, like the reference: uniform
+Sierpinski meshes (fortran/swe_sfc_mesh.f90), scenario initial conditions
+(fortran/swe_scenarios.f90) and the swe_run time loop
+(fortran/swe_driver.f90).  This module exposes them to the Python harness.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import _native
+from .swe import G, Mesh
+
+HOST_LIB = os.path.join(_native.LIB_DIR, "libswe_host.so")
+CLI = os.path.join(os.path.dirname(_native.LIB_DIR), "bin", "swe_mi355x")
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def load():
+    global _lib
+    if _lib is None:
+        _native.load()  # one HIP runtime per process, shared with torch
+        if not os.path.exists(HOST_LIB):
+            raise RuntimeError(f"{HOST_LIB} is not built; run `python samoa_amd/build.py`")
+        L = C.CDLL(HOST_LIB)
+        L.swe_host_mesh_uniform.restype = C.c_int64
+        L.swe_host_mesh_uniform.argtypes = [C.c_int, _i8p, _i32p, _dp]
+        L.swe_host_initial_dt.restype = C.c_double
+        L.swe_host_initial_dt.argtypes = [C.c_int64, _i8p, _dp, _dp, _dp, _dp, C.c_double, C.c_double]
+        for name in ("swe_host_reference_dam_break", "swe_host_radial_dam_break"):
+            fn = getattr(L, name)
+            fn.restype = None
+            fn.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp]
+        L.swe_host_tsunami_draw.restype = None
+        L.swe_host_tsunami_draw.argtypes = [C.c_int64, C.c_int, C.c_int] + [C.POINTER(C.c_double)] * 4 + [
+            C.POINTER(C.c_int)]
+        L.swe_host_tsunami.restype = None
+        L.swe_host_tsunami.argtypes = [C.c_int64, _dp, C.c_int64] + [C.c_double] * 5 + [_dp] * 4
+        L.swe_host_time_loop.restype = C.c_int
+        L.swe_host_time_loop.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int64,
+                                         C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                         C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+def mesh_uniform(depth: int) -> Mesh:
+    n = 1 << (depth + 1)
+    plotter = np.empty(n, np.int8)
+    nbr = np.empty(3 * n, np.int32)
+    coords = np.empty(6 * n)
+    got = load().swe_host_mesh_uniform(depth, plotter, nbr, coords)
+    if got != n:
+        raise RuntimeError(f"swe_host_mesh_uniform({depth}) failed: {got}")
+    return Mesh(np.full(n, depth, np.int8), plotter, nbr.reshape(n, 3), coords.reshape(n, 3, 2))
+
+
+def _state(mesh, fn):
+    out = [np.empty(mesh.n) for _ in range(4)]
+    fn(mesh.n, np.ascontiguousarray(mesh.coords.reshape(-1)), *out)
+    return tuple(out)
+
+
+def reference_dam_break(mesh: Mesh):
+    """The reference's built-in scenario (SWE_initialize.f90:349-370), domain
+    [-5,5]^2: use scaling 10."""
+    return _state(mesh, load().swe_host_reference_dam_break)
+
+
+def radial_dam_break(mesh: Mesh):
+    return _state(mesh, load().swe_host_radial_dam_break)
+
+
+def tsunami_draw(seed: int, dlo: int = 14, dhi: int = 20) -> dict:
+    a, r, x, y = (C.c_double() for _ in range(4))
+    d = C.c_int()
+    load().swe_host_tsunami_draw(seed, dlo, dhi, C.byref(a), C.byref(r), C.byref(x), C.byref(y),
+                                 C.byref(d))
+    return dict(seed=seed, amplitude=a.value, ratio=r.value, x0=x.value, y0=y.value, depth=d.value)
+
+
+def tsunami(mesh: Mesh, seed: int, drytol: float = 1e-3, amplitude=None, ratio=None, x0=None,
+            y0=None):
+    """Tsunami scenario on the 1000 km square (use scaling 1e6)."""
+    p = tsunami_draw(seed)
+    amp = p["amplitude"] if amplitude is None else amplitude
+    rat = p["ratio"] if ratio is None else ratio
+    xx = p["x0"] if x0 is None else x0
+    yy = p["y0"] if y0 is None else y0
+    out = [np.empty(mesh.n) for _ in range(4)]
+    load().swe_host_tsunami(mesh.n, np.ascontiguousarray(mesh.coords.reshape(-1)), seed, amp, rat,
+                            xx, yy, drytol, *out)
+    return tuple(out)
+
+
+def initial_dt(mesh: Mesh, state, courant: float, g: float = G) -> float:
+    """The init traversal's dt estimate (SWE_initialize.f90:327-335, :240)."""
+    eta, hu, hv, b = (np.ascontiguousarray(a, dtype=np.float64) for a in state)
+    return load().swe_host_initial_dt(mesh.n, np.ascontiguousarray(mesh.depth, np.int8), eta, hu,
+                                      hv, b, g, courant)
+
+
+def time_loop(op, *, dt: float, time: float = 0.0, forward: bool = False, nmax: int = -1,
+              tmax: float = -1.0, tout: float = -1.0, host_steps: bool = False):
+    """The Fortran swe_run loop (SWE.f90:380-429) on an EulerTimestep handle.
+    Returns (dt, time, forward, steps)."""
+    d, t = C.c_double(dt), C.c_double(time)
+    fw = C.c_int(int(bool(forward)))
+    steps = C.c_int64()
+    rc = load().swe_host_time_loop(op._h, op.courant, tmax, tout, nmax, int(bool(host_steps)),
+                                   C.byref(d), C.byref(t), C.byref(fw), C.byref(steps))
+    _native.check("swe_host_time_loop", rc)
+    return d.value, t.value, bool(fw.value), steps.value

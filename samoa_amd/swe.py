@@ -123,6 +123,10 @@ class EulerTimestep:
             self._h, eta, hu, hv, b, eta_out, hu_out, hv_out, float(dt), int(bool(forward)),
             dt_cfl, n_refine or None, refine or None, stream or None))
 
+    def set_stream(self, stream: int) -> None:
+        """Launch on a caller's HIP stream (0 = the library's own)."""
+        _native.check("swe_set_stream", self._lib.swe_set_stream(self._h, stream or None))
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.swe_destroy(self._h)

@@ -1,0 +1,90 @@
+"""The Fortran host over the C ABI on the GPU: the swe_mi355x command-line
+driver against the reference's golden runs, and the Fortran time loop
+against the oracle."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import golden_io
+import pyoracle as po
+from samoa_amd import host, swe
+
+pytestmark = pytest.mark.gpu
+
+
+def read_dump(path):
+    raw = open(path, "rb").read()
+    n = int(np.frombuffer(raw[:8], "<i8")[0])
+    t = float(np.frombuffer(raw[8:16], "<f8")[0])
+    arr = np.frombuffer(raw[16:], "<f8").reshape(4, n)
+    return t, arr
+
+
+@pytest.mark.parametrize("hoststeps", [False, True])
+@pytest.mark.parametrize("name", ["plain_d6_n1", "plain_d8_n3", "plain_d10_n5"])
+def test_cli_matches_reference_run(name, hoststeps):
+    _, z, meta = {r[0]: r for r in golden_io.runs()}[name]
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [host.CLI, "-scenario", "dambreak", "-dmax", str(meta["depth"]), "-nmax",
+               str(meta["steps"]), "-nout", str(meta["nout"]), "-tmax", "-1", "-courant", "0.45",
+               "-drytolerance", "0.01", "-dump", os.path.join(td, "s")]
+        if hoststeps:
+            cmd.append("-hoststeps")
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for i in range(len(z["times"])):
+            t, arr = read_dump(os.path.join(td, f"s_{i}.bin"))
+            assert t == z["times"][i], f"output {i}"
+            assert np.array_equal(arr, z["states"][i]), f"output {i}"
+
+
+def test_fortran_time_loop_vs_oracle():
+    d, S = 12, 1.0e6
+    mesh = host.mesh_uniform(d)
+    st = host.tsunami(mesh, seed=9, drytol=1e-3)
+    courant = 0.45
+    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3, courant=courant)
+    op.upload(*st)
+    dt0 = host.initial_dt(mesh, st, courant)
+    # host-stepped loop with a time limit (pre_traversal_grid_op clamps to tmax)
+    dt, t, fwd, steps = host.time_loop(op, dt=dt0, forward=False, nmax=9, tmax=-1.0,
+                                       host_steps=True)
+    ref = list(st)
+    rdt, rt, rfwd = dt0, 0.0, False
+    for _ in range(9):
+        e, u, v, _, cand, _ = po.step(mesh.depth, mesh.plotter, mesh.nbr, ref, rdt, scaling=S,
+                                      drytol=1e-3, forward=rfwd)
+        ref = [e, u, v, ref[3]]
+        rt += rdt
+        rdt = courant * cand
+        rfwd = not rfwd
+    assert steps == 9 and dt == rdt and t == rt and fwd == rfwd
+    assert all(np.array_equal(a, b) for a, b in zip(op.download(), ref))
+    # device chain continues the same run
+    dt2, t2, fwd2, steps2 = host.time_loop(op, dt=dt, time=t, forward=fwd, nmax=6)
+    for _ in range(6):
+        e, u, v, _, cand, _ = po.step(mesh.depth, mesh.plotter, mesh.nbr, ref, rdt, scaling=S,
+                                      drytol=1e-3, forward=rfwd)
+        ref = [e, u, v, ref[3]]
+        rt += rdt
+        rdt = courant * cand
+        rfwd = not rfwd
+    assert steps2 == 6 and dt2 == rdt and fwd2 == rfwd
+    assert abs(t2 - rt) <= 1e-15 * rt  # time summed on the device in another order
+    assert all(np.array_equal(a, b) for a, b in zip(op.download(), ref))
+    op.close()
+
+
+def test_tmax_stops_and_clamps():
+    d = 8
+    mesh = host.mesh_uniform(d)
+    st = host.reference_dam_break(mesh)
+    op = swe.EulerTimestep(mesh, scaling=10.0, dry_tolerance=0.01, courant=0.45)
+    op.upload(*st)
+    dt0 = host.initial_dt(mesh, st, 0.45)
+    dt, t, fwd, steps = host.time_loop(op, dt=dt0, forward=False, tmax=0.05, host_steps=True)
+    assert t >= 0.05 and steps > 1
+    op.close()

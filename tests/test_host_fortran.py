@@ -1,0 +1,71 @@
+"""The Fortran host (mesh, scenarios, initial dt) against the oracle and the
+reference's golden runs.  CPU only: no GPU call."""
+import numpy as np
+import pytest
+
+import golden_io
+import pyoracle as po
+from samoa_amd import host
+
+
+@pytest.mark.parametrize("d", list(range(0, 15)))
+def test_mesh_matches_oracle(d):
+    m = host.mesh_uniform(d)
+    plotter, co, nbr = po.mesh_uniform(d)
+    assert np.array_equal(m.plotter, plotter)
+    assert np.array_equal(m.nbr, nbr)
+    assert np.array_equal(m.coords, co)
+
+
+def test_mesh_large_is_consistent():
+    m = host.mesh_uniform(20)
+    n = m.n
+    assert n == 2 ** 21
+    nb = m.nbr
+    assert (nb < 0).sum() == 4 * 2 ** 10
+    # symmetry of the neighbour relation
+    c = np.repeat(np.arange(n), 3)
+    o = nb.reshape(-1)
+    inner = o >= 0
+    back = (nb[o[inner]] == c[inner, None]).any(axis=1)
+    assert back.all()
+    # consecutive SFC cells always share an edge
+    assert ((nb[:-1] == np.arange(1, n)[:, None]).any(axis=1)).all()
+
+
+@pytest.mark.parametrize("name", [r[0] for r in golden_io.runs()])
+def test_dam_break_and_initial_dt_match_reference(name):
+    _, z, meta = {r[0]: r for r in golden_io.runs()}[name]
+    if not meta["binary"].endswith("samoa_swe"):
+        pytest.skip("scenario-patched reference variant")
+    m = host.mesh_uniform(meta["depth"])
+    st = host.reference_dam_break(m)
+    assert np.array_equal(np.stack(st), z["states"][0])
+    dt0 = host.initial_dt(m, st, meta["courant"])
+    assert dt0 == z["times"][1] if meta["nout"] == 1 else dt0 == po.initial_dt(m.depth, st, meta["courant"])
+
+
+def test_initial_dt_matches_oracle_on_wet_dry_state():
+    m = host.mesh_uniform(10)
+    st = host.tsunami(m, seed=3)
+    for c in (0.45, float(np.float32(0.45)), 0.9):
+        assert host.initial_dt(m, st, c) == po.initial_dt(m.depth, st, c)
+
+
+def test_tsunami_scenario_properties():
+    m = host.mesh_uniform(12)
+    eta, hu, hv, b = host.tsunami(m, seed=5)
+    cen = m.centroids() * 1e6
+    assert not hu.any() and not hv.any()
+    deep = cen[:, 0] < 7e5
+    assert abs(b[deep].mean() + 4000) < 400 and 300 < b[deep].std() < 1300
+    assert b[cen[:, 0] > 0.99e6].min() > 0  # the shelf rises above sea level at the edge
+    dry = eta - b < 1e-3
+    assert dry.any() and (eta[dry] == np.minimum(b[dry], 0)).all()
+    assert 0.4 < eta.max() < 2.1
+    # deterministic in the seed, different across seeds
+    assert np.array_equal(host.tsunami(m, seed=5)[3], b)
+    assert not np.array_equal(host.tsunami(m, seed=6)[3], b)
+    p = [host.tsunami_draw(s) for s in range(64)]
+    ds = [q["depth"] for q in p]
+    assert min(ds) >= 14 and max(ds) <= 20 and len(set(ds)) >= 5
