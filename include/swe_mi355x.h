@@ -98,10 +98,12 @@ int swe_step(swe_handle *h, double dt, int forward, double *dt_cfl, int64_t *n_r
  * dt chain of post_traversal_grid_op / pre_traversal_grid_op is evaluated on
  * the device (dt_{k+1} = min(dt_clamp, courant * dt_cfl_k); dt_clamp <= 0
  * disables the clamp).  The direction starts at `forward` and flips after
- * every step.  Returns the simulated time added and the dt for the next step.
- * The steps are captured once into a HIP graph and replayed. */
+ * every step.  *time is the simulated time before the steps on entry and is
+ * advanced by each step's dt on the device (t = t + dt, in the reference's
+ * order); *dt_next receives the dt of the next step.  The steps are captured
+ * once into a HIP graph and replayed. */
 int swe_run(swe_handle *h, int64_t n_steps, double dt0, int forward, double dt_clamp,
-            double *time_added, double *dt_next);
+            double *time, double *dt_next);
 
 /* Device-pointer step for callers that own device memory (the PyTorch
  * binding): reads eta/hu/hv/b at the given device addresses, writes
@@ -127,8 +129,44 @@ int swe_destroy(swe_handle *h);
  * terminated when size > 0); returns its full length. */
 size_t swe_last_error(char *buf, size_t size);
 
-/* Number of cells a workgroup processes (the LDS chunk), for tests/tuning. */
+/* Default number of cells a workgroup processes (the LDS chunk). */
 int swe_chunk_cells(void);
+
+/* ---- Batched ensembles ----------------------------------------------------
+ * Many independent scenarios (meshes of uneven size, each with its own
+ * scaling, dry tolerance and dt chain) advanced together by one kernel launch
+ * per step through a per-scenario cell-offset table.  No counterpart in the
+ * reference (one grid per process, SFC_traversal.f90:38-139): each scenario
+ * behaves exactly as a single-mesh handle would.  Scenarios whose desc points
+ * at the same depth/plotter/nbr arrays share one device copy of the topology.
+ * solver, boundary and g must be equal across scenarios.  All scenarios step
+ * in the same traversal direction. */
+typedef struct swe_batch swe_batch;
+
+int swe_batch_create(int32_t n_scenarios, const swe_mesh_desc *descs, swe_batch **out);
+int swe_batch_upload(swe_batch *b, int32_t scenario, const double *eta, const double *hu,
+                     const double *hv, const double *bathymetry);
+int swe_batch_download(swe_batch *b, int32_t scenario, double *eta, double *hu, double *hv,
+                       double *bathymetry);
+/* One traversal of every scenario: dt[s] in, dt_cfl[s] out (and n_refine[s],
+ * nullable), as swe_step per scenario.  Synchronous. */
+int swe_batch_step(swe_batch *b, const double *dt, int forward, double *dt_cfl,
+                   int64_t *n_refine);
+/* n_steps traversals of every scenario with per-scenario device dt chains, as
+ * swe_run per scenario: dt0[s] and time[s] (in/out), dt_next[s] out.  The
+ * steps are replayed from a captured HIP graph. */
+int swe_batch_run(swe_batch *b, int64_t n_steps, const double *dt0, int forward,
+                  double dt_clamp, double *time, double *dt_next);
+/* Launch shape: chunk_cells in {128, 256, 512} (cells per workgroup, one LDS
+ * tile), grid_blocks 0 (one workgroup per chunk) or a multiple of 8 (a
+ * persistent grid that strides over the chunks).  Results do not depend on
+ * it; the autotuner (samoa_amd/autotune.py) picks the fastest. */
+int swe_batch_set_tuning(swe_batch *b, int chunk_cells, int64_t grid_blocks);
+int swe_batch_get_tuning(swe_batch *b, int *chunk_cells, int64_t *grid_blocks);
+int swe_batch_set_stream(swe_batch *b, void *stream);
+/* Total cells of all scenarios, or -1 for a NULL batch. */
+int64_t swe_batch_cells(swe_batch *b);
+int swe_batch_destroy(swe_batch *b);
 
 #ifdef __cplusplus
 }

@@ -4,4 +4,4 @@ The compute path is the HIP library samoa_amd/lib/libswe_mi355x.so behind the
 C ABI of include/swe_mi355x.h.
 """
 from .swe import (BC_OUTFLOW, BC_WALL, G, SOLVER_AUG_RIEMANN, SOLVER_FWAVE,  # noqa: F401
-                  EulerTimestep, Grid, Mesh, StepResult)
+                  Batch, EulerTimestep, Grid, Mesh, StepResult)

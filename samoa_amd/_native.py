@@ -58,6 +58,16 @@ SIGNATURES = {
     "swe_destroy": (_I, [_P]),
     "swe_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
     "swe_chunk_cells": (_I, []),
+    "swe_batch_create": (_I, [C.c_int32, _P, C.POINTER(_P)]),
+    "swe_batch_upload": (_I, [_P, C.c_int32, _P, _P, _P, _P]),
+    "swe_batch_download": (_I, [_P, C.c_int32, _P, _P, _P, _P]),
+    "swe_batch_step": (_I, [_P, _P, _I, _P, _P]),
+    "swe_batch_run": (_I, [_P, _I64, _P, _I, _D, _P, _P]),
+    "swe_batch_set_tuning": (_I, [_P, _I, _I64]),
+    "swe_batch_get_tuning": (_I, [_P, C.POINTER(_I), C.POINTER(_I64)]),
+    "swe_batch_set_stream": (_I, [_P, _P]),
+    "swe_batch_cells": (_I64, [_P]),
+    "swe_batch_destroy": (_I, [_P]),
 }
 
 _lib = None

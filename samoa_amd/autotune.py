@@ -1,0 +1,86 @@
+"""Launch-shape autotuner for batched ensembles.
+
+Tries every compiled chunk size (cells per workgroup, i.e. the LDS tile) and a
+few grid sizes (one workgroup per chunk, or persistent grids of k workgroups
+per CU) on the batch's own meshes and state, timing a few graph-replayed
+steps of each with HIP events on the stream the kernels run on.  The state is
+saved before and restored after, so tuning does not advance the simulation.
+Results are cached per device and batch signature in
+~/.cache/samoa_amd/autotune.json.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+
+import numpy as np
+
+CHUNKS = (128, 256, 512)
+CACHE = os.path.join(os.path.expanduser("~"), ".cache", "samoa_amd", "autotune.json")
+
+
+def signature(batch) -> str:
+    sizes = sorted(m.n for m in batch.meshes)
+    h = hashlib.sha1(json.dumps(sizes).encode()).hexdigest()[:16]
+    return f"n{batch.n_scen}_c{batch.cells}_{h}"
+
+
+def candidates(n_cus: int):
+    out = []
+    for chunk in CHUNKS:
+        out.append((chunk, 0))
+        for k in (2, 4, 8):
+            out.append((chunk, 8 * ((k * n_cus + 7) // 8)))
+    return out
+
+
+def tune(batch, dt0, steps: int = 6, use_cache: bool = True, verbose: bool = False) -> dict:
+    import torch
+
+    dev = torch.cuda.current_device()
+    props = torch.cuda.get_device_properties(dev)
+    key = f"{props.name}|{signature(batch)}"
+    cache = {}
+    if use_cache and os.path.exists(CACHE):
+        try:
+            cache = json.load(open(CACHE))
+        except (OSError, ValueError):
+            cache = {}
+        if key in cache:
+            best = cache[key]
+            batch.set_tuning(best["chunk"], best["grid"])
+            return best
+
+    saved = [batch.download(s) for s in range(batch.n_scen)]
+    stream = torch.cuda.Stream(device=dev)
+    batch.set_stream(stream.cuda_stream)
+    timings = []
+    try:
+        for chunk, grid in candidates(props.multi_processor_count):
+            batch.set_tuning(chunk, grid)
+            batch.run(2, dt0, forward=False)  # capture + warm
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            start.record(stream)
+            batch.run(steps, dt0, forward=False)
+            end.record(stream)
+            torch.cuda.synchronize()
+            ms = start.elapsed_time(end) / steps
+            timings.append({"chunk": chunk, "grid": grid, "ms_per_step": ms})
+            if verbose:
+                print(f"autotune: chunk {chunk} grid {grid}: {ms:.4f} ms/step", flush=True)
+    finally:
+        for s, st in enumerate(saved):
+            batch.upload(s, *st)
+        batch.set_stream(0)
+    best = min(timings, key=lambda r: r["ms_per_step"])
+    result = {"chunk": best["chunk"], "grid": best["grid"], "ms_per_step": best["ms_per_step"],
+              "trials": timings}
+    batch.set_tuning(result["chunk"], result["grid"])
+    if use_cache:
+        cache[key] = result
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        with open(CACHE, "w") as f:
+            json.dump(cache, f, indent=1)
+    return result

@@ -1,17 +1,22 @@
 // swe_api.hip -- host side of the C ABI declared in include/swe_mi355x.h.
 //
-// Owns the device buffers of one frozen mesh (SoA state double-buffered,
-// packed topology), launches the fused step kernel, and keeps the reference's
-// dt bookkeeping either on the host (swe_step) or on the device inside a
-// captured HIP graph (swe_run).
+// A `Batch` owns the device buffers of one or more frozen meshes (SoA state
+// double-buffered and concatenated, topology shared between scenarios that
+// pass the same mesh arrays), the per-scenario table the step kernel reads,
+// and the reference's dt bookkeeping either on the host (swe_step,
+// swe_batch_step) or on the device inside a captured HIP graph (swe_run,
+// swe_batch_run).  A single-mesh handle is a batch of one.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "swe_internal.hpp"
@@ -37,6 +42,14 @@ int fail(int code, const char *fmt, ...) {
         if (e_ != hipSuccess)                                                            \
             return fail(SWE_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));     \
     } while (0)
+
+#define SWE_TRY(expr)                \
+    do {                             \
+        int rc_ = (expr);            \
+        if (rc_ != SWE_OK) return rc_; \
+    } while (0)
+
+const double HUGE_SR = 1.7976931348623157e308;  // huge(1.0_SR)
 
 // Geometry tables, computed exactly as the reference initialises them.
 // SFC_data_types.f90:627-686: Jacobian of each plotter type rounded to whole
@@ -107,82 +120,416 @@ int ensure_tables(int device) {
 }
 
 // The captured pair of steps bakes in which buffer holds the state, the
-// starting direction and the clamp; a pair returns the state to the buffer it
-// started from, so the graph stays valid across swe_run calls.
+// starting direction and the launch shape; a pair returns the state to the
+// buffer it started from, so the graph stays valid across run calls.
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
-    int cur = -1;
-    int forward = -1;
-    double dt_clamp = 0.0;
+    int cur = -1, forward = -1, chunk = 0;
+    int64_t grid = -1;
 };
+
+// Validates a mesh and packs its neighbours as (o << 2) | back edge.
+int pack_mesh(const swe_mesh_desc &d, const double *normal, std::vector<int32_t> &packed,
+              int *uniform_depth) {
+    const int64_t n = d.n_cells;
+    packed.assign((size_t)(3 * n), -1);
+    int ud = d.depth[0];
+    for (int64_t c = 0; c < n; ++c) {
+        const int dep = d.depth[c];
+        const int p = d.plotter[c];
+        if (dep < 0 || dep > 60) return fail(SWE_ERR_ARG, "depth[%lld] = %d", (long long)c, dep);
+        if (dep != ud) ud = -1;
+        if (p < -8 || p > 8 || p == 0)
+            return fail(SWE_ERR_ARG, "plotter[%lld] = %d", (long long)c, p);
+        for (int k = 0; k < 3; ++k) {
+            const int32_t o = d.nbr[3 * c + k];
+            if (o < 0) {
+                if (o != -1) return fail(SWE_ERR_ARG, "nbr[%lld] = %d", (long long)(3 * c + k), o);
+                continue;
+            }
+            if (o >= n || o == c)
+                return fail(SWE_ERR_ARG, "nbr[%lld] = %d invalid", (long long)(3 * c + k), o);
+            int ko = -1;
+            for (int j = 0; j < 3; ++j)
+                if (d.nbr[3 * (int64_t)o + j] == (int32_t)c) ko = j;
+            if (ko < 0)
+                return fail(SWE_ERR_ARG, "neighbour relation not symmetric at cell %lld edge %d",
+                            (long long)c, k);
+            const int po = d.plotter[o];
+            const double *nc = &normal[((p + 8) * 3 + k) * 2];
+            const double *no = &normal[((po + 8) * 3 + ko) * 2];
+            if (!(nc[0] == -no[0] && nc[1] == -no[1]))
+                return fail(SWE_ERR_ARG,
+                            "cells %lld and %d disagree on the normal of their shared edge",
+                            (long long)c, o);
+            packed[3 * c + k] = (int32_t)((o << 2) | ko);
+        }
+    }
+    *uniform_depth = ud;
+    return SWE_OK;
+}
 
 }  // namespace
 
-struct swe_handle {
+struct Batch {
     int device = 0;
-    int64_t n = 0;
-    swe_mesh_desc desc{};
-    double refine_thr = 0.0;
-    hipStream_t stream = nullptr;         // where work is launched (caller may replace it)
-    hipStream_t own_stream = nullptr;     // created by the library; used for graph capture
+    hipStream_t stream = nullptr;      // where work is launched (caller may replace it)
+    hipStream_t own_stream = nullptr;  // created by the library; used for graph capture
+    int solver = 0, bc = 0;
+    double g = 9.80665;
+    int32_t nscen = 0;
+    std::vector<swe::Scen> scen;
+    std::vector<int> uniform_depth;
+    int64_t ncells = 0, ntopo = 0, nchunks = 0;
+    int chunk = swe::DEFAULT_CHUNK;
+    int64_t grid = 0;
+    swe::Scen *d_scen = nullptr;
+    int32_t *d_chunk_scen = nullptr;
+    swe::Slots *d_slots = nullptr;
+    double *d_dt = nullptr;  // [nscen] host-chosen dt per scenario
     double *eta[2] = {nullptr, nullptr}, *hu[2] = {nullptr, nullptr}, *hv[2] = {nullptr, nullptr};
     double *b = nullptr;
     int32_t *nbr = nullptr;
     int8_t *plotter = nullptr, *depth = nullptr, *refine = nullptr;
-    swe::Slots *slots = nullptr;  // reduction slots and device dt chain
     int cur = 0;
-    bool has_state = false;
+    std::vector<char> uploaded;
     GraphCache graph;
+};
+
+struct swe_handle {
+    Batch B;
+};
+struct swe_batch {
+    Batch B;
 };
 
 namespace {
 
-void release(swe_handle *h) {
-    if (!h) return;
-    // teardown: errors cannot be reported from here, the handle is gone either way
-    if (h->graph.exec) (void)hipGraphExecDestroy(h->graph.exec);
+void release(Batch &B) {
+    // teardown: errors cannot be reported from here, the buffers go either way
+    if (B.graph.exec) (void)hipGraphExecDestroy(B.graph.exec);
     for (int i = 0; i < 2; ++i) {
-        (void)hipFree(h->eta[i]);
-        (void)hipFree(h->hu[i]);
-        (void)hipFree(h->hv[i]);
+        (void)hipFree(B.eta[i]);
+        (void)hipFree(B.hu[i]);
+        (void)hipFree(B.hv[i]);
     }
-    (void)hipFree(h->b);
-    (void)hipFree(h->nbr);
-    (void)hipFree(h->plotter);
-    (void)hipFree(h->depth);
-    (void)hipFree(h->refine);
-    (void)hipFree(h->slots);
-    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
-    delete h;
+    (void)hipFree(B.b);
+    (void)hipFree(B.nbr);
+    (void)hipFree(B.plotter);
+    (void)hipFree(B.depth);
+    (void)hipFree(B.refine);
+    (void)hipFree(B.d_scen);
+    (void)hipFree(B.d_chunk_scen);
+    (void)hipFree(B.d_slots);
+    (void)hipFree(B.d_dt);
+    if (B.own_stream) (void)hipStreamDestroy(B.own_stream);
 }
 
-swe::StepArgs make_args(const swe_handle *h, const double *eta, const double *hu,
-                        const double *hv, const double *b, double *eo, double *ho, double *vo,
-                        double dt, int forward) {
+bool all_uploaded(const Batch &B) {
+    return std::all_of(B.uploaded.begin(), B.uploaded.end(), [](char c) { return c != 0; });
+}
+
+// Chunk tables for the current chunk size: chunk offsets per scenario and the
+// chunk -> scenario map (not needed for a single scenario).
+int rebuild_chunks(Batch &B) {
+    int64_t off = 0;
+    for (auto &s : B.scen) {
+        s.chunk_off = off;
+        s.nchunks = (s.n + B.chunk - 1) / B.chunk;
+        off += s.nchunks;
+    }
+    B.nchunks = off;
+    if (B.nchunks >= (int64_t(1) << 31))
+        return fail(SWE_ERR_ARG, "too many chunks (%lld)", (long long)B.nchunks);
+    (void)hipFree(B.d_chunk_scen);
+    B.d_chunk_scen = nullptr;
+    if (B.nscen > 1) {
+        std::vector<int32_t> map((size_t)B.nchunks);
+        for (int32_t s = 0; s < B.nscen; ++s)
+            std::fill(map.begin() + B.scen[s].chunk_off,
+                      map.begin() + B.scen[s].chunk_off + B.scen[s].nchunks, s);
+        if (hipMalloc(&B.d_chunk_scen, sizeof(int32_t) * map.size()) != hipSuccess)
+            return fail(SWE_ERR_NOMEM, "chunk table allocation failed");
+        HIP_TRY(hipMemcpy(B.d_chunk_scen, map.data(), sizeof(int32_t) * map.size(),
+                          hipMemcpyHostToDevice));
+    }
+    HIP_TRY(hipMemcpy(B.d_scen, B.scen.data(), sizeof(swe::Scen) * B.scen.size(),
+                      hipMemcpyHostToDevice));
+    if (B.graph.exec) {
+        (void)hipGraphExecDestroy(B.graph.exec);
+        B.graph = GraphCache{};
+    }
+    return SWE_OK;
+}
+
+int batch_init(Batch &B, int32_t nscen, const swe_mesh_desc *descs) {
+    if (nscen <= 0 || !descs) return fail(SWE_ERR_ARG, "no scenarios");
+    B.nscen = nscen;
+    B.solver = descs[0].solver;
+    B.bc = descs[0].boundary;
+    B.g = descs[0].g;
+    double normal[17 * 3 * 2], leg[64];
+    normal_table(normal);
+    leg_table(leg);
+    // unique topologies, keyed by the caller's mesh arrays
+    std::map<std::tuple<const void *, const void *, const void *, int64_t>, int64_t> topo_off;
+    std::vector<std::vector<int32_t>> packs;
+    std::vector<const swe_mesh_desc *> topo_src;
+    B.scen.resize((size_t)nscen);
+    B.uniform_depth.resize((size_t)nscen);
+    for (int32_t s = 0; s < nscen; ++s) {
+        const swe_mesh_desc &d = descs[s];
+        const int64_t n = d.n_cells;
+        if (n <= 0 || n > (int64_t(1) << 29))
+            return fail(SWE_ERR_ARG, "scenario %d: n_cells %lld out of range (1..2^29)", s,
+                        (long long)n);
+        if (!d.depth || !d.plotter || !d.nbr)
+            return fail(SWE_ERR_ARG, "scenario %d: null mesh array", s);
+        if (!(d.scaling > 0.0) || !(d.g > 0.0) || !(d.dry_tolerance >= 0.0))
+            return fail(SWE_ERR_ARG, "scenario %d: scaling and g must be > 0, dry_tolerance >= 0", s);
+        if (d.solver != SWE_SOLVER_AUG_RIEMANN && d.solver != SWE_SOLVER_FWAVE)
+            return fail(SWE_ERR_ARG, "scenario %d: unknown solver %d", s, d.solver);
+        if (d.boundary != SWE_BC_OUTFLOW && d.boundary != SWE_BC_WALL)
+            return fail(SWE_ERR_ARG, "scenario %d: unknown boundary %d", s, d.boundary);
+        if (d.solver != B.solver || d.boundary != B.bc || d.g != B.g)
+            return fail(SWE_ERR_ARG, "scenario %d: solver, boundary and g must match scenario 0", s);
+        if (d.max_depth < 0 || d.max_depth > 60)
+            return fail(SWE_ERR_ARG, "scenario %d: max_depth %d out of range", s, d.max_depth);
+        auto key = std::make_tuple((const void *)d.nbr, (const void *)d.plotter,
+                                   (const void *)d.depth, n);
+        auto it = topo_off.find(key);
+        int ud;
+        if (it == topo_off.end()) {
+            packs.emplace_back();
+            int rc = pack_mesh(d, normal, packs.back(), &ud);
+            if (rc != SWE_OK) return fail(rc, "scenario %d: %s", s, g_last_error.c_str());
+            topo_off[key] = B.ntopo;
+            topo_src.push_back(&d);
+            B.scen[s].topo_off = B.ntopo;
+            B.ntopo += n;
+        } else {
+            B.scen[s].topo_off = it->second;
+            ud = d.depth[0];
+            for (int64_t c = 1; c < n && ud >= 0; ++c)
+                if (d.depth[c] != ud) ud = -1;
+        }
+        swe::Scen &sc = B.scen[s];
+        sc.cell_off = B.ncells;
+        sc.n = n;
+        sc.scaling = d.scaling;
+        sc.drytol = d.dry_tolerance;
+        sc.refine_thr = (5.0 * d.scaling) * leg[d.max_depth + 1];  // volume_op :339
+        sc.courant = d.courant;
+        sc.dt_clamp = -1.0;
+        sc.depth = ud;
+        sc.dmin = d.min_depth;
+        sc.dmax = d.max_depth;
+        B.ncells += n;
+    }
+    HIP_TRY(hipGetDevice(&B.device));
+    SWE_TRY(ensure_tables(B.device));
+    const size_t bytes = sizeof(double) * (size_t)B.ncells;
+    bool ok = hipStreamCreate(&B.own_stream) == hipSuccess;
+    B.stream = B.own_stream;
+    for (int i = 0; i < 2 && ok; ++i)
+        ok = hipMalloc(&B.eta[i], bytes) == hipSuccess && hipMalloc(&B.hu[i], bytes) == hipSuccess &&
+             hipMalloc(&B.hv[i], bytes) == hipSuccess;
+    ok = ok && hipMalloc(&B.b, bytes) == hipSuccess &&
+         hipMalloc(&B.nbr, sizeof(int32_t) * 3 * (size_t)B.ntopo) == hipSuccess &&
+         hipMalloc(&B.plotter, (size_t)B.ntopo) == hipSuccess &&
+         hipMalloc(&B.depth, (size_t)B.ntopo) == hipSuccess &&
+         hipMalloc(&B.refine, (size_t)B.ncells) == hipSuccess &&
+         hipMalloc(&B.d_scen, sizeof(swe::Scen) * (size_t)nscen) == hipSuccess &&
+         hipMalloc(&B.d_slots, sizeof(swe::Slots) * (size_t)nscen) == hipSuccess &&
+         hipMalloc(&B.d_dt, sizeof(double) * (size_t)nscen) == hipSuccess;
+    if (!ok) return fail(SWE_ERR_NOMEM, "device allocation of %lld cells failed", (long long)B.ncells);
+    for (size_t i = 0; i < packs.size(); ++i) {
+        const swe_mesh_desc &d = *topo_src[i];
+        const int64_t off = topo_off[std::make_tuple((const void *)d.nbr, (const void *)d.plotter,
+                                                     (const void *)d.depth, d.n_cells)];
+        HIP_TRY(hipMemcpy(B.nbr + 3 * off, packs[i].data(), sizeof(int32_t) * packs[i].size(),
+                          hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(B.plotter + off, d.plotter, (size_t)d.n_cells, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(B.depth + off, d.depth, (size_t)d.n_cells, hipMemcpyHostToDevice));
+    }
+    std::vector<swe::Slots> init((size_t)nscen);
+    for (auto &sl : init) std::memcpy(&sl.dtmin, &HUGE_SR, sizeof HUGE_SR);
+    HIP_TRY(hipMemcpy(B.d_slots, init.data(), sizeof(swe::Slots) * init.size(),
+                      hipMemcpyHostToDevice));
+    B.uploaded.assign((size_t)nscen, 0);
+    return rebuild_chunks(B);
+}
+
+swe::StepArgs make_args(const Batch &B, int from, int to, int forward) {
     swe::StepArgs a{};
-    a.eta = eta;
-    a.hu = hu;
-    a.hv = hv;
-    a.b = b;
-    a.eta_out = eo;
-    a.hu_out = ho;
-    a.hv_out = vo;
-    a.nbr = h->nbr;
-    a.plotter = h->plotter;
-    a.depth = h->depth;
-    a.n = h->n;
-    a.dt = dt;
-    a.scaling = h->desc.scaling;
-    a.g = h->desc.g;
-    a.drytol = h->desc.dry_tolerance;
-    a.refine_thr = h->refine_thr;
-    a.dmin = h->desc.min_depth;
-    a.dmax = h->desc.max_depth;
-    a.bc = h->desc.boundary;
+    a.eta = B.eta[from];
+    a.hu = B.hu[from];
+    a.hv = B.hv[from];
+    a.b = B.b;
+    a.eta_out = B.eta[to];
+    a.hu_out = B.hu[to];
+    a.hv_out = B.hv[to];
+    a.nbr = B.nbr;
+    a.plotter = B.plotter;
+    a.depth = B.depth;
+    a.scen = B.d_scen;
+    a.chunk_scen = B.d_chunk_scen;
+    a.slots = B.d_slots;
+    a.nchunks = B.nchunks;
+    a.g = B.g;
+    a.inv16g = 1.0 / (16.0 * B.g);
+    a.bc = B.bc;
     a.forward = forward ? 1 : 0;
-    a.courant = h->desc.courant;
-    a.slots = h->slots;
     return a;
+}
+
+int batch_upload(Batch &B, int32_t s, const double *eta, const double *hu, const double *hv,
+                 const double *b) {
+    if (s < 0 || s >= B.nscen) return fail(SWE_ERR_ARG, "scenario %d out of range", s);
+    if (!eta || !hu || !hv || !b) return fail(SWE_ERR_ARG, "upload: null argument");
+    const size_t bytes = sizeof(double) * (size_t)B.scen[s].n;
+    const int64_t o = B.scen[s].cell_off;
+    HIP_TRY(hipMemcpyAsync(B.eta[B.cur] + o, eta, bytes, hipMemcpyHostToDevice, B.stream));
+    HIP_TRY(hipMemcpyAsync(B.hu[B.cur] + o, hu, bytes, hipMemcpyHostToDevice, B.stream));
+    HIP_TRY(hipMemcpyAsync(B.hv[B.cur] + o, hv, bytes, hipMemcpyHostToDevice, B.stream));
+    HIP_TRY(hipMemcpyAsync(B.b + o, b, bytes, hipMemcpyHostToDevice, B.stream));
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    B.uploaded[s] = 1;
+    return SWE_OK;
+}
+
+int batch_download(Batch &B, int32_t s, double *eta, double *hu, double *hv, double *b) {
+    if (s < 0 || s >= B.nscen) return fail(SWE_ERR_ARG, "scenario %d out of range", s);
+    if (!B.uploaded[s]) return fail(SWE_ERR_STATE, "download: scenario %d has no state", s);
+    const size_t bytes = sizeof(double) * (size_t)B.scen[s].n;
+    const int64_t o = B.scen[s].cell_off;
+    if (eta) HIP_TRY(hipMemcpyAsync(eta, B.eta[B.cur] + o, bytes, hipMemcpyDeviceToHost, B.stream));
+    if (hu) HIP_TRY(hipMemcpyAsync(hu, B.hu[B.cur] + o, bytes, hipMemcpyDeviceToHost, B.stream));
+    if (hv) HIP_TRY(hipMemcpyAsync(hv, B.hv[B.cur] + o, bytes, hipMemcpyDeviceToHost, B.stream));
+    if (b) HIP_TRY(hipMemcpyAsync(b, B.b + o, bytes, hipMemcpyDeviceToHost, B.stream));
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    return SWE_OK;
+}
+
+// One traversal with host-chosen dt per scenario; results per scenario.
+int batch_step(Batch &B, const double *dt, int forward, double *dt_cfl, int64_t *n_refine,
+               int8_t *refine_host) {
+    if (!dt || !dt_cfl) return fail(SWE_ERR_ARG, "step: null argument");
+    if (!all_uploaded(B)) return fail(SWE_ERR_STATE, "step: not every scenario has a state");
+    HIP_TRY(hipMemcpyAsync(B.d_dt, dt, sizeof(double) * (size_t)B.nscen, hipMemcpyHostToDevice,
+                           B.stream));
+    swe::StepArgs a = make_args(B, B.cur, 1 - B.cur, forward);
+    a.dt_host = B.d_dt;
+    a.refine = refine_host ? B.refine : nullptr;
+    HIP_TRY(swe::launch_step(a, B.solver, B.chunk, B.grid, B.stream));
+    std::vector<swe::Slots> res((size_t)B.nscen);
+    HIP_TRY(hipMemcpyAsync(res.data(), B.d_slots, sizeof(swe::Slots) * res.size(),
+                           hipMemcpyDeviceToHost, B.stream));
+    if (refine_host)
+        HIP_TRY(hipMemcpyAsync(refine_host, B.refine, (size_t)B.ncells, hipMemcpyDeviceToHost,
+                               B.stream));
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    for (int32_t s = 0; s < B.nscen; ++s) {
+        dt_cfl[s] = res[s].res_dtcfl;
+        if (n_refine) n_refine[s] = (int64_t)res[s].res_nref;
+    }
+    B.cur = 1 - B.cur;
+    return SWE_OK;
+}
+
+// n_steps traversals with the dt chain on the device, per scenario.
+int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double dt_clamp,
+              double *time, double *dt_next) {
+    if (n_steps < 0 || !dt0 || !time) return fail(SWE_ERR_ARG, "run: invalid argument");
+    if (!all_uploaded(B)) return fail(SWE_ERR_STATE, "run: not every scenario has a state");
+    bool table_changed = false;
+    for (auto &s : B.scen)
+        if (s.dt_clamp != dt_clamp) {
+            s.dt_clamp = dt_clamp;
+            table_changed = true;
+        }
+    if (table_changed)
+        HIP_TRY(hipMemcpyAsync(B.d_scen, B.scen.data(), sizeof(swe::Scen) * B.scen.size(),
+                               hipMemcpyHostToDevice, B.stream));
+    // the chain starts from dt0 (clamped as pre_traversal_grid_op does) and the
+    // caller's time; the reduction slots keep their between-launch invariant
+    std::vector<swe::Slots> init((size_t)B.nscen);
+    for (int32_t s = 0; s < B.nscen; ++s) {
+        std::memcpy(&init[s].dtmin, &HUGE_SR, sizeof HUGE_SR);
+        init[s].dt = (dt_clamp > 0.0 && dt_clamp < dt0[s]) ? dt_clamp : dt0[s];
+        init[s].time = time[s];
+    }
+    HIP_TRY(hipMemcpyAsync(B.d_slots, init.data(), sizeof(swe::Slots) * init.size(),
+                           hipMemcpyHostToDevice, B.stream));
+
+    auto one_step = [&](hipStream_t st, int from, int fwd) -> hipError_t {
+        swe::StepArgs a = make_args(B, from, 1 - from, fwd);
+        a.device_dt = 1;
+        return swe::launch_step(a, B.solver, B.chunk, B.grid, st);
+    };
+    int fwd = forward ? 1 : 0;
+    const int64_t pairs = n_steps / 2;
+    if (pairs > 0) {
+        GraphCache &gc = B.graph;
+        if (!gc.exec || gc.cur != B.cur || gc.forward != fwd || gc.chunk != B.chunk ||
+            gc.grid != B.grid) {
+            if (gc.exec) {
+                (void)hipGraphExecDestroy(gc.exec);
+                gc.exec = nullptr;
+            }
+            // capture on the library's own stream, replay on B.stream
+            HIP_TRY(hipStreamSynchronize(B.stream));
+            hipGraph_t graph;
+            HIP_TRY(hipStreamBeginCapture(B.own_stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e1 = one_step(B.own_stream, B.cur, fwd);
+            hipError_t e2 = e1 == hipSuccess ? one_step(B.own_stream, 1 - B.cur, 1 - fwd) : e1;
+            hipError_t e3 = hipStreamEndCapture(B.own_stream, &graph);
+            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+                return fail(SWE_ERR_HIP, "run: graph capture failed: %s",
+                            hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3)));
+            hipError_t e4 = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e4 != hipSuccess) return fail(SWE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e4));
+            gc.cur = B.cur;
+            gc.forward = fwd;
+            gc.chunk = B.chunk;
+            gc.grid = B.grid;
+        }
+        for (int64_t i = 0; i < pairs; ++i) HIP_TRY(hipGraphLaunch(gc.exec, B.stream));
+    }
+    if (n_steps % 2) {
+        HIP_TRY(one_step(B.stream, B.cur, fwd));
+        B.cur = 1 - B.cur;
+    }
+    std::vector<swe::Slots> res((size_t)B.nscen);
+    HIP_TRY(hipMemcpyAsync(res.data(), B.d_slots, sizeof(swe::Slots) * res.size(),
+                           hipMemcpyDeviceToHost, B.stream));
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    for (int32_t s = 0; s < B.nscen; ++s) {
+        time[s] = res[s].time;
+        if (dt_next) dt_next[s] = res[s].dt;
+    }
+    return SWE_OK;
+}
+
+int batch_set_stream(Batch &B, void *stream) {
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    B.stream = stream ? (hipStream_t)stream : B.own_stream;
+    return SWE_OK;
+}
+
+int batch_set_tuning(Batch &B, int chunk, int64_t grid) {
+    bool known = false;
+    for (int c : swe::CHUNK_SIZES) known = known || c == chunk;
+    if (!known) return fail(SWE_ERR_ARG, "chunk size %d not compiled (128, 256 or 512)", chunk);
+    if (grid < 0 || grid % 8) return fail(SWE_ERR_ARG, "grid %lld must be 0 or a multiple of 8", (long long)grid);
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    B.grid = grid;
+    if (chunk != B.chunk) {
+        B.chunk = chunk;
+        SWE_TRY(rebuild_chunks(B));
+    }
+    return SWE_OK;
 }
 
 }  // namespace
@@ -197,101 +544,20 @@ size_t swe_last_error(char *buf, size_t size) {
     return g_last_error.size();
 }
 
-int swe_chunk_cells(void) { return swe::CHUNK; }
+int swe_chunk_cells(void) { return swe::DEFAULT_CHUNK; }
+
+// ---- single mesh -----------------------------------------------------------
 
 int swe_create(const swe_mesh_desc *desc, swe_handle **out) {
     if (!desc || !out) return fail(SWE_ERR_ARG, "swe_create: null argument");
     *out = nullptr;
-    const int64_t n = desc->n_cells;
-    if (n <= 0 || n > (int64_t(1) << 29))
-        return fail(SWE_ERR_ARG, "swe_create: n_cells %lld out of range (1..2^29)", (long long)n);
-    if (!desc->depth || !desc->plotter || !desc->nbr)
-        return fail(SWE_ERR_ARG, "swe_create: null mesh array");
-    if (!(desc->scaling > 0.0) || !(desc->g > 0.0) || !(desc->dry_tolerance >= 0.0))
-        return fail(SWE_ERR_ARG, "swe_create: scaling and g must be > 0, dry_tolerance >= 0");
-    if (desc->solver != SWE_SOLVER_AUG_RIEMANN && desc->solver != SWE_SOLVER_FWAVE)
-        return fail(SWE_ERR_ARG, "swe_create: unknown solver %d", desc->solver);
-    if (desc->boundary != SWE_BC_OUTFLOW && desc->boundary != SWE_BC_WALL)
-        return fail(SWE_ERR_ARG, "swe_create: unknown boundary %d", desc->boundary);
-    if (desc->max_depth < 0 || desc->max_depth > 60)
-        return fail(SWE_ERR_ARG, "swe_create: max_depth %d out of range", desc->max_depth);
-
-    // validate the mesh and pack neighbours as (o << 2) | back edge
-    double normal[17 * 3 * 2], leg[64];
-    normal_table(normal);
-    leg_table(leg);
-    std::vector<int32_t> packed((size_t)(3 * n));
-    for (int64_t c = 0; c < n; ++c) {
-        const int d = desc->depth[c];
-        const int p = desc->plotter[c];
-        if (d < 0 || d > 60) return fail(SWE_ERR_ARG, "swe_create: depth[%lld] = %d", (long long)c, d);
-        if (p < -8 || p > 8 || p == 0)
-            return fail(SWE_ERR_ARG, "swe_create: plotter[%lld] = %d", (long long)c, p);
-        for (int k = 0; k < 3; ++k) {
-            const int32_t o = desc->nbr[3 * c + k];
-            if (o < 0) {
-                if (o != -1)
-                    return fail(SWE_ERR_ARG, "swe_create: nbr[%lld] = %d", (long long)(3 * c + k), o);
-                packed[3 * c + k] = -1;
-                continue;
-            }
-            if (o >= n || o == c)
-                return fail(SWE_ERR_ARG, "swe_create: nbr[%lld] = %d invalid", (long long)(3 * c + k), o);
-            int ko = -1;
-            for (int j = 0; j < 3; ++j)
-                if (desc->nbr[3 * (int64_t)o + j] == (int32_t)c) ko = j;
-            if (ko < 0)
-                return fail(SWE_ERR_ARG, "swe_create: neighbour relation not symmetric at cell %lld edge %d",
-                            (long long)c, k);
-            const int po = desc->plotter[o];
-            const double *nc = &normal[((p + 8) * 3 + k) * 2];
-            const double *no = &normal[((po + 8) * 3 + ko) * 2];
-            if (!(nc[0] == -no[0] && nc[1] == -no[1]))
-                return fail(SWE_ERR_ARG, "swe_create: cells %lld and %d disagree on the normal of their shared edge",
-                            (long long)c, o);
-            packed[3 * c + k] = (int32_t)((o << 2) | ko);
-        }
-    }
-
     swe_handle *h = new swe_handle();
-    h->n = n;
-    h->desc = *desc;
-    h->desc.depth = nullptr;
-    h->desc.plotter = nullptr;
-    h->desc.nbr = nullptr;
-    h->refine_thr = (5.0 * desc->scaling) * leg[desc->max_depth + 1];
-    hipError_t e = hipGetDevice(&h->device);
-    int rc = SWE_OK;
-    if (e == hipSuccess) rc = ensure_tables(h->device);
-    if (e != hipSuccess || rc != SWE_OK) {
-        release(h);
-        return e != hipSuccess ? fail(SWE_ERR_HIP, "hipGetDevice: %s", hipGetErrorString(e)) : rc;
-    }
-    const size_t bytes = sizeof(double) * (size_t)n;
-    bool ok = hipStreamCreate(&h->own_stream) == hipSuccess;
-    h->stream = h->own_stream;
-    for (int i = 0; i < 2 && ok; ++i)
-        ok = hipMalloc(&h->eta[i], bytes) == hipSuccess && hipMalloc(&h->hu[i], bytes) == hipSuccess &&
-             hipMalloc(&h->hv[i], bytes) == hipSuccess;
-    ok = ok && hipMalloc(&h->b, bytes) == hipSuccess &&
-         hipMalloc(&h->nbr, sizeof(int32_t) * 3 * (size_t)n) == hipSuccess &&
-         hipMalloc(&h->plotter, (size_t)n) == hipSuccess && hipMalloc(&h->depth, (size_t)n) == hipSuccess &&
-         hipMalloc(&h->refine, (size_t)n) == hipSuccess &&
-         hipMalloc(&h->slots, sizeof(swe::Slots)) == hipSuccess;
-    if (!ok) {
-        release(h);
-        return fail(SWE_ERR_NOMEM, "swe_create: device allocation of %lld cells failed", (long long)n);
-    }
-    ok = hipMemcpy(h->nbr, packed.data(), sizeof(int32_t) * 3 * (size_t)n, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(h->plotter, desc->plotter, (size_t)n, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(h->depth, desc->depth, (size_t)n, hipMemcpyHostToDevice) == hipSuccess;
-    swe::Slots init{};
-    const double huge_sr = 1.7976931348623157e308;
-    std::memcpy(&init.dtmin, &huge_sr, sizeof huge_sr);
-    ok = ok && hipMemcpy(h->slots, &init, sizeof init, hipMemcpyHostToDevice) == hipSuccess;
-    if (!ok) {
-        release(h);
-        return fail(SWE_ERR_HIP, "swe_create: topology upload failed");
+    int rc = batch_init(h->B, 1, desc);
+    if (rc != SWE_OK) {
+        std::string msg = "swe_create: " + g_last_error;
+        release(h->B);
+        delete h;
+        return fail(rc, "%s", msg.c_str());
     }
     *out = h;
     return SWE_OK;
@@ -299,119 +565,38 @@ int swe_create(const swe_mesh_desc *desc, swe_handle **out) {
 
 int swe_destroy(swe_handle *h) {
     if (!h) return SWE_OK;
-    (void)hipStreamSynchronize(h->stream);
-    release(h);
+    (void)hipStreamSynchronize(h->B.stream);
+    release(h->B);
+    delete h;
     return SWE_OK;
 }
 
 int swe_upload(swe_handle *h, const double *eta, const double *hu, const double *hv,
                const double *b) {
-    if (!h || !eta || !hu || !hv || !b) return fail(SWE_ERR_ARG, "swe_upload: null argument");
-    const size_t bytes = sizeof(double) * (size_t)h->n;
-    HIP_TRY(hipMemcpyAsync(h->eta[h->cur], eta, bytes, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->hu[h->cur], hu, bytes, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->hv[h->cur], hv, bytes, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->b, b, bytes, hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    h->has_state = true;
-    return SWE_OK;
+    if (!h) return fail(SWE_ERR_ARG, "swe_upload: null handle");
+    return batch_upload(h->B, 0, eta, hu, hv, b);
 }
 
 int swe_download(swe_handle *h, double *eta, double *hu, double *hv, double *b) {
     if (!h) return fail(SWE_ERR_ARG, "swe_download: null handle");
-    if (!h->has_state) return fail(SWE_ERR_STATE, "swe_download: no state uploaded");
-    const size_t bytes = sizeof(double) * (size_t)h->n;
-    if (eta) HIP_TRY(hipMemcpyAsync(eta, h->eta[h->cur], bytes, hipMemcpyDeviceToHost, h->stream));
-    if (hu) HIP_TRY(hipMemcpyAsync(hu, h->hu[h->cur], bytes, hipMemcpyDeviceToHost, h->stream));
-    if (hv) HIP_TRY(hipMemcpyAsync(hv, h->hv[h->cur], bytes, hipMemcpyDeviceToHost, h->stream));
-    if (b) HIP_TRY(hipMemcpyAsync(b, h->b, bytes, hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    return SWE_OK;
+    return batch_download(h->B, 0, eta, hu, hv, b);
 }
 
 int swe_step(swe_handle *h, double dt, int forward, double *dt_cfl, int64_t *n_refine,
              int8_t *refine) {
-    if (!h || !dt_cfl) return fail(SWE_ERR_ARG, "swe_step: null argument");
-    if (!h->has_state) return fail(SWE_ERR_STATE, "swe_step: no state uploaded");
-    const int c = h->cur, o = 1 - h->cur;
-    swe::StepArgs a = make_args(h, h->eta[c], h->hu[c], h->hv[c], h->b, h->eta[o], h->hu[o],
-                                h->hv[o], dt, forward);
-    a.refine = refine ? h->refine : nullptr;
-    HIP_TRY(swe::launch_step(a, h->desc.solver, h->stream));
-    swe::Slots res;
-    HIP_TRY(hipMemcpyAsync(&res, h->slots, sizeof res, hipMemcpyDeviceToHost, h->stream));
-    if (refine)
-        HIP_TRY(hipMemcpyAsync(refine, h->refine, (size_t)h->n, hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    *dt_cfl = res.res_dtcfl;
-    if (n_refine) *n_refine = (int64_t)res.res_nref;
-    h->cur = o;
-    return SWE_OK;
+    if (!h) return fail(SWE_ERR_ARG, "swe_step: null handle");
+    return batch_step(h->B, &dt, forward, dt_cfl, n_refine, refine);
 }
 
 int swe_run(swe_handle *h, int64_t n_steps, double dt0, int forward, double dt_clamp,
-            double *time_added, double *dt_next) {
-    if (!h || n_steps < 0) return fail(SWE_ERR_ARG, "swe_run: invalid argument");
-    if (!h->has_state) return fail(SWE_ERR_STATE, "swe_run: no state uploaded");
-    // the chain starts from dt0 and zero time; kernels then carry it
-    double init[2] = {dt0, 0.0};
-    if (dt_clamp > 0.0 && dt_clamp < dt0) init[0] = dt_clamp;
-    HIP_TRY(hipMemcpyAsync(&h->slots->dt, init, sizeof init, hipMemcpyHostToDevice, h->stream));
-
-    auto one_step = [&](hipStream_t s, int from, int fwd) -> hipError_t {
-        const int to = 1 - from;
-        swe::StepArgs a = make_args(h, h->eta[from], h->hu[from], h->hv[from], h->b, h->eta[to],
-                                    h->hu[to], h->hv[to], 0.0, fwd);
-        a.device_dt = 1;
-        a.dt_clamp = dt_clamp;
-        return swe::launch_step(a, h->desc.solver, s);
-    };
-
-    int fwd = forward ? 1 : 0;
-    const int64_t pairs = n_steps / 2;
-    if (pairs > 0) {
-        GraphCache &gc = h->graph;
-        if (!gc.exec || gc.cur != h->cur || gc.forward != fwd || gc.dt_clamp != dt_clamp) {
-            if (gc.exec) {
-                (void)hipGraphExecDestroy(gc.exec);
-                gc.exec = nullptr;
-            }
-            // capture on the library's own stream, replay on h->stream
-            HIP_TRY(hipStreamSynchronize(h->stream));
-            hipGraph_t graph;
-            HIP_TRY(hipStreamBeginCapture(h->own_stream, hipStreamCaptureModeThreadLocal));
-            hipError_t e1 = one_step(h->own_stream, h->cur, fwd);
-            hipError_t e2 = e1 == hipSuccess ? one_step(h->own_stream, 1 - h->cur, 1 - fwd) : e1;
-            hipError_t e3 = hipStreamEndCapture(h->own_stream, &graph);
-            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
-                return fail(SWE_ERR_HIP, "swe_run: graph capture failed: %s",
-                            hipGetErrorString(e1 != hipSuccess ? e1 : (e2 != hipSuccess ? e2 : e3)));
-            hipError_t e4 = hipGraphInstantiate(&gc.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e4 != hipSuccess) return fail(SWE_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e4));
-            gc.cur = h->cur;
-            gc.forward = fwd;
-            gc.dt_clamp = dt_clamp;
-        }
-        for (int64_t i = 0; i < pairs; ++i) HIP_TRY(hipGraphLaunch(gc.exec, h->stream));
-    }
-    if (n_steps % 2) {
-        HIP_TRY(one_step(h->stream, h->cur, fwd));
-        h->cur = 1 - h->cur;
-    }
-    double res[2];
-    HIP_TRY(hipMemcpyAsync(res, &h->slots->dt, sizeof res, hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    if (dt_next) *dt_next = res[0];
-    if (time_added) *time_added = res[1];
-    return SWE_OK;
+            double *time, double *dt_next) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_run: null handle");
+    return batch_run(h->B, n_steps, &dt0, forward, dt_clamp, time, dt_next);
 }
 
 int swe_set_stream(swe_handle *h, void *stream) {
     if (!h) return fail(SWE_ERR_ARG, "swe_set_stream: null handle");
-    HIP_TRY(hipStreamSynchronize(h->stream));
-    h->stream = stream ? (hipStream_t)stream : h->own_stream;
-    return SWE_OK;
+    return batch_set_stream(h->B, stream);
 }
 
 int swe_step_device(swe_handle *h, const double *eta, const double *hu, const double *hv,
@@ -422,13 +607,90 @@ int swe_step_device(swe_handle *h, const double *eta, const double *hu, const do
         return fail(SWE_ERR_ARG, "swe_step_device: null argument");
     if (eta_out == eta || hu_out == hu || hv_out == hv)
         return fail(SWE_ERR_ARG, "swe_step_device: outputs must not alias inputs");
-    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-    swe::StepArgs a = make_args(h, eta, hu, hv, b, eta_out, hu_out, hv_out, dt, forward);
+    Batch &B = h->B;
+    hipStream_t s = stream ? (hipStream_t)stream : B.stream;
+    swe::StepArgs a = make_args(B, 0, 1, forward);
+    a.eta = eta;
+    a.hu = hu;
+    a.hv = hv;
+    a.b = b;
+    a.eta_out = eta_out;
+    a.hu_out = hu_out;
+    a.hv_out = hv_out;
+    a.dt = dt;
     a.refine = refine_dev;
     a.out_dtcfl = dt_cfl_dev;
     a.out_nref = n_refine_dev;
-    HIP_TRY(swe::launch_step(a, h->desc.solver, s));
+    HIP_TRY(swe::launch_step(a, B.solver, B.chunk, B.grid, s));
     return SWE_OK;
 }
+
+// ---- batch of meshes -------------------------------------------------------
+
+int swe_batch_create(int32_t n_scenarios, const swe_mesh_desc *descs, swe_batch **out) {
+    if (!descs || !out || n_scenarios <= 0) return fail(SWE_ERR_ARG, "swe_batch_create: invalid argument");
+    *out = nullptr;
+    swe_batch *b = new swe_batch();
+    int rc = batch_init(b->B, n_scenarios, descs);
+    if (rc != SWE_OK) {
+        std::string msg = "swe_batch_create: " + g_last_error;
+        release(b->B);
+        delete b;
+        return fail(rc, "%s", msg.c_str());
+    }
+    *out = b;
+    return SWE_OK;
+}
+
+int swe_batch_destroy(swe_batch *b) {
+    if (!b) return SWE_OK;
+    (void)hipStreamSynchronize(b->B.stream);
+    release(b->B);
+    delete b;
+    return SWE_OK;
+}
+
+int swe_batch_upload(swe_batch *b, int32_t scenario, const double *eta, const double *hu,
+                     const double *hv, const double *bathymetry) {
+    if (!b) return fail(SWE_ERR_ARG, "swe_batch_upload: null batch");
+    return batch_upload(b->B, scenario, eta, hu, hv, bathymetry);
+}
+
+int swe_batch_download(swe_batch *b, int32_t scenario, double *eta, double *hu, double *hv,
+                       double *bathymetry) {
+    if (!b) return fail(SWE_ERR_ARG, "swe_batch_download: null batch");
+    return batch_download(b->B, scenario, eta, hu, hv, bathymetry);
+}
+
+int swe_batch_step(swe_batch *b, const double *dt, int forward, double *dt_cfl,
+                   int64_t *n_refine) {
+    if (!b) return fail(SWE_ERR_ARG, "swe_batch_step: null batch");
+    return batch_step(b->B, dt, forward, dt_cfl, n_refine, nullptr);
+}
+
+int swe_batch_run(swe_batch *b, int64_t n_steps, const double *dt0, int forward,
+                  double dt_clamp, double *time, double *dt_next) {
+    if (!b) return fail(SWE_ERR_ARG, "swe_batch_run: null batch");
+    return batch_run(b->B, n_steps, dt0, forward, dt_clamp, time, dt_next);
+}
+
+int swe_batch_set_tuning(swe_batch *b, int chunk_cells, int64_t grid_blocks) {
+    if (!b) return fail(SWE_ERR_ARG, "swe_batch_set_tuning: null batch");
+    return batch_set_tuning(b->B, chunk_cells, grid_blocks);
+}
+
+int swe_batch_get_tuning(swe_batch *b, int *chunk_cells, int64_t *grid_blocks) {
+    if (!b || !chunk_cells || !grid_blocks) return fail(SWE_ERR_ARG, "swe_batch_get_tuning: null argument");
+    *chunk_cells = b->B.chunk;
+    *grid_blocks = b->B.grid;
+    return SWE_OK;
+}
+
+int swe_batch_set_stream(swe_batch *b, void *stream) {
+    if (!b) return fail(SWE_ERR_ARG, "swe_batch_set_stream: null batch");
+    return batch_set_stream(b->B, stream);
+}
+
+int64_t swe_batch_cells(swe_batch *b) { return b ? b->B.ncells : -1; }
 
 }  // extern "C"

@@ -7,41 +7,62 @@
 
 namespace swe {
 
-constexpr int CHUNK = 256;  // cells per workgroup (four wave64s)
+// Chunk sizes (cells per workgroup, one thread per cell) the library is
+// compiled for; the batch autotuner picks one of them.
+constexpr int CHUNK_SIZES[3] = {128, 256, 512};
+constexpr int DEFAULT_CHUNK = 256;
 
-// Reduction slots of one step.  Invariant between launches: dtmin holds the
-// bits of huge(1.0), counter and nref are 0.  The last workgroup of a step
-// restores the invariant after publishing the results.
+// Per-scenario reduction slots.  Invariant between launches: dtmin holds the
+// bits of huge(1.0), counter and nref are 0.  The last workgroup of a
+// scenario's step restores the invariant after publishing the results.
 struct Slots {
     unsigned long long dtmin;    // running min of the CFL candidates (double bits)
     unsigned long long nref;     // running count of refinement flags
-    unsigned int counter;        // workgroups done
+    unsigned int counter;        // chunks of this scenario done
     unsigned int pad;
     double dt;                   // dt of the next step (device dt chain)
-    double time;                 // accumulated simulated time (device dt chain)
+    double time;                 // simulated time (device dt chain)
     double res_dtcfl;            // published CFL minimum of the last step
     unsigned long long res_nref; // published refinement count of the last step
+};
+
+// One scenario (mesh + constants) of a batch.  A single mesh is a batch of
+// one.  Scenarios of equal meshes share one copy of the topology.
+struct Scen {
+    int64_t cell_off;    // first cell in the state arrays
+    int64_t topo_off;    // first cell in the topology arrays
+    int64_t n;           // cells
+    int64_t chunk_off;   // first global chunk
+    int64_t nchunks;     // chunks of this scenario
+    double scaling, drytol, refine_thr, courant, dt_clamp;
+    int32_t depth;       // uniform depth, or -1 to read depth[] per cell
+    int32_t dmin, dmax;
+    int32_t pad;
 };
 
 struct StepArgs {
     const double *eta, *hu, *hv, *b;       // state at t_n, SoA, SFC order
     double *eta_out, *hu_out, *hv_out;     // state at t_{n+1}
-    const int32_t *nbr;                    // [3n] (o << 2) | back-edge, -1 boundary
-    const int8_t *plotter;                 // [n] forward plotter type
-    const int8_t *depth;                   // [n]
-    int8_t *refine;                        // [n] or null
-    Slots *slots;
-    double *out_dtcfl;                     // extra copy of the CFL minimum, or null
-    int64_t *out_nref;                     // extra copy of the refinement count, or null
-    int64_t n;
-    double dt;                             // used when device_dt == 0
-    double scaling, g, drytol, refine_thr;
-    double courant, dt_clamp;              // device dt chain: dt' = min(clamp, courant * min)
-    int dmin, dmax, bc, forward;
-    int device_dt;                         // 1: read dt from slots->dt and write the next one
+    const int32_t *nbr;                    // topology [3n] (o << 2) | back edge, -1 boundary
+    const int8_t *plotter;                 // topology [n] forward plotter type
+    const int8_t *depth;                   // topology [n] (used when Scen::depth < 0)
+    int8_t *refine;                        // state-indexed [n] or null
+    const Scen *scen;                      // [n_scen]
+    const int32_t *chunk_scen;             // [nchunks] scenario of each chunk, or null (one scenario)
+    Slots *slots;                          // [n_scen]
+    const double *dt_host;                 // [n_scen] dt per scenario when device_dt == 0, or null
+    double *out_dtcfl;                     // single-scenario extra copy of the CFL minimum, or null
+    int64_t *out_nref;                     // single-scenario extra copy of the refinement count, or null
+    int64_t nchunks;
+    double dt;                             // dt when device_dt == 0 and dt_host is null
+    double g, inv16g;                      // inv16g = 1 / (16 g), riemannsolvers.f:556
+    int bc, forward;
+    int device_dt;                         // 1: read dt from slots[s].dt and write the next one
 };
 
 hipError_t upload_tables(const double *normal, const double *leg);
-hipError_t launch_step(const StepArgs &a, int solver, hipStream_t s);
+// grid_blocks: number of workgroups (multiple of 8) or 0 for one per chunk.
+hipError_t launch_step(const StepArgs &a, int solver, int chunk, int64_t grid_blocks,
+                       hipStream_t s);
 
 }  // namespace swe

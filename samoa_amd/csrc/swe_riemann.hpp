@@ -27,39 +27,45 @@ __device__ __forceinline__ double dmin2(double a, double b) { return a < b ? a :
 __device__ __forceinline__ double dmax2(double a, double b) { return a > b ? a : b; }
 
 // riemannsolvers.f:507-618 (maxiter = 1); returns hm, s1m, s2m.
+// sqL = sqrt(g*hL) and sqR = sqrt(g*hR) are passed in: the reference
+// evaluates the same expressions here, so reusing the caller's values is
+// exact.  inv16g = 1/(16 g) (:556), computed once on the host.
 __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, double uR,
-                                            double drytol, double g, double &hm,
-                                            double &s1m, double &s2m) {
-    const double h_min = dmin2(hR, hL);
-    const double h_max = dmax2(hR, hL);
+                                            double sqL, double sqR, double drytol, double g,
+                                            double inv16g, double &hm, double &s1m,
+                                            double &s2m) {
+    const bool r_min = hR < hL;  // dmin2(hR, hL) picks hR
+    const double h_min = r_min ? hR : hL;
+    const double h_max = hR > hL ? hR : hL;
+    const double sqh_min = r_min ? sqR : sqL;
+    const double sqh_max = hR > hL ? sqR : sqL;
     const double delu = uR - uL;
     if (h_min <= drytol) {
         hm = 0.0;
-        s1m = ((uR + uL) - 2.0 * sqrt(g * hR)) + 2.0 * sqrt(g * hL);
+        s1m = ((uR + uL) - 2.0 * sqR) + 2.0 * sqL;
         s2m = s1m;
         return;
     }
-    const double sqh_min = sqrt(g * h_min), sqh_max = sqrt(g * h_max);
     const double F_min = delu + 2.0 * (sqh_min - sqh_max);
     const double F_max =
         delu + (h_max - h_min) * sqrt(((0.5 * g) * (h_max + h_min)) / (h_max * h_min));
-    const double sqL = sqrt(g * hL), sqR = sqrt(g * hR);
     if (F_min > 0.0) {  // two rarefactions
         const double t = dmax2(0.0, -delu + 2.0 * (sqL + sqR));
-        hm = (1.0 / (16.0 * g)) * (t * t);
+        hm = inv16g * (t * t);
         const double sqm = sqrt(g * hm);
         s1m = (uL + 2.0 * sqL) - 3.0 * sqm;
         s2m = (uR - 2.0 * sqR) + 3.0 * sqm;
     } else if (F_max <= 0.0) {  // two shocks: one Newton step on sqrt(h)
-        double h0 = h_max;
+        const double h0 = h_max;
         const double gL = sqrt((0.5 * g) * (1.0 / h0 + 1.0 / hL));
         const double gR = sqrt((0.5 * g) * (1.0 / h0 + 1.0 / hR));
         const double F0 = (delu + (h0 - hL) * gL) + (h0 - hR) * gR;
         const double h02 = h0 * h0;
         const double dfdh = ((gL - (g * (h0 - hL)) / ((4.0 * h02) * gL)) + gR) -
                             (g * (h0 - hR)) / ((4.0 * h02) * gR);
-        const double slope = (2.0 * sqrt(h0)) * dfdh;
-        const double q = sqrt(h0) - F0 / slope;
+        const double sq0 = sqrt(h0);
+        const double slope = (2.0 * sq0) * dfdh;
+        const double q = sq0 - F0 / slope;
         hm = q * q;
         const double u1m = uL - (hm - hL) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hL));
         const double u2m = uR + (hm - hR) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hR));
@@ -67,8 +73,8 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
         s1m = u1m - sqm;
         s2m = u2m + sqm;
     } else {  // one shock, one rarefaction: one secant step
-        double h0 = h_min;
-        const double F0 = (delu + 2.0 * (sqrt(g * h0) - sqh_max)) +
+        const double h0 = h_min;
+        const double F0 = (delu + 2.0 * (sqh_min - sqh_max)) +
                           (h0 - h_min) * sqrt((0.5 * g) * (1.0 / h0 + 1.0 / h_min));
         const double slope = (F_max - F0) / (h_max - h_min);
         hm = h0 - F0 / slope;
@@ -89,7 +95,7 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
 template <int SOLVER>
 __device__ __forceinline__ void geoclaw(double hL, double hR, double huL, double huR,
                                         double hvL, double hvR, double bL, double bR,
-                                        double drytol, double g, double netL[3],
+                                        double drytol, double g, double inv16g, double netL[3],
                                         double netR[3], double &maxs) {
     netL[0] = netL[1] = netL[2] = 0.0;
     netR[0] = netR[1] = netR[2] = 0.0;
@@ -121,7 +127,8 @@ __device__ __forceinline__ void geoclaw(double hL, double hR, double huL, double
     double wall0 = 1.0, wall1 = 1.0, wall2 = 1.0;
     if (hR <= drytol) {  // :87-100, right cell dry: wall test
         double hstar, t1, t2;
-        riemanntype(hL, hL, uL, -uL, drytol, g, hstar, t1, t2);
+        const double sq = sqrt(g * hL);
+        riemanntype(hL, hL, uL, -uL, sq, sq, drytol, g, inv16g, hstar, t1, t2);
         const double hstartest = dmax2(hL, hstar);
         if (hstartest + bL < bR) {
             wall1 = 0.0;
@@ -136,7 +143,8 @@ __device__ __forceinline__ void geoclaw(double hL, double hR, double huL, double
         }
     } else if (hL <= drytol) {  // :101-114
         double hstar, t1, t2;
-        riemanntype(hR, hR, -uR, uR, drytol, g, hstar, t1, t2);
+        const double sq = sqrt(g * hR);
+        riemanntype(hR, hR, -uR, uR, sq, sq, drytol, g, inv16g, hstar, t1, t2);
         const double hstartest = dmax2(hR, hstar);
         if (hstartest + bR < bL) {
             wall0 = 0.0;
@@ -188,7 +196,7 @@ __device__ __forceinline__ void geoclaw(double hL, double hR, double huL, double
     } else {  // riemann_aug_JCP, riemannsolvers.f:2-258
         const double delh = hR - hL;
         double hm, s1m, s2m;
-        riemanntype(hL, hR, uL, uR, drytol, g, hm, s1m, s2m);
+        riemanntype(hL, hR, uL, uR, sqL, sqR, drytol, g, inv16g, hm, s1m, s2m);
         const double l0 = dmin2(sE1, s2m);  // :57-60
         const double l2 = dmax2(sE2, s1m);
         sE1 = l0;
@@ -301,13 +309,14 @@ template <int SOLVER>
 __device__ __forceinline__ void edge_flux(double nx, double ny, double etaL, double huL,
                                           double hvL, double bL, double etaR, double huR,
                                           double hvR, double bR, double drytol, double g,
-                                          Flux &fL, Flux &fR) {
+                                          double inv16g, Flux &fL, Flux &fR) {
     const double pL1 = nx * huL + ny * hvL;
     const double pL2 = (-ny) * huL + nx * hvL;
     const double pR1 = nx * huR + ny * hvR;
     const double pR2 = (-ny) * huR + nx * hvR;
     double netL[3], netR[3], s;
-    geoclaw<SOLVER>(etaL - bL, etaR - bR, pL1, pR1, pL2, pR2, bL, bR, drytol, g, netL, netR, s);
+    geoclaw<SOLVER>(etaL - bL, etaR - bR, pL1, pR1, pL2, pR2, bL, bR, drytol, g, inv16g, netL,
+                    netR, s);
     fL.h = netL[0];
     fL.hu = netL[1] * nx + netL[2] * (-ny);
     fL.hv = netL[1] * ny + netL[2] * nx;

@@ -80,7 +80,7 @@ contains
         real(c_double), intent(inout)      :: dt, time
         logical, intent(inout)             :: forward
         integer(c_int64_t), intent(out)    :: steps
-        real(c_double) :: dt_cfl, added, clamp
+        real(c_double) :: dt_cfl, clamp
         integer(c_int64_t) :: nref
         integer(c_int) :: fw
 
@@ -91,9 +91,8 @@ contains
             if (tout > 0.0_c_double) clamp = tout
             if (tout > 0.0_c_double) dt = min(tout, dt)
             fw = merge(1_c_int, 0_c_int, forward)
-            rc = swe_run(handle, nmax, dt, fw, clamp, added, dt_cfl)
+            rc = swe_run(handle, nmax, dt, fw, clamp, time, dt_cfl)
             if (rc /= SWE_OK) return
-            time = time + added
             dt = dt_cfl
             steps = nmax
             if (mod(nmax, 2_c_int64_t) == 1) forward = .not. forward

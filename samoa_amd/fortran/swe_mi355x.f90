@@ -57,14 +57,15 @@ module swe_mi355x
             type(c_ptr), value          :: refine
         end function
 
-        integer(c_int) function swe_run(handle, n_steps, dt0, forward, dt_clamp, time_added, dt_next) bind(c, name='swe_run')
+        integer(c_int) function swe_run(handle, n_steps, dt0, forward, dt_clamp, time, dt_next) bind(c, name='swe_run')
             import :: c_int, c_ptr, c_double, c_int64_t
             type(c_ptr), value          :: handle
             integer(c_int64_t), value   :: n_steps
             real(c_double), value       :: dt0
             integer(c_int), value       :: forward
             real(c_double), value       :: dt_clamp
-            real(c_double), intent(out) :: time_added, dt_next
+            real(c_double), intent(inout) :: time
+            real(c_double), intent(out) :: dt_next
         end function
 
         integer(c_int) function swe_destroy(handle) bind(c, name='swe_destroy')

@@ -106,10 +106,12 @@ class EulerTimestep:
             _ptr(rf) if flags else None))
         return StepResult(dtc.value, nref.value, rf)
 
-    def run(self, n_steps: int, dt0: float, forward: bool, dt_clamp: float = -1.0):
-        """n_steps traversals with the dt chain on the device; returns
-        (time added, dt of the next step)."""
-        t = C.c_double()
+    def run(self, n_steps: int, dt0: float, forward: bool, dt_clamp: float = -1.0,
+            time: float = 0.0):
+        """n_steps traversals with the dt chain on the device, starting at
+        simulated time `time`; returns (time after the steps, dt of the next
+        step)."""
+        t = C.c_double(time)
         dtn = C.c_double()
         _native.check("swe_run", self._lib.swe_run(self._h, int(n_steps), float(dt0),
                                                    int(bool(forward)), float(dt_clamp),
@@ -143,6 +145,100 @@ class EulerTimestep:
 
     def __exit__(self, *exc):
         self.close()
+
+
+class Batch:
+    """Many frozen meshes advanced by one launch per step (swe_batch_* C ABI).
+
+    `meshes[s]` with constants `params[s]` (keyword arguments of
+    EulerTimestep).  Pass the same Mesh object for scenarios of equal meshes:
+    they then share one device copy of the topology."""
+
+    def __init__(self, meshes, params):
+        self._lib = _native.load()
+        if len(meshes) != len(params) or not meshes:
+            raise ValueError("one parameter set per mesh required")
+        self.meshes = list(meshes)
+        self._keep = []
+        descs = (_native.MeshDesc * len(meshes))()
+        arrays = {}
+        for i, (m, p) in enumerate(zip(meshes, params)):
+            if id(m) not in arrays:
+                arrays[id(m)] = (np.ascontiguousarray(m.depth, dtype=np.int8),
+                                 np.ascontiguousarray(m.plotter, dtype=np.int8),
+                                 np.ascontiguousarray(m.nbr, dtype=np.int32).reshape(-1))
+            dep, plo, nbr = arrays[id(m)]
+            descs[i] = _native.MeshDesc(
+                n_cells=m.n, depth=_ptr(dep), plotter=_ptr(plo), nbr=_ptr(nbr),
+                scaling=float(p.get("scaling", 1.0)), g=float(p.get("g", G)),
+                dry_tolerance=float(p.get("dry_tolerance", 0.01)),
+                courant=float(p.get("courant", 0.45)),
+                min_depth=int(p.get("min_depth", int(dep.min()))),
+                max_depth=int(p.get("max_depth", int(dep.max()))),
+                solver=int(p.get("solver", SOLVER_AUG_RIEMANN)),
+                boundary=int(p.get("boundary", BC_OUTFLOW)))
+        self._keep.append(arrays)
+        h = C.c_void_p()
+        _native.check("swe_batch_create",
+                      self._lib.swe_batch_create(len(meshes), C.cast(descs, C.c_void_p), C.byref(h)))
+        self._h = h
+        self.n_scen = len(meshes)
+        self.courant = np.array([float(p.get("courant", 0.45)) for p in params])
+        self.cells = int(self._lib.swe_batch_cells(h))
+
+    def upload(self, s: int, eta, hu, hv, b) -> None:
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (eta, hu, hv, b)]
+        _native.check("swe_batch_upload", self._lib.swe_batch_upload(self._h, s, *[_ptr(a) for a in arrs]))
+
+    def download(self, s: int):
+        n = self.meshes[s].n
+        out = [np.empty(n) for _ in range(4)]
+        _native.check("swe_batch_download",
+                      self._lib.swe_batch_download(self._h, s, *[_ptr(a) for a in out]))
+        return tuple(out)
+
+    def step(self, dt, forward: bool):
+        """One traversal of every scenario; returns (dt_cfl[s], n_refine[s])."""
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        dtc = np.empty(self.n_scen)
+        nref = np.empty(self.n_scen, np.int64)
+        _native.check("swe_batch_step", self._lib.swe_batch_step(
+            self._h, _ptr(dt), int(bool(forward)), _ptr(dtc), _ptr(nref)))
+        return dtc, nref
+
+    def run(self, n_steps: int, dt0, forward: bool, dt_clamp: float = -1.0, time=None):
+        """n_steps traversals with per-scenario device dt chains; returns
+        (time[s], dt_next[s])."""
+        dt0 = np.ascontiguousarray(dt0, dtype=np.float64)
+        t = np.zeros(self.n_scen) if time is None else np.array(time, dtype=np.float64)
+        dtn = np.empty(self.n_scen)
+        _native.check("swe_batch_run", self._lib.swe_batch_run(
+            self._h, int(n_steps), _ptr(dt0), int(bool(forward)), float(dt_clamp), _ptr(t),
+            _ptr(dtn)))
+        return t, dtn
+
+    def set_tuning(self, chunk: int, grid: int = 0) -> None:
+        _native.check("swe_batch_set_tuning", self._lib.swe_batch_set_tuning(self._h, chunk, grid))
+
+    def get_tuning(self):
+        c, g = C.c_int(), C.c_int64()
+        _native.check("swe_batch_get_tuning",
+                      self._lib.swe_batch_get_tuning(self._h, C.byref(c), C.byref(g)))
+        return c.value, g.value
+
+    def set_stream(self, stream: int) -> None:
+        _native.check("swe_batch_set_stream", self._lib.swe_batch_set_stream(self._h, stream or None))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.swe_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 @dataclass

@@ -112,7 +112,7 @@ def test_device_run_matches_host_chain():
     for a, b_ in zip(got, ref):
         assert np.array_equal(a, b_)
     # a second run continues from the same buffers and graph
-    t2, dt2 = op.run(4, dt_next, forward=fwd)
+    t2, dt2 = op.run(4, dt_next, forward=fwd, time=t_added)
     for _ in range(4):
         e, u, v, _, cand, _ = po.step(mesh.depth, mesh.plotter, mesh.nbr, ref, dt, scaling=S,
                                       drytol=1e-3, forward=fwd)
