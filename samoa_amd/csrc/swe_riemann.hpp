@@ -17,20 +17,24 @@
 
 #include <hip/hip_runtime.h>
 
+// Every solver function is usable on the host too, so that the adjoint can
+// be checked on the CPU; host sqrt is correctly rounded like the device's.
+#define SWE_HD __host__ __device__ __forceinline__
+
 namespace swe {
 
 struct Flux {
     double h, hu, hv, s;  // net update (h, hu, hv) and max wave speed
 };
 
-__device__ __forceinline__ double dmin2(double a, double b) { return a < b ? a : b; }
-__device__ __forceinline__ double dmax2(double a, double b) { return a > b ? a : b; }
+SWE_HD double dmin2(double a, double b) { return a < b ? a : b; }
+SWE_HD double dmax2(double a, double b) { return a > b ? a : b; }
 
 // riemannsolvers.f:507-618 (maxiter = 1); returns hm, s1m, s2m.
 // sqL = sqrt(g*hL) and sqR = sqrt(g*hR) are passed in: the reference
 // evaluates the same expressions here, so reusing the caller's values is
 // exact.  inv16g = 1/(16 g) (:556), computed once on the host.
-__device__ __forceinline__ void riemanntype(double hL, double hR, double uL, double uR,
+SWE_HD void riemanntype(double hL, double hR, double uL, double uR,
                                             double sqL, double sqR, double drytol, double g,
                                             double inv16g, double &hm, double &s1m,
                                             double &s2m) {
@@ -93,7 +97,7 @@ __device__ __forceinline__ void riemanntype(double hL, double hR, double uL, dou
 // frame: h = depth, hu = normal momentum, hv = tangential momentum.
 // net[0..2] left-going (A^- dQ), net[3..5] right-going (A^+ dQ).
 template <int SOLVER>
-__device__ __forceinline__ void geoclaw(double hL, double hR, double huL, double huR,
+SWE_HD void geoclaw(double hL, double hR, double huL, double huR,
                                         double hvL, double hvR, double bL, double bR,
                                         double drytol, double g, double inv16g, double netL[3],
                                         double netR[3], double &maxs) {
@@ -306,7 +310,7 @@ __device__ __forceinline__ void geoclaw(double hL, double hR, double huL, double
 // QL/QR = (eta, hu, hv, b) in world coordinates, n the left cell's outward
 // normal.
 template <int SOLVER>
-__device__ __forceinline__ void edge_flux(double nx, double ny, double etaL, double huL,
+SWE_HD void edge_flux(double nx, double ny, double etaL, double huL,
                                           double hvL, double bL, double etaR, double huR,
                                           double hvR, double bR, double drytol, double g,
                                           double inv16g, Flux &fL, Flux &fR) {
