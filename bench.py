@@ -12,7 +12,9 @@ range of that configuration; remeshing is not part of the timed step):
 `value` is this plain run.  The same line carries the batched-ensemble
 configuration (64 such scenarios from seeds 0-63 with uneven frozen depths
 drawn from 14-20, one launch per step, autotuned launch shape, 500 steps)
-under "ensemble".
+under "ensemble", and the differentiable inversion (depth-20 mesh, 200
+steps, 32 gauges, sqrt(200) checkpoints) with its adjoint/forward cost ratio
+under "adjoint".
 
 A step is one Euler traversal (flux on every edge, cell update with the
 wet/dry limiter, CFL reduction and the device-side dt update).  The K timed
@@ -41,10 +43,12 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=1000)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--workload", default="all", choices=["all", "tsunami", "ensemble"])
+    p.add_argument("--workload", default="all", choices=["all", "tsunami", "ensemble", "adjoint"])
     p.add_argument("--ens-steps", type=int, default=500)
     p.add_argument("--ens-scenarios", type=int, default=64)
     p.add_argument("--depth", type=int, default=21)
+    p.add_argument("--adj-depth", type=int, default=20)
+    p.add_argument("--adj-steps", type=int, default=200)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a 1 GiB device copy (counter calibration under rocprofv3)")
@@ -122,6 +126,92 @@ def ensemble(args, torch):
     }
 
 
+def adjoint(args, torch):
+    """Differentiable single inversion: the tsunami scenario frozen at a
+    uniform depth-20 mesh, 200 steps with the forward dt sequence, squared
+    eta misfit at 32 seeded gauges every step against a target run from a
+    perturbed source; gradients with respect to eta0 and b through the
+    discrete adjoint with ceil(sqrt(200)) checkpoints.
+
+    ratio = (backward sweep: recompute + adjoint) / (forward sweep), both over
+    the same steps; kernel_ratio = one adjoint launch / one forward launch."""
+    from samoa_amd import host
+    from samoa_amd.adjoint import GaugeInversion
+
+    d, N = args.adj_depth, args.adj_steps
+    mesh = host.mesh_uniform(d)
+    eta, hu, hv, b = host.tsunami(mesh, seed=0, drytol=1e-3)
+    eta_t = host.tsunami(mesh, seed=0, drytol=1e-3,
+                         amplitude=1.25 * host.tsunami_draw(0)["amplitude"])[0]
+    dt0 = host.initial_dt(mesh, (eta, hu, hv, b), 0.45)
+    gauges = np.random.default_rng(2024).choice(mesh.n, 32, replace=False)
+    dev = torch.device("cuda:0")
+    prob = GaugeInversion(mesh, gauges, N, dict(scaling=1.0e6, dry_tolerance=1e-3, courant=0.45))
+    e0 = torch.tensor(eta, device=dev)
+    bb = torch.tensor(b, device=dev)
+    prob.record_dt(e0, bb, dt0)
+    target = prob.gauge_series(torch.tensor(eta_t, device=dev), bb).detach()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    for rep in range(2):  # the first repetition warms up
+        e0g = e0.clone().requires_grad_()
+        bg = bb.clone().requires_grad_()
+        torch.cuda.synchronize()
+        ev[0].record()
+        loss = ((prob.gauge_series(e0g, bg) - target) ** 2).sum()
+        ev[1].record()
+        loss.backward()
+        ev[2].record()
+        torch.cuda.synchronize()
+    t_fwd = ev[0].elapsed_time(ev[1])
+    t_bwd = ev[1].elapsed_time(ev[2])
+
+    # single-launch costs: one forward traversal with masks vs one adjoint traversal
+    z = [torch.zeros_like(e0) for _ in range(9)]
+    q, out, lam = z[0:3], z[3:6], z[6:9]
+    q[0].copy_(e0)
+    masks = torch.empty(mesh.n, dtype=torch.int8, device=dev)
+    gb = torch.zeros_like(e0)
+    P = lambda t: t.data_ptr()  # noqa: E731
+    stream = torch.cuda.current_stream().cuda_stream
+    reps = 50
+
+    def fwd_launch():
+        prob.op.forward_device(P(q[0]), P(q[1]), P(q[2]), P(bb), P(out[0]), P(out[1]), P(out[2]),
+                               P(prob.dt), False, masks=P(masks), stream=stream)
+
+    def adj_launch():
+        prob.op.adjoint_device(P(q[0]), P(q[1]), P(q[2]), P(bb), P(masks), P(lam[0]), P(lam[1]),
+                               P(lam[2]), P(out[0]), P(out[1]), P(out[2]), P(gb), P(prob.dt),
+                               False, stream=stream)
+
+    times = {}
+    for name, fn in (("fwd", fwd_launch), ("adj", adj_launch)):
+        fn()
+        torch.cuda.synchronize()
+        ev[0].record()
+        for _ in range(reps):
+            fn()
+        ev[1].record()
+        torch.cuda.synchronize()
+        times[name] = ev[0].elapsed_time(ev[1]) / reps
+    g_e = e0g.grad
+    finite = bool(torch.isfinite(g_e).all()) and bool(torch.isfinite(bg.grad).all())
+    prob.close()
+    adj_bytes = 96 * mesh.n / (times["adj"] * 1e-3)
+    return {
+        "workload": f"tsunami_inversion_uniform_d{d}_{N}steps_32gauges",
+        "cells": mesh.n, "steps": N, "checkpoints": int(np.ceil(N / prob.K)), "K": prob.K,
+        "ms_forward_sweep": t_fwd, "ms_backward_sweep": t_bwd,
+        "ratio": t_bwd / t_fwd,
+        "ms_forward_launch": times["fwd"], "ms_adjoint_launch": times["adj"],
+        "kernel_ratio": times["adj"] / times["fwd"],
+        "loss": float(loss), "gradients_finite": finite,
+        "roofline_adjoint": {"bound": "hbm", "achieved": adj_bytes / 1e9, "peak": HBM_PEAK / 1e9,
+                             "unit": "GB/s", "frac": adj_bytes / HBM_PEAK,
+                             "bytes_per_cell_step": 96},
+    }
+
+
 import numpy as np  # noqa: E402
 
 
@@ -139,6 +229,9 @@ def main():
 
     if args.workload == "ensemble":
         print(json.dumps({"metric": METRIC, "ensemble": ensemble(args, torch)}), flush=True)
+        return
+    if args.workload == "adjoint":
+        print(json.dumps({"metric": METRIC, "adjoint": adjoint(args, torch)}), flush=True)
         return
 
     d = args.depth
@@ -221,6 +314,7 @@ def main():
     op.close()
     if args.workload == "all":
         out["ensemble"] = ensemble(args, torch)
+        out["adjoint"] = adjoint(args, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, state, dt0 * S)
     print(json.dumps(out), flush=True)

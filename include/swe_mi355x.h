@@ -117,6 +117,28 @@ int swe_step_device(swe_handle *h, const double *eta, const double *hu, const do
                     double dt, int forward, double *dt_cfl_dev, int64_t *n_refine_dev,
                     int8_t *refine_dev, void *stream);
 
+/* ---- Discrete adjoint ------------------------------------------------------
+ * No counterpart in the reference (the reference has no adjoint).  For the
+ * traversal Q^{n+1} = F(Q^n, b; dt_n) with dt_n and the wet/dry limiter
+ * decisions frozen from the forward pass, swe_adjoint_step_device computes
+ * lam^n = (dF/dQ^n)^T lam^{n+1} and adds (dF/db)^T lam^{n+1} to gb.
+ * All pointers are device addresses; asynchronous on `stream`.
+ *
+ * swe_forward_device: one traversal on caller buffers with dt read from
+ * device memory (*dt_dev), the CFL minimum written to *dt_cfl_dev (nullable)
+ * and the limiter masks (bit 0 wetting fix, bit 1 drying fix of
+ * cell_update_op, SWE_euler_timestep.f90:271-283) to masks_dev (nullable),
+ * for the recomputation sweeps of a checkpointed adjoint. */
+int swe_forward_device(swe_handle *h, const double *eta, const double *hu, const double *hv,
+                       const double *b, double *eta_out, double *hu_out, double *hv_out,
+                       const double *dt_dev, int forward, double *dt_cfl_dev, int8_t *masks_dev,
+                       void *stream);
+int swe_adjoint_step_device(swe_handle *h, const double *eta, const double *hu, const double *hv,
+                            const double *b, const int8_t *masks, const double *leta,
+                            const double *lhu, const double *lhv, double *leta_out,
+                            double *lhu_out, double *lhv_out, double *gb, const double *dt_dev,
+                            int forward, void *stream);
+
 /* Launch subsequent work of this handle on `stream` (a hipStream_t; NULL
  * restores the library's own stream), e.g. to time it with events the caller
  * records on that stream.  Waits for work already queued. */
@@ -166,6 +188,20 @@ int swe_batch_get_tuning(swe_batch *b, int *chunk_cells, int64_t *grid_blocks);
 int swe_batch_set_stream(swe_batch *b, void *stream);
 /* Total cells of all scenarios, or -1 for a NULL batch. */
 int64_t swe_batch_cells(swe_batch *b);
+/* First cell of a scenario in the concatenated state arrays, or -1. */
+int64_t swe_batch_offset(swe_batch *b, int32_t scenario);
+/* Batched forms of swe_forward_device / swe_adjoint_step_device over the
+ * concatenated state arrays of all scenarios; dt_dev[s] per scenario;
+ * dt_cfl_dev[s] (nullable) per scenario. */
+int swe_batch_forward_device(swe_batch *bt, const double *eta, const double *hu, const double *hv,
+                             const double *b, double *eta_out, double *hu_out, double *hv_out,
+                             const double *dt_dev, int forward, double *dt_cfl_dev,
+                             int8_t *masks_dev, void *stream);
+int swe_batch_adjoint_device(swe_batch *bt, const double *eta, const double *hu, const double *hv,
+                             const double *b, const int8_t *masks, const double *leta,
+                             const double *lhu, const double *lhv, double *leta_out,
+                             double *lhu_out, double *lhv_out, double *gb, const double *dt_dev,
+                             int forward, void *stream);
 int swe_batch_destroy(swe_batch *b);
 
 #ifdef __cplusplus

@@ -68,6 +68,11 @@ SIGNATURES = {
     "swe_batch_set_stream": (_I, [_P, _P]),
     "swe_batch_cells": (_I64, [_P]),
     "swe_batch_destroy": (_I, [_P]),
+    "swe_batch_offset": (_I64, [_P, C.c_int32]),
+    "swe_forward_device": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
+    "swe_adjoint_step_device": (_I, [_P] + [_P] * 13 + [_I, _P]),
+    "swe_batch_forward_device": (_I, [_P] + [_P] * 8 + [_I, _P, _P, _P]),
+    "swe_batch_adjoint_device": (_I, [_P] + [_P] * 13 + [_I, _P]),
 }
 
 _lib = None

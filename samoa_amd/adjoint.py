@@ -1,0 +1,190 @@
+"""Gradients of gauge time series through the SWE time loop (discrete adjoint).
+
+`GaugeInversion` freezes one forward run of the scheme: the mesh (or a batch
+of meshes advanced by one launch per step), the traversal directions
+(alternating, as the reference traverses, SWE.f90:393-403) and the dt
+sequence of that run (dt_{n+1} = courant * CFL minimum of step n, as
+SWE_euler_timestep.f90:340-343 and SWE.f90:395-398 chain it).
+`gauge_series(eta0, b)` then maps the initial surface elevation and the
+bathymetry (FP64 device tensors; concatenated over the scenarios of a batch)
+to the surface elevation at the gauge cells after every step, as a
+differentiable PyTorch operation:
+
+    G = problem.gauge_series(eta0, b)          # (n_steps, n_gauges)
+    loss = ((G - target) ** 2).sum()
+    loss.backward()                            # eta0.grad, b.grad
+
+The backward pass is the hand-written discrete adjoint
+(swe_adjoint_step_device): with dt and the wet/dry limiter decisions of the
+forward pass frozen, lam^n = (dF/dQ^n)^T lam^{n+1} + dL/dQ^n and
+gb += (dF/db)^T lam^{n+1}.  Memory is bounded by two-level checkpointing:
+the forward pass keeps every K-th state (K = ceil(sqrt(n_steps)) by default,
+so about 2 sqrt(n_steps) states are alive at once); the backward pass
+recomputes one segment of K states at a time, recording the limiter masks,
+and runs the adjoint through it backwards.  The recompute costs one extra
+forward sweep.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+
+from .swe import Batch, EulerTimestep, Mesh
+
+
+def _p(t: torch.Tensor) -> int:
+    return t.data_ptr()
+
+
+class GaugeInversion:
+    """A frozen forward problem: mesh(es), gauges, step count, dt sequence.
+
+    `mesh` is a Mesh (with the EulerTimestep keyword arguments in `params`)
+    or a list of Meshes with a list of such dicts (one scenario each).
+    Gauges index cells of the concatenated state."""
+
+    def __init__(self, mesh, gauges, n_steps: int, params=None, *, forward0: bool = False,
+                 checkpoint_every: int | None = None, device="cuda:0"):
+        if isinstance(mesh, Mesh):
+            params = dict(params or {})
+            self.op = EulerTimestep(mesh, **params)
+            self.n = mesh.n
+            self.n_scen = 1
+            courant = [params.get("courant", 0.45)]
+        else:
+            params = list(params)
+            self.op = Batch(mesh, params)
+            self.n = self.op.cells
+            self.n_scen = self.op.n_scen
+            courant = [p.get("courant", 0.45) for p in params]
+        self.device = torch.device(device)
+        self.courant = torch.tensor(courant, dtype=torch.float64, device=self.device)
+        self.n_steps = int(n_steps)
+        if self.n_steps < 1:
+            raise ValueError("n_steps must be positive")
+        gauges = np.asarray(gauges, np.int64)
+        if gauges.ndim != 1 or (gauges < 0).any() or (gauges >= self.n).any():
+            raise ValueError("gauges must be cell indices in [0, n)")
+        self.gauges = torch.as_tensor(gauges, device=self.device)
+        self.forward0 = bool(forward0)
+        self.K = int(checkpoint_every or max(1, math.ceil(math.sqrt(self.n_steps))))
+        self.dt = None  # frozen dt sequence, device [n_steps + 1, n_scen]
+
+    def direction(self, step: int) -> bool:
+        """Traversal direction of step `step` (0-based): alternating."""
+        return self.forward0 ^ (step % 2 == 1)
+
+    def _zeros(self):
+        return torch.zeros(self.n, dtype=torch.float64, device=self.device)
+
+    def _stream(self):
+        return torch.cuda.current_stream(self.device).cuda_stream
+
+    def _dt(self, step):
+        return _p(self.dt) + 8 * self.n_scen * step
+
+    def _check(self, t, name):
+        if t.dtype != torch.float64 or t.device != self.device or t.numel() != self.n:
+            raise ValueError(f"{name} must be a float64 tensor of {self.n} cells on {self.device}")
+
+    def _fwd(self, q, out, b, step, masks=None, cfl=None):
+        self.op.forward_device(_p(q[0]), _p(q[1]), _p(q[2]), _p(b), _p(out[0]), _p(out[1]),
+                               _p(out[2]), self._dt(step), self.direction(step),
+                               dt_cfl=_p(cfl[step]) if cfl is not None else 0,
+                               masks=_p(masks) if masks is not None else 0,
+                               stream=self._stream())
+
+    def record_dt(self, eta0: torch.Tensor, b: torch.Tensor, dt0) -> torch.Tensor:
+        """Runs the scheme once from (eta0, p = 0, b) with the reference's dt
+        chain evaluated on the device and freezes the dt sequence it
+        produces; returns it as [n_steps, n_scen]."""
+        self._check(eta0, "eta0")
+        self._check(b, "b")
+        self.dt = torch.empty(self.n_steps + 1, self.n_scen, dtype=torch.float64, device=self.device)
+        self.dt[0] = torch.as_tensor(np.broadcast_to(np.asarray(dt0, np.float64), (self.n_scen,)).copy())
+        cfl = torch.empty(self.n_steps, self.n_scen, dtype=torch.float64, device=self.device)
+        b = b.detach().contiguous()
+        q = [eta0.detach().contiguous().clone(), self._zeros(), self._zeros()]
+        out = [self._zeros() for _ in range(3)]
+        for n in range(self.n_steps):
+            self._fwd(q, out, b, n, cfl=cfl)
+            torch.mul(self.courant, cfl[n], out=self.dt[n + 1])
+            q, out = out, q
+        torch.cuda.synchronize(self.device)
+        return self.dt[: self.n_steps]
+
+    def gauge_series(self, eta0: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+        if self.dt is None:
+            raise RuntimeError("record_dt() first: the dt sequence must be frozen")
+        self._check(eta0, "eta0")
+        self._check(b, "b")
+        return _GaugeSeries.apply(eta0, b, self)
+
+    # -- used by the autograd function ------------------------------------
+    def run_forward(self, eta0, b):
+        """Forward sweep: gauge values after each step and the checkpoints
+        Q^{jK} (j = 0, 1, ...)."""
+        q = [eta0.detach().contiguous().clone(), self._zeros(), self._zeros()]
+        out = [self._zeros() for _ in range(3)]
+        series = torch.empty(self.n_steps, len(self.gauges), dtype=torch.float64, device=self.device)
+        checkpoints = {}
+        for n in range(self.n_steps):
+            if n % self.K == 0:
+                checkpoints[n] = [x.clone() for x in q]
+            self._fwd(q, out, b, n)
+            torch.index_select(out[0], 0, self.gauges, out=series[n])
+            q, out = out, q
+        return series, checkpoints
+
+    def run_backward(self, b, checkpoints, g_series):
+        """Adjoint sweep; returns (d/d eta0, d/d b)."""
+        lam = [self._zeros() for _ in range(3)]
+        lam_out = [self._zeros() for _ in range(3)]
+        gb = self._zeros()
+        lam[0].index_add_(0, self.gauges, g_series[self.n_steps - 1])
+        starts = sorted(checkpoints)
+        masks = [torch.empty(self.n, dtype=torch.int8, device=self.device) for _ in range(self.K)]
+        states = [[self._zeros() for _ in range(3)] for _ in range(self.K)]
+        spare = [self._zeros() for _ in range(3)]
+        for j in reversed(range(len(starts))):
+            s0 = starts[j]
+            s1 = starts[j + 1] if j + 1 < len(starts) else self.n_steps
+            # recompute the segment's states Q^{s0..s1-1} and the masks of its steps
+            for x, y in zip(states[0], checkpoints[s0]):
+                x.copy_(y)
+            for n in range(s0, s1):
+                i = n - s0
+                dst = states[i + 1] if i + 1 < s1 - s0 else spare
+                self._fwd(states[i], dst, b, n, masks=masks[i])
+            for n in reversed(range(s0, s1)):
+                q = states[n - s0]
+                self.op.adjoint_device(
+                    _p(q[0]), _p(q[1]), _p(q[2]), _p(b), _p(masks[n - s0]),
+                    _p(lam[0]), _p(lam[1]), _p(lam[2]), _p(lam_out[0]), _p(lam_out[1]),
+                    _p(lam_out[2]), _p(gb), self._dt(n), self.direction(n), stream=self._stream())
+                lam, lam_out = lam_out, lam
+                if n >= 1:
+                    lam[0].index_add_(0, self.gauges, g_series[n - 1])
+        return lam[0].clone(), gb
+
+    def close(self):
+        self.op.close()
+
+
+class _GaugeSeries(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, eta0, b, problem):
+        b = b.detach().contiguous()
+        series, checkpoints = problem.run_forward(eta0, b)
+        ctx.problem = problem
+        ctx.checkpoints = checkpoints
+        ctx.save_for_backward(b)
+        return series
+
+    @staticmethod
+    def backward(ctx, g_series):
+        (b,) = ctx.saved_tensors
+        g_eta0, g_b = ctx.problem.run_backward(b, ctx.checkpoints, g_series.contiguous())
+        return g_eta0, g_b, None

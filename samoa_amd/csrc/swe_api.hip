@@ -115,6 +115,7 @@ int ensure_tables(int device) {
     normal_table(normal);
     leg_table(leg);
     HIP_TRY(swe::upload_tables(normal, leg));
+    HIP_TRY(swe::upload_tables_adjoint(normal, leg));
     g_tables_on_device.push_back(device);
     return SWE_OK;
 }
@@ -512,6 +513,77 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
     return SWE_OK;
 }
 
+// Forward traversal on caller-owned device buffers (all scenarios), dt read
+// from device memory per scenario, limiter masks recorded (nullable).
+int batch_forward_device(Batch &B, const double *eta, const double *hu, const double *hv,
+                         const double *b, double *eta_out, double *hu_out, double *hv_out,
+                         const double *dt_dev, int forward, double *dt_cfl_dev, int8_t *masks,
+                         void *stream) {
+    if (!eta || !hu || !hv || !b || !eta_out || !hu_out || !hv_out || !dt_dev)
+        return fail(SWE_ERR_ARG, "forward_device: null argument");
+    if (eta_out == eta || hu_out == hu || hv_out == hv)
+        return fail(SWE_ERR_ARG, "forward_device: outputs must not alias inputs");
+    hipStream_t st = stream ? (hipStream_t)stream : B.stream;
+    swe::StepArgs a = make_args(B, 0, 1, forward);
+    a.eta = eta;
+    a.hu = hu;
+    a.hv = hv;
+    a.b = b;
+    a.eta_out = eta_out;
+    a.hu_out = hu_out;
+    a.hv_out = hv_out;
+    a.dt_host = dt_dev;
+    a.masks = masks;
+    a.out_dtcfl = B.nscen == 1 ? dt_cfl_dev : nullptr;
+    HIP_TRY(swe::launch_step(a, B.solver, B.chunk, B.grid, st));
+    if (dt_cfl_dev && B.nscen > 1) {
+        // per-scenario CFL minima are published in the slots by the last chunk
+        for (int32_t s = 0; s < B.nscen; ++s)
+            HIP_TRY(hipMemcpyAsync(dt_cfl_dev + s, &B.d_slots[s].res_dtcfl, sizeof(double),
+                                   hipMemcpyDeviceToDevice, st));
+    }
+    return SWE_OK;
+}
+
+int batch_adjoint_device(Batch &B, const double *eta, const double *hu, const double *hv,
+                         const double *b, const int8_t *masks, const double *leta,
+                         const double *lhu, const double *lhv, double *leta_out, double *lhu_out,
+                         double *lhv_out, double *gb, const double *dt_dev, int forward,
+                         void *stream) {
+    if (!eta || !hu || !hv || !b || !masks || !leta || !lhu || !lhv || !leta_out || !lhu_out ||
+        !lhv_out || !gb || !dt_dev)
+        return fail(SWE_ERR_ARG, "adjoint_device: null argument");
+    if (leta_out == leta || lhu_out == lhu || lhv_out == lhv)
+        return fail(SWE_ERR_ARG, "adjoint_device: outputs must not alias inputs");
+    hipStream_t st = stream ? (hipStream_t)stream : B.stream;
+    swe::AdjArgs a{};
+    a.eta = eta;
+    a.hu = hu;
+    a.hv = hv;
+    a.b = b;
+    a.masks = masks;
+    a.leta = leta;
+    a.lhu = lhu;
+    a.lhv = lhv;
+    a.leta_out = leta_out;
+    a.lhu_out = lhu_out;
+    a.lhv_out = lhv_out;
+    a.gb = gb;
+    a.nbr = B.nbr;
+    a.plotter = B.plotter;
+    a.depth = B.depth;
+    a.scen = B.d_scen;
+    a.chunk_scen = B.d_chunk_scen;
+    a.nchunks = B.nchunks;
+    a.dt_dev = dt_dev;
+    a.g = B.g;
+    a.inv16g = 1.0 / (16.0 * B.g);
+    a.bc = B.bc;
+    a.forward = forward ? 1 : 0;
+    HIP_TRY(swe::launch_adjoint(a, B.solver, B.chunk, B.grid, st));
+    return SWE_OK;
+}
+
 int batch_set_stream(Batch &B, void *stream) {
     HIP_TRY(hipStreamSynchronize(B.stream));
     B.stream = stream ? (hipStream_t)stream : B.own_stream;
@@ -625,6 +697,25 @@ int swe_step_device(swe_handle *h, const double *eta, const double *hu, const do
     return SWE_OK;
 }
 
+int swe_forward_device(swe_handle *h, const double *eta, const double *hu, const double *hv,
+                       const double *b, double *eta_out, double *hu_out, double *hv_out,
+                       const double *dt_dev, int forward, double *dt_cfl_dev, int8_t *masks_dev,
+                       void *stream) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_forward_device: null handle");
+    return batch_forward_device(h->B, eta, hu, hv, b, eta_out, hu_out, hv_out, dt_dev, forward,
+                                dt_cfl_dev, masks_dev, stream);
+}
+
+int swe_adjoint_step_device(swe_handle *h, const double *eta, const double *hu, const double *hv,
+                            const double *b, const int8_t *masks, const double *leta,
+                            const double *lhu, const double *lhv, double *leta_out,
+                            double *lhu_out, double *lhv_out, double *gb, const double *dt_dev,
+                            int forward, void *stream) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_adjoint_step_device: null handle");
+    return batch_adjoint_device(h->B, eta, hu, hv, b, masks, leta, lhu, lhv, leta_out, lhu_out,
+                                lhv_out, gb, dt_dev, forward, stream);
+}
+
 // ---- batch of meshes -------------------------------------------------------
 
 int swe_batch_create(int32_t n_scenarios, const swe_mesh_desc *descs, swe_batch **out) {
@@ -692,5 +783,29 @@ int swe_batch_set_stream(swe_batch *b, void *stream) {
 }
 
 int64_t swe_batch_cells(swe_batch *b) { return b ? b->B.ncells : -1; }
+
+int swe_batch_forward_device(swe_batch *bt, const double *eta, const double *hu, const double *hv,
+                             const double *b, double *eta_out, double *hu_out, double *hv_out,
+                             const double *dt_dev, int forward, double *dt_cfl_dev,
+                             int8_t *masks_dev, void *stream) {
+    if (!bt) return fail(SWE_ERR_ARG, "swe_batch_forward_device: null batch");
+    return batch_forward_device(bt->B, eta, hu, hv, b, eta_out, hu_out, hv_out, dt_dev, forward,
+                                dt_cfl_dev, masks_dev, stream);
+}
+
+int swe_batch_adjoint_device(swe_batch *bt, const double *eta, const double *hu, const double *hv,
+                             const double *b, const int8_t *masks, const double *leta,
+                             const double *lhu, const double *lhv, double *leta_out,
+                             double *lhu_out, double *lhv_out, double *gb, const double *dt_dev,
+                             int forward, void *stream) {
+    if (!bt) return fail(SWE_ERR_ARG, "swe_batch_adjoint_device: null batch");
+    return batch_adjoint_device(bt->B, eta, hu, hv, b, masks, leta, lhu, lhv, leta_out, lhu_out,
+                                lhv_out, gb, dt_dev, forward, stream);
+}
+
+int64_t swe_batch_offset(swe_batch *b, int32_t scenario) {
+    if (!b || scenario < 0 || scenario >= b->B.nscen) return -1;
+    return b->B.scen[scenario].cell_off;
+}
 
 }  // extern "C"

@@ -47,6 +47,8 @@ struct StepArgs {
     const int8_t *plotter;                 // topology [n] forward plotter type
     const int8_t *depth;                   // topology [n] (used when Scen::depth < 0)
     int8_t *refine;                        // state-indexed [n] or null
+    int8_t *masks;                         // state-indexed [n] limiter masks (bit 0 wetting,
+                                           // bit 1 drying fix taken) or null
     const Scen *scen;                      // [n_scen]
     const int32_t *chunk_scen;             // [nchunks] scenario of each chunk, or null (one scenario)
     Slots *slots;                          // [n_scen]
@@ -60,7 +62,29 @@ struct StepArgs {
     int device_dt;                         // 1: read dt from slots[s].dt and write the next one
 };
 
+// Discrete adjoint of one traversal (swe_adjoint_step.hip).
+struct AdjArgs {
+    const double *eta, *hu, *hv, *b;        // state at t_n
+    const int8_t *masks;                    // limiter masks of the forward traversal of t_n
+    const double *leta, *lhu, *lhv;         // adjoint of the state at t_{n+1}
+    double *leta_out, *lhu_out, *lhv_out;   // adjoint of the state at t_n
+    double *gb;                             // adjoint of b, accumulated (+=)
+    const int32_t *nbr;
+    const int8_t *plotter;
+    const int8_t *depth;
+    const Scen *scen;
+    const int32_t *chunk_scen;
+    int64_t nchunks;
+    const double *dt_dev;                   // [n_scen] dt per scenario, or null: use dt
+    double dt;
+    double g, inv16g;
+    int bc, forward;
+};
+
 hipError_t upload_tables(const double *normal, const double *leg);
+hipError_t upload_tables_adjoint(const double *normal, const double *leg);
+hipError_t launch_adjoint(const AdjArgs &a, int solver, int chunk, int64_t grid_blocks,
+                          hipStream_t s);
 // grid_blocks: number of workgroups (multiple of 8) or 0 for one per chunk.
 hipError_t launch_step(const StepArgs &a, int solver, int chunk, int64_t grid_blocks,
                        hipStream_t s);

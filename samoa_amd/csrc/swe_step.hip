@@ -234,10 +234,12 @@ __global__ __launch_bounds__(CH) void euler_step_kernel(StepArgs a) {
 
             double qh = s_q[0][t], q1 = s_q[1][t], q2 = s_q[2][t];
             const double qb = s_q[3][t];
+            int mk = 0;
             if (qh < qb + drytol && dh > 0.0) {  // wetting, :271-275
                 qh = qb + drytol;
                 q1 = 0.0;
                 q2 = 0.0;
+                mk = 1;
             }
             qh = qh + dh;
             q1 = q1 + d1;
@@ -246,7 +248,9 @@ __global__ __launch_bounds__(CH) void euler_step_kernel(StepArgs a) {
                 qh = dmin2(qb, 0.0);
                 q1 = 0.0;
                 q2 = 0.0;
+                mk |= 2;
             }
+            if (a.masks) a.masks[gs + lc_t] = (int8_t)mk;
             a.eta_out[gs + lc_t] = qh;
             a.hu_out[gs + lc_t] = q1;
             a.hv_out[gs + lc_t] = q2;

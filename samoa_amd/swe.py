@@ -125,6 +125,24 @@ class EulerTimestep:
             self._h, eta, hu, hv, b, eta_out, hu_out, hv_out, float(dt), int(bool(forward)),
             dt_cfl, n_refine or None, refine or None, stream or None))
 
+    def forward_device(self, eta, hu, hv, b, eta_out, hu_out, hv_out, dt_dev, forward,
+                       dt_cfl=0, masks=0, stream=0) -> None:
+        """Traversal with dt read from device memory (dt_dev), optionally
+        recording the limiter masks the adjoint needs.  Integers are device
+        addresses."""
+        _native.check("swe_forward_device", self._lib.swe_forward_device(
+            self._h, eta, hu, hv, b, eta_out, hu_out, hv_out, dt_dev, int(bool(forward)),
+            dt_cfl or None, masks or None, stream or None))
+
+    def adjoint_device(self, eta, hu, hv, b, masks, leta, lhu, lhv, leta_out, lhu_out, lhv_out,
+                       gb, dt_dev, forward, stream=0) -> None:
+        """Discrete adjoint of forward_device: (leta_out, lhu_out, lhv_out) =
+        (dQ'/dQ)^T (leta, lhu, lhv) and gb += (dQ'/db)^T (leta, lhu, lhv), with
+        dt and the masks frozen."""
+        _native.check("swe_adjoint_step_device", self._lib.swe_adjoint_step_device(
+            self._h, eta, hu, hv, b, masks, leta, lhu, lhv, leta_out, lhu_out, lhv_out, gb,
+            dt_dev, int(bool(forward)), stream or None))
+
     def set_stream(self, stream: int) -> None:
         """Launch on a caller's HIP stream (0 = the library's own)."""
         _native.check("swe_set_stream", self._lib.swe_set_stream(self._h, stream or None))
@@ -216,6 +234,24 @@ class Batch:
             self._h, int(n_steps), _ptr(dt0), int(bool(forward)), float(dt_clamp), _ptr(t),
             _ptr(dtn)))
         return t, dtn
+
+    def offset(self, s: int) -> int:
+        """First cell of scenario s in the batch's concatenated state arrays."""
+        return int(self._lib.swe_batch_offset(self._h, s))
+
+    def forward_device(self, eta, hu, hv, b, eta_out, hu_out, hv_out, dt_dev, forward,
+                       dt_cfl=0, masks=0, stream=0) -> None:
+        """One traversal of every scenario over caller-owned concatenated
+        arrays; dt_dev holds one dt per scenario (device addresses)."""
+        _native.check("swe_batch_forward_device", self._lib.swe_batch_forward_device(
+            self._h, eta, hu, hv, b, eta_out, hu_out, hv_out, dt_dev, int(bool(forward)),
+            dt_cfl or None, masks or None, stream or None))
+
+    def adjoint_device(self, eta, hu, hv, b, masks, leta, lhu, lhv, leta_out, lhu_out, lhv_out,
+                       gb, dt_dev, forward, stream=0) -> None:
+        _native.check("swe_batch_adjoint_device", self._lib.swe_batch_adjoint_device(
+            self._h, eta, hu, hv, b, masks, leta, lhu, lhv, leta_out, lhu_out, lhv_out, gb,
+            dt_dev, int(bool(forward)), stream or None))
 
     def set_tuning(self, chunk: int, grid: int = 0) -> None:
         _native.check("swe_batch_set_tuning", self._lib.swe_batch_set_tuning(self._h, chunk, grid))
