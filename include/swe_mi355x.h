@@ -59,7 +59,7 @@ typedef struct swe_mesh_desc {
     const int32_t *nbr;     /* [3*n_cells] */
     double scaling;         /* cfg%scaling: edge length of the unit square in metres */
     double g;               /* gravitational constant (SWE_data_types.f90:34: 9.80665) */
-    double dry_tolerance;   /* cfg%dry_tolerance */
+    double dry_tolerance;   /* cfg%dry_tolerance, >= 1e-100 */
     double courant;         /* cfg%courant_number (applied by swe_run only) */
     int32_t min_depth;      /* cfg%i_min_depth, for the refinement flags */
     int32_t max_depth;      /* cfg%i_max_depth */

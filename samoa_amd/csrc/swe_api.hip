@@ -21,6 +21,7 @@
 
 #include "swe_internal.hpp"
 #include "swe_mi355x.h"
+#include "swe_riemann.hpp"
 
 namespace {
 
@@ -286,8 +287,9 @@ int batch_init(Batch &B, int32_t nscen, const swe_mesh_desc *descs) {
                         (long long)n);
         if (!d.depth || !d.plotter || !d.nbr)
             return fail(SWE_ERR_ARG, "scenario %d: null mesh array", s);
-        if (!(d.scaling > 0.0) || !(d.g > 0.0) || !(d.dry_tolerance >= 0.0))
-            return fail(SWE_ERR_ARG, "scenario %d: scaling and g must be > 0, dry_tolerance >= 0", s);
+        if (!(d.scaling > 0.0) || !(d.g > 0.0) || !(d.dry_tolerance >= swe::MIN_DRY_TOLERANCE))
+            return fail(SWE_ERR_ARG, "scenario %d: scaling and g must be > 0, dry_tolerance >= %g",
+                        s, swe::MIN_DRY_TOLERANCE);
         if (d.solver != SWE_SOLVER_AUG_RIEMANN && d.solver != SWE_SOLVER_FWAVE)
             return fail(SWE_ERR_ARG, "scenario %d: unknown solver %d", s, d.solver);
         if (d.boundary != SWE_BC_OUTFLOW && d.boundary != SWE_BC_WALL)

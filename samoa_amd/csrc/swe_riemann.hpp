@@ -30,6 +30,11 @@ struct Flux {
 SWE_HD double dmin2(double a, double b) { return a < b ? a : b; }
 SWE_HD double dmax2(double a, double b) { return a > b ? a : b; }
 
+// Smallest dry tolerance the solver accepts: above it every depth the solver
+// divides by is a normal number whose square and reciprocal are finite, which
+// the exact-zero arguments of wall_hstar and riemanntype rely on.
+constexpr double MIN_DRY_TOLERANCE = 1.0e-100;
+
 // riemannsolvers.f:507-618 (maxiter = 1); returns hm, s1m, s2m.
 // sqL = sqrt(g*hL) and sqR = sqrt(g*hR) are passed in: the reference
 // evaluates the same expressions here, so reusing the caller's values is
@@ -77,9 +82,12 @@ SWE_HD void riemanntype(double hL, double hR, double uL, double uR,
         s1m = u1m - sqm;
         s2m = u2m + sqm;
     } else {  // one shock, one rarefaction: one secant step
+        // The reference evaluates F0 at h0 = h_min as
+        //   (delu + 2 (sqrt(g h_min) - sqrt(g h_max))) + (h0 - h_min) sqrt(g/2 (1/h0 + 1/h_min)),
+        // whose second term is (+0) * (a finite value, since h_min > drytol >=
+        // MIN_DRY_TOLERANCE) = +0; adding +0 is exact, so F0 = F_min + 0.
         const double h0 = h_min;
-        const double F0 = (delu + 2.0 * (sqh_min - sqh_max)) +
-                          (h0 - h_min) * sqrt((0.5 * g) * (1.0 / h0 + 1.0 / h_min));
+        const double F0 = F_min + 0.0;
         const double slope = (F_max - F0) / (h_max - h_min);
         hm = h0 - F0 / slope;
         const double sqm = sqrt(g * hm);
@@ -91,6 +99,30 @@ SWE_HD void riemanntype(double hL, double hR, double uL, double uR,
             s1m = (uR - 2.0 * sqR) + sqm;
         }
     }
+}
+
+// hm of riemanntype(h, h, a, c) for the wall test of a wet cell of depth h
+// against a dry neighbour (c_bind_riemannsolvers.f90:88,102).  With equal
+// depths both sides' sqrt(g h) are equal, so the reference's terms
+// 2 (sq - sq), (h - h) * sqrt(...) and g (h0 - h) / (...) are exact zeros
+// (their other factors are finite because h > drytol >= MIN_DRY_TOLERANCE):
+// F_min = F_max = delu + 0, the shock-rarefaction branch is unreachable, and
+// in the two-shock branch gL = gR, F0 = delu + 0, dfdh = gL + gL.  s1m and
+// s2m are not needed by the wall test.
+SWE_HD double wall_hstar(double h, double a, double c, double sq, double g, double inv16g) {
+    const double delu = c - a;
+    const double F = delu + 0.0;
+    if (F > 0.0) {  // two rarefactions
+        const double t = dmax2(0.0, -delu + 2.0 * (sq + sq));
+        return inv16g * (t * t);
+    }
+    const double inv = 1.0 / h;  // two shocks, h0 = hL = hR = h
+    const double gL = sqrt((0.5 * g) * (inv + inv));
+    const double dfdh = gL + gL;
+    const double sq0 = sqrt(h);
+    const double slope = (2.0 * sq0) * dfdh;
+    const double q = sq0 - F / slope;
+    return q * q;
 }
 
 // c_bind_geoclaw_solver_dp (c_bind_riemannsolvers.f90:16-176) in the edge
@@ -129,37 +161,38 @@ SWE_HD void geoclaw(double hL, double hR, double huL, double huR,
     }
 
     double wall0 = 1.0, wall1 = 1.0, wall2 = 1.0;
-    if (hR <= drytol) {  // :87-100, right cell dry: wall test
-        double hstar, t1, t2;
-        const double sq = sqrt(g * hL);
-        riemanntype(hL, hL, uL, -uL, sq, sq, drytol, g, inv16g, hstar, t1, t2);
-        const double hstartest = dmax2(hL, hstar);
-        if (hstartest + bL < bR) {
+    if (hR <= drytol || hL <= drytol) {
+        // :87-114, exactly one side dry (dry/dry returned above): wall test
+        // riemanntype(hL, hL, uL, -uL) if the right cell is dry, else
+        // riemanntype(hR, hR, -uR, uR); both branches share one code path.
+        const bool rdry = hR <= drytol;
+        const double hw = rdry ? hL : hR;
+        const double bw = rdry ? bL : bR, bd = rdry ? bR : bL;
+        const double sq = sqrt(g * hw);
+        const double hstar = wall_hstar(hw, rdry ? uL : -uR, rdry ? -uL : uR, sq, g, inv16g);
+        const double hstartest = dmax2(hw, hstar);
+        if (hstartest + bw < bd) {
             wall1 = 0.0;
-            wall2 = 0.0;
-            hR = hL;
-            huR = -huL;
-            bR = bL;
-            uR = -uL;
-            vR = vL;
-        } else if (hL + bL < bR) {
-            bR = hL + bL;
-        }
-    } else if (hL <= drytol) {  // :101-114
-        double hstar, t1, t2;
-        const double sq = sqrt(g * hR);
-        riemanntype(hR, hR, -uR, uR, sq, sq, drytol, g, inv16g, hstar, t1, t2);
-        const double hstartest = dmax2(hR, hstar);
-        if (hstartest + bR < bL) {
-            wall0 = 0.0;
-            wall1 = 0.0;
-            hL = hR;
-            huL = -huR;
-            bL = bR;
-            uL = -uR;
-            vL = vR;
-        } else if (hR + bR < bL) {
-            bL = hR + bR;
+            if (rdry) {
+                wall2 = 0.0;
+                hR = hL;
+                huR = -huL;
+                bR = bL;
+                uR = -uL;
+                vR = vL;
+            } else {
+                wall0 = 0.0;
+                hL = hR;
+                huL = -huR;
+                bL = bR;
+                uL = -uR;
+                vL = vR;
+            }
+        } else if (hw + bw < bd) {
+            if (rdry)
+                bR = hL + bL;
+            else
+                bL = hR + bR;
         }
     }
 

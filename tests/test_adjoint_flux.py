@@ -38,7 +38,7 @@ def torch_vjp(solver, n, QL, QR, dry, gfL, gfR):
 
 def test_host_build_of_solver_matches_oracle():
     rng = np.random.default_rng(1)
-    for QL, QR, dry in kat_states(600):
+    for QL, QR, dry in kat_states(7000):
         for solver in (0, 1):
             n = [(1.0, 0.0), (0.0, -1.0), (np.sqrt(0.5), np.sqrt(0.5))][rng.integers(3)]
             a = ns.edge_flux(solver, n, QL, QR, dry, po.G)
