@@ -134,7 +134,9 @@ def adjoint(args, torch):
     discrete adjoint with ceil(sqrt(200)) checkpoints.
 
     ratio = (backward sweep: recompute + adjoint) / (forward sweep), both over
-    the same steps; kernel_ratio = one adjoint launch / one forward launch."""
+    the same steps, with ceil(sqrt(N)) checkpoints; sweeps["all"] gives the
+    same with every state kept in HBM (no recompute); kernel_ratio = one
+    adjoint launch / one forward launch."""
     from samoa_amd import host
     from samoa_amd.adjoint import GaugeInversion
 
@@ -146,24 +148,39 @@ def adjoint(args, torch):
     dt0 = host.initial_dt(mesh, (eta, hu, hv, b), 0.45)
     gauges = np.random.default_rng(2024).choice(mesh.n, 32, replace=False)
     dev = torch.device("cuda:0")
-    prob = GaugeInversion(mesh, gauges, N, dict(scaling=1.0e6, dry_tolerance=1e-3, courant=0.45))
     e0 = torch.tensor(eta, device=dev)
     bb = torch.tensor(b, device=dev)
-    prob.record_dt(e0, bb, dt0)
-    target = prob.gauge_series(torch.tensor(eta_t, device=dev), bb).detach()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    for rep in range(2):  # the first repetition warms up
-        e0g = e0.clone().requires_grad_()
-        bg = bb.clone().requires_grad_()
-        torch.cuda.synchronize()
-        ev[0].record()
-        loss = ((prob.gauge_series(e0g, bg) - target) ** 2).sum()
-        ev[1].record()
-        loss.backward()
-        ev[2].record()
-        torch.cuda.synchronize()
-    t_fwd = ev[0].elapsed_time(ev[1])
-    t_bwd = ev[1].elapsed_time(ev[2])
+    sweeps = {}
+    for mode in ("sqrt", "all"):
+        prob = GaugeInversion(mesh, gauges, N, dict(scaling=1.0e6, dry_tolerance=1e-3, courant=0.45),
+                              checkpoint_every=(int(np.ceil(np.sqrt(N))) if mode == "sqrt" else "all"))
+        prob.record_dt(e0, bb, dt0)
+        target = prob.gauge_series(torch.tensor(eta_t, device=dev), bb).detach()
+        for rep in range(2):  # the first repetition warms up
+            e0g = e0.clone().requires_grad_()
+            bg = bb.clone().requires_grad_()
+            torch.cuda.synchronize()
+            ev[0].record()
+            loss = ((prob.gauge_series(e0g, bg) - target) ** 2).sum()
+            ev[1].record()
+            loss.backward()
+            ev[2].record()
+            torch.cuda.synchronize()
+        t_fwd = ev[0].elapsed_time(ev[1])
+        t_bwd = ev[1].elapsed_time(ev[2])
+        finite = bool(torch.isfinite(e0g.grad).all()) and bool(torch.isfinite(bg.grad).all())
+        sweeps[mode] = {"K": prob.K, "store_all": prob.store_all, "ms_forward_sweep": t_fwd,
+                        "ms_backward_sweep": t_bwd, "ratio": t_bwd / t_fwd, "loss": float(loss),
+                        "gradients_finite": finite}
+        if mode == "sqrt":
+            g_sqrt = (e0g.grad.clone(), bg.grad.clone())
+        else:
+            sweeps["gradients_equal_across_modes"] = bool(
+                torch.equal(g_sqrt[0], e0g.grad) and torch.equal(g_sqrt[1], bg.grad))
+        if mode == "all":
+            break
+        prob.close()
 
     # single-launch costs: one forward traversal with masks vs one adjoint traversal
     z = [torch.zeros_like(e0) for _ in range(9)]
@@ -194,18 +211,15 @@ def adjoint(args, torch):
         ev[1].record()
         torch.cuda.synchronize()
         times[name] = ev[0].elapsed_time(ev[1]) / reps
-    g_e = e0g.grad
-    finite = bool(torch.isfinite(g_e).all()) and bool(torch.isfinite(bg.grad).all())
     prob.close()
     adj_bytes = 96 * mesh.n / (times["adj"] * 1e-3)
     return {
         "workload": f"tsunami_inversion_uniform_d{d}_{N}steps_32gauges",
-        "cells": mesh.n, "steps": N, "checkpoints": int(np.ceil(N / prob.K)), "K": prob.K,
-        "ms_forward_sweep": t_fwd, "ms_backward_sweep": t_bwd,
-        "ratio": t_bwd / t_fwd,
+        "cells": mesh.n, "steps": N,
+        "ratio": sweeps["sqrt"]["ratio"],
+        "sweeps": sweeps,
         "ms_forward_launch": times["fwd"], "ms_adjoint_launch": times["adj"],
         "kernel_ratio": times["adj"] / times["fwd"],
-        "loss": float(loss), "gradients_finite": finite,
         "roofline_adjoint": {"bound": "hbm", "achieved": adj_bytes / 1e9, "peak": HBM_PEAK / 1e9,
                              "unit": "GB/s", "frac": adj_bytes / HBM_PEAK,
                              "bytes_per_cell_step": 96},
@@ -219,7 +233,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not run "
-              "(the multi-GPU sharding is a design note, see DESIGN.md)", file=sys.stderr)
+              "(sharding across GPUs is designed but not built)", file=sys.stderr)
         sys.exit(1)
 
     import numpy as np

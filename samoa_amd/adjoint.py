@@ -17,12 +17,15 @@ differentiable PyTorch operation:
 The backward pass is the hand-written discrete adjoint
 (swe_adjoint_step_device): with dt and the wet/dry limiter decisions of the
 forward pass frozen, lam^n = (dF/dQ^n)^T lam^{n+1} + dL/dQ^n and
-gb += (dF/db)^T lam^{n+1}.  Memory is bounded by two-level checkpointing:
-the forward pass keeps every K-th state (K = ceil(sqrt(n_steps)) by default,
-so about 2 sqrt(n_steps) states are alive at once); the backward pass
-recomputes one segment of K states at a time, recording the limiter masks,
-and runs the adjoint through it backwards.  The recompute costs one extra
-forward sweep.
+gb += (dF/db)^T lam^{n+1}.  The forward sweep is taped in one of two ways
+(plan_tape): when every state and mask fits the HBM budget (a depth-20 mesh
+needs 50 MB per step, so 200 steps take 10 GB of the 288 GB), all of them
+are kept and the backward sweep runs the adjoint kernels only; otherwise
+memory is bounded by two-level checkpointing: the forward pass keeps every
+K-th state (K = ceil(sqrt(n_steps)), so about 2 sqrt(n_steps) states are
+alive at once) and the backward pass recomputes one segment of K states at a
+time, recording the limiter masks, then runs the adjoint through it
+backwards.  The recompute costs one extra forward sweep.
 """
 from __future__ import annotations
 
@@ -46,7 +49,7 @@ class GaugeInversion:
     Gauges index cells of the concatenated state."""
 
     def __init__(self, mesh, gauges, n_steps: int, params=None, *, forward0: bool = False,
-                 checkpoint_every: int | None = None, device="cuda:0"):
+                 checkpoint_every="auto", hbm_budget: float | None = None, device="cuda:0"):
         if isinstance(mesh, Mesh):
             params = dict(params or {})
             self.op = EulerTimestep(mesh, **params)
@@ -69,8 +72,42 @@ class GaugeInversion:
             raise ValueError("gauges must be cell indices in [0, n)")
         self.gauges = torch.as_tensor(gauges, device=self.device)
         self.forward0 = bool(forward0)
-        self.K = int(checkpoint_every or max(1, math.ceil(math.sqrt(self.n_steps))))
+        self.store_all, self.K = self.plan_tape(self.n, self.n_steps, checkpoint_every, hbm_budget,
+                                                self.device)
         self.dt = None  # frozen dt sequence, device [n_steps + 1, n_scen]
+
+    @staticmethod
+    def plan_tape(n_cells: int, n_steps: int, checkpoint_every="auto", hbm_budget=None,
+                  device="cuda:0"):
+        """Chooses how the forward sweep is taped.  Returns (store_all, K).
+
+        store_all: every state Q^n (24 B/cell) and the limiter masks of every
+        step (1 B/cell) are kept, so the backward sweep runs the adjoint
+        kernels only.  Otherwise every K-th state is kept and the backward
+        sweep recomputes one segment of K states at a time (two-level
+        checkpointing, peak about (n_steps / K + K) states).
+
+        checkpoint_every: "all", an int K, or "auto": store everything when
+        it fits in hbm_budget bytes (default: half of the free device memory),
+        else K = ceil(sqrt(n_steps))."""
+        K_sqrt = max(1, math.ceil(math.sqrt(n_steps)))
+        if checkpoint_every == "all":
+            return True, 1
+        if checkpoint_every != "auto":
+            K = int(checkpoint_every)
+            if K < 1:
+                raise ValueError("checkpoint_every must be >= 1")
+            return False, K
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info(torch.device(device))
+            hbm_budget = 0.5 * free
+        state = 24.0 * n_cells
+        if n_steps * (state + n_cells) <= hbm_budget:
+            return True, 1
+        if (math.ceil(n_steps / K_sqrt) + K_sqrt + 2) * (state + n_cells) > hbm_budget:
+            raise MemoryError(f"{n_cells} cells x {n_steps} steps do not fit a "
+                              f"{hbm_budget / 2**30:.1f} GiB checkpoint budget")
+        return False, K_sqrt
 
     def direction(self, step: int) -> bool:
         """Traversal direction of step `step` (0-based): alternating."""
@@ -124,11 +161,20 @@ class GaugeInversion:
 
     # -- used by the autograd function ------------------------------------
     def run_forward(self, eta0, b):
-        """Forward sweep: gauge values after each step and the checkpoints
-        Q^{jK} (j = 0, 1, ...)."""
+        """Forward sweep: gauge values after each step and the tape: every
+        state and mask (store_all) or the checkpoints Q^{jK}."""
         q = [eta0.detach().contiguous().clone(), self._zeros(), self._zeros()]
-        out = [self._zeros() for _ in range(3)]
         series = torch.empty(self.n_steps, len(self.gauges), dtype=torch.float64, device=self.device)
+        if self.store_all:
+            states = [q] + [[self._zeros() for _ in range(3)] for _ in range(self.n_steps - 1)]
+            masks = torch.empty(self.n_steps, self.n, dtype=torch.int8, device=self.device)
+            last = [self._zeros() for _ in range(3)]
+            for n in range(self.n_steps):
+                out = states[n + 1] if n + 1 < self.n_steps else last
+                self._fwd(states[n], out, b, n, masks=masks[n])
+                torch.index_select(out[0], 0, self.gauges, out=series[n])
+            return series, {"states": states, "masks": masks}
+        out = [self._zeros() for _ in range(3)]
         checkpoints = {}
         for n in range(self.n_steps):
             if n % self.K == 0:
@@ -136,14 +182,28 @@ class GaugeInversion:
             self._fwd(q, out, b, n)
             torch.index_select(out[0], 0, self.gauges, out=series[n])
             q, out = out, q
-        return series, checkpoints
+        return series, {"checkpoints": checkpoints}
 
-    def run_backward(self, b, checkpoints, g_series):
+    def _adjoint(self, q, mask, b, lam, lam_out, gb, n):
+        self.op.adjoint_device(
+            _p(q[0]), _p(q[1]), _p(q[2]), _p(b), _p(mask), _p(lam[0]), _p(lam[1]), _p(lam[2]),
+            _p(lam_out[0]), _p(lam_out[1]), _p(lam_out[2]), _p(gb), self._dt(n),
+            self.direction(n), stream=self._stream())
+
+    def run_backward(self, b, tape, g_series):
         """Adjoint sweep; returns (d/d eta0, d/d b)."""
         lam = [self._zeros() for _ in range(3)]
         lam_out = [self._zeros() for _ in range(3)]
         gb = self._zeros()
         lam[0].index_add_(0, self.gauges, g_series[self.n_steps - 1])
+        if self.store_all:
+            for n in reversed(range(self.n_steps)):
+                self._adjoint(tape["states"][n], tape["masks"][n], b, lam, lam_out, gb, n)
+                lam, lam_out = lam_out, lam
+                if n >= 1:
+                    lam[0].index_add_(0, self.gauges, g_series[n - 1])
+            return lam[0].clone(), gb
+        checkpoints = tape["checkpoints"]
         starts = sorted(checkpoints)
         masks = [torch.empty(self.n, dtype=torch.int8, device=self.device) for _ in range(self.K)]
         states = [[self._zeros() for _ in range(3)] for _ in range(self.K)]
@@ -159,11 +219,7 @@ class GaugeInversion:
                 dst = states[i + 1] if i + 1 < s1 - s0 else spare
                 self._fwd(states[i], dst, b, n, masks=masks[i])
             for n in reversed(range(s0, s1)):
-                q = states[n - s0]
-                self.op.adjoint_device(
-                    _p(q[0]), _p(q[1]), _p(q[2]), _p(b), _p(masks[n - s0]),
-                    _p(lam[0]), _p(lam[1]), _p(lam[2]), _p(lam_out[0]), _p(lam_out[1]),
-                    _p(lam_out[2]), _p(gb), self._dt(n), self.direction(n), stream=self._stream())
+                self._adjoint(states[n - s0], masks[n - s0], b, lam, lam_out, gb, n)
                 lam, lam_out = lam_out, lam
                 if n >= 1:
                     lam[0].index_add_(0, self.gauges, g_series[n - 1])
@@ -177,14 +233,15 @@ class _GaugeSeries(torch.autograd.Function):
     @staticmethod
     def forward(ctx, eta0, b, problem):
         b = b.detach().contiguous()
-        series, checkpoints = problem.run_forward(eta0, b)
+        series, tape = problem.run_forward(eta0, b)
         ctx.problem = problem
-        ctx.checkpoints = checkpoints
+        ctx.tape = tape
         ctx.save_for_backward(b)
         return series
 
     @staticmethod
     def backward(ctx, g_series):
         (b,) = ctx.saved_tensors
-        g_eta0, g_b = ctx.problem.run_backward(b, ctx.checkpoints, g_series.contiguous())
+        g_eta0, g_b = ctx.problem.run_backward(b, ctx.tape, g_series.contiguous())
+        ctx.tape = None
         return g_eta0, g_b, None

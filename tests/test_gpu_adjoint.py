@@ -162,7 +162,7 @@ def test_checkpoint_interval_does_not_change_gradients():
     gauges = np.arange(0, mesh.n, 37)
     dt0 = po.initial_dt(mesh.depth, (eta, np.zeros(mesh.n), np.zeros(mesh.n), b), 0.45) * S
     res = []
-    for K in (1, 3, None, N):
+    for K in (1, 3, N, "all", "auto"):
         prob = GaugeInversion(mesh, gauges, N, dict(scaling=S, dry_tolerance=DRY),
                               checkpoint_every=K)
         e0 = torch.tensor(eta, device=DEV, requires_grad=True)
