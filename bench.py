@@ -14,7 +14,8 @@ configuration (64 such scenarios from seeds 0-63 with uneven frozen depths
 drawn from 14-20, one launch per step, autotuned launch shape, 500 steps)
 under "ensemble", and the differentiable inversion (depth-20 mesh, 200
 steps, 32 gauges, sqrt(200) checkpoints) with its adjoint/forward cost ratio
-under "adjoint".
+under "adjoint", and 16 batched source inversions (20 lockstep L-BFGS
+iterations) with their recovery error under "inversion".
 
 A step is one Euler traversal (flux on every edge, cell update with the
 wet/dry limiter, CFL reduction and the device-side dt update).  The K timed
@@ -43,12 +44,15 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=1000)
     p.add_argument("--warmup", type=int, default=20)
-    p.add_argument("--workload", default="all", choices=["all", "tsunami", "ensemble", "adjoint"])
+    p.add_argument("--workload", default="all",
+                   choices=["all", "tsunami", "ensemble", "adjoint", "inversion"])
     p.add_argument("--ens-steps", type=int, default=500)
     p.add_argument("--ens-scenarios", type=int, default=64)
     p.add_argument("--depth", type=int, default=21)
     p.add_argument("--adj-depth", type=int, default=20)
     p.add_argument("--adj-steps", type=int, default=200)
+    p.add_argument("--inv-instances", type=int, default=16)
+    p.add_argument("--inv-iters", type=int, default=20)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a 1 GiB device copy (counter calibration under rocprofv3)")
@@ -129,7 +133,8 @@ def ensemble(args, torch):
 def adjoint(args, torch):
     """Differentiable single inversion: the tsunami scenario frozen at a
     uniform depth-20 mesh, 200 steps with the forward dt sequence, squared
-    eta misfit at 32 seeded gauges every step against a target run from a
+    eta misfit at 32 seeded gauges (cells within 150 km of the source) every
+    step against a target run from a
     perturbed source; gradients with respect to eta0 and b through the
     discrete adjoint with ceil(sqrt(200)) checkpoints.
 
@@ -139,6 +144,7 @@ def adjoint(args, torch):
     adjoint launch / one forward launch."""
     from samoa_amd import host
     from samoa_amd.adjoint import GaugeInversion
+    from samoa_amd.inversion import gauges_near
 
     d, N = args.adj_depth, args.adj_steps
     mesh = host.mesh_uniform(d)
@@ -146,7 +152,8 @@ def adjoint(args, torch):
     eta_t = host.tsunami(mesh, seed=0, drytol=1e-3,
                          amplitude=1.25 * host.tsunami_draw(0)["amplitude"])[0]
     dt0 = host.initial_dt(mesh, (eta, hu, hv, b), 0.45)
-    gauges = np.random.default_rng(2024).choice(mesh.n, 32, replace=False)
+    src = host.tsunami_draw(0)
+    gauges = gauges_near(mesh, src["x0"], src["y0"], 1.5e5, 32, seed=2024)
     dev = torch.device("cuda:0")
     e0 = torch.tensor(eta, device=dev)
     bb = torch.tensor(b, device=dev)
@@ -226,6 +233,64 @@ def adjoint(args, torch):
     }
 
 
+def inversion(args, torch):
+    """Batched adjoint inversions: 16 tsunami sources (seeds 0-15, uniform
+    depths drawn from 16-20) recovered from 32 gauges each over 200-step
+    windows, all instances' forward and adjoint sweeps batched into one
+    launch per step, 20 lockstep L-BFGS iterations; reports the final source
+    recovery error."""
+    from samoa_amd import host
+    from samoa_amd.inversion import SCALE, SourceInversion, gauges_near, lbfgs
+
+    n_inst, N = args.inv_instances, args.adj_steps
+    dev = torch.device("cuda:0")
+    draws = [host.tsunami_draw(s, 16, 20) for s in range(n_inst)]
+    meshes = {d: host.mesh_uniform(d) for d in sorted({p["depth"] for p in draws})}
+    ms, truth, b, gauges, dt0 = [], [], [], [], []
+    for s, p in enumerate(draws):
+        m = meshes[p["depth"]]
+        eta, hu, hv, bb = host.tsunami(m, seed=s)
+        ms.append(m)
+        truth.append([p["amplitude"], p["ratio"], p["x0"], p["y0"]])
+        b.append(bb)
+        gauges.append(gauges_near(m, p["x0"], p["y0"], 1.5e5, 32, seed=s))
+        dt0.append(host.initial_dt(m, (eta, hu, hv, bb), 0.45))
+    inv = SourceInversion(ms, b, gauges, N, dt0)
+    theta_true = torch.tensor(truth, dtype=torch.float64, device=dev)
+    inv.set_target(theta_true)
+    scale = torch.tensor(SCALE, dtype=torch.float64, device=dev)
+    guess = theta_true * torch.tensor([0.6, 1.0, 1.0, 1.0], dtype=torch.float64, device=dev)
+    guess = guess + torch.tensor([0.0, 0.4, 3.0e4, -2.0e4], dtype=torch.float64, device=dev)
+    evals = [0]
+
+    def fg(z):
+        evals[0] += 1
+        return inv.loss_and_grad(z)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    z, f, hist = lbfgs(fg, guess / scale, iters=args.inv_iters)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    theta = z * scale
+    rel = ((theta - theta_true).abs() / theta_true.abs()).cpu().numpy()
+    with torch.no_grad():
+        e_rec, e_true = inv.eta0(theta), inv.eta0(theta_true)
+        src_err = float((e_rec - e_true).norm() / e_true.norm())
+    cells = int(inv.problem.n)
+    store_all = inv.problem.store_all
+    inv.close()
+    return {
+        "workload": f"tsunami_source_inversions_{n_inst}_uniform_d16-20_{N}steps_32gauges",
+        "instances": n_inst, "cells": cells, "steps": N, "lbfgs_iterations": args.inv_iters,
+        "depths": {str(d): [p["depth"] for p in draws].count(d) for d in sorted(meshes)},
+        "function_evaluations": evals[0], "seconds": wall, "store_all": store_all,
+        "loss_initial_max": float(hist[0].max()), "loss_final_max": float(f.max()),
+        "param_rel_error_max": float(rel.max()), "param_rel_error_median": float(np.median(rel)),
+        "source_rel_l2_error": src_err,
+    }
+
+
 import numpy as np  # noqa: E402
 
 
@@ -246,6 +311,9 @@ def main():
         return
     if args.workload == "adjoint":
         print(json.dumps({"metric": METRIC, "adjoint": adjoint(args, torch)}), flush=True)
+        return
+    if args.workload == "inversion":
+        print(json.dumps({"metric": METRIC, "inversion": inversion(args, torch)}), flush=True)
         return
 
     d = args.depth
@@ -329,6 +397,7 @@ def main():
     if args.workload == "all":
         out["ensemble"] = ensemble(args, torch)
         out["adjoint"] = adjoint(args, torch)
+        out["inversion"] = inversion(args, torch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, state, dt0 * S)
     print(json.dumps(out), flush=True)
