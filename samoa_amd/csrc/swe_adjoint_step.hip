@@ -189,12 +189,15 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
             }
             const bool c_left = bnd || (a.forward ? (cc < o) : (cc > o));
             const double sx = c_left ? nx : -nx, sy = c_left ? ny : -ny;
-            const double gfc[3] = {sc0, sc1, sc2}, gfo[3] = {so0, so1, so2};
+            // seeds of the left and right sides, selected element-wise (a
+            // pointer selection between two arrays would put them in scratch)
+            const double gfl[3] = {c_left ? sc0 : so0, c_left ? sc1 : so1, c_left ? sc2 : so2};
+            const double gfr[3] = {c_left ? so0 : sc0, c_left ? so1 : sc1, c_left ? so2 : sc2};
             double gQL[4] = {0, 0, 0, 0}, gQR[4] = {0, 0, 0, 0};
             edge_flux_adj<SOLVER>(sx, sy, c_left ? qe : oe, c_left ? qu : ou, c_left ? qv : ov,
                                   c_left ? qb : ob, c_left ? oe : qe, c_left ? ou : qu,
                                   c_left ? ov : qv, c_left ? ob : qb, drytol, g, inv16g,
-                                  c_left ? gfc : gfo, c_left ? gfo : gfc, gQL, gQR);
+                                  gfl, gfr, gQL, gQR);
             double gc[4], go[4];
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
