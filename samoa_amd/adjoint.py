@@ -116,6 +116,9 @@ class GaugeInversion:
     def _zeros(self):
         return torch.zeros(self.n, dtype=torch.float64, device=self.device)
 
+    def _empty(self):
+        return torch.empty(self.n, dtype=torch.float64, device=self.device)
+
     def _stream(self):
         return torch.cuda.current_stream(self.device).cuda_stream
 
@@ -166,15 +169,16 @@ class GaugeInversion:
         q = [eta0.detach().contiguous().clone(), self._zeros(), self._zeros()]
         series = torch.empty(self.n_steps, len(self.gauges), dtype=torch.float64, device=self.device)
         if self.store_all:
-            states = [q] + [[self._zeros() for _ in range(3)] for _ in range(self.n_steps - 1)]
+            # every slot is written by a traversal before it is read
+            states = [q] + [[self._empty() for _ in range(3)] for _ in range(self.n_steps - 1)]
             masks = torch.empty(self.n_steps, self.n, dtype=torch.int8, device=self.device)
-            last = [self._zeros() for _ in range(3)]
+            last = [self._empty() for _ in range(3)]
             for n in range(self.n_steps):
                 out = states[n + 1] if n + 1 < self.n_steps else last
                 self._fwd(states[n], out, b, n, masks=masks[n])
                 torch.index_select(out[0], 0, self.gauges, out=series[n])
             return series, {"states": states, "masks": masks}
-        out = [self._zeros() for _ in range(3)]
+        out = [self._empty() for _ in range(3)]
         checkpoints = {}
         for n in range(self.n_steps):
             if n % self.K == 0:
@@ -206,8 +210,8 @@ class GaugeInversion:
         checkpoints = tape["checkpoints"]
         starts = sorted(checkpoints)
         masks = [torch.empty(self.n, dtype=torch.int8, device=self.device) for _ in range(self.K)]
-        states = [[self._zeros() for _ in range(3)] for _ in range(self.K)]
-        spare = [self._zeros() for _ in range(3)]
+        states = [[self._empty() for _ in range(3)] for _ in range(self.K)]
+        spare = [self._empty() for _ in range(3)]
         for j in reversed(range(len(starts))):
             s0 = starts[j]
             s1 = starts[j + 1] if j + 1 < len(starts) else self.n_steps

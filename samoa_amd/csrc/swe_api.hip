@@ -50,7 +50,6 @@ int fail(int code, const char *fmt, ...) {
         if (rc_ != SWE_OK) return rc_; \
     } while (0)
 
-const double HUGE_SR = 1.7976931348623157e308;  // huge(1.0_SR)
 
 // Geometry tables, computed exactly as the reference initialises them.
 // SFC_data_types.f90:627-686: Jacobian of each plotter type rounded to whole
@@ -189,6 +188,9 @@ struct Batch {
     int32_t *d_chunk_scen = nullptr;
     swe::Slots *d_slots = nullptr;
     double *d_dt = nullptr;  // [nscen] host-chosen dt per scenario
+    double *d_part_min = nullptr;     // [part_cap] CFL minimum per chunk
+    int32_t *d_part_nref = nullptr;   // [part_cap] refinement count per chunk
+    int64_t part_cap = 0;
     double *eta[2] = {nullptr, nullptr}, *hu[2] = {nullptr, nullptr}, *hv[2] = {nullptr, nullptr};
     double *b = nullptr;
     int32_t *nbr = nullptr;
@@ -224,6 +226,8 @@ void release(Batch &B) {
     (void)hipFree(B.d_chunk_scen);
     (void)hipFree(B.d_slots);
     (void)hipFree(B.d_dt);
+    (void)hipFree(B.d_part_min);
+    (void)hipFree(B.d_part_nref);
     if (B.own_stream) (void)hipStreamDestroy(B.own_stream);
 }
 
@@ -243,6 +247,17 @@ int rebuild_chunks(Batch &B) {
     B.nchunks = off;
     if (B.nchunks >= (int64_t(1) << 31))
         return fail(SWE_ERR_ARG, "too many chunks (%lld)", (long long)B.nchunks);
+    if (B.nchunks > B.part_cap) {
+        (void)hipFree(B.d_part_min);
+        (void)hipFree(B.d_part_nref);
+        B.d_part_min = nullptr;
+        B.d_part_nref = nullptr;
+        B.part_cap = 0;
+        if (hipMalloc(&B.d_part_min, sizeof(double) * (size_t)B.nchunks) != hipSuccess ||
+            hipMalloc(&B.d_part_nref, sizeof(int32_t) * (size_t)B.nchunks) != hipSuccess)
+            return fail(SWE_ERR_NOMEM, "chunk partials allocation failed");
+        B.part_cap = B.nchunks;
+    }
     (void)hipFree(B.d_chunk_scen);
     B.d_chunk_scen = nullptr;
     if (B.nscen > 1) {
@@ -356,7 +371,6 @@ int batch_init(Batch &B, int32_t nscen, const swe_mesh_desc *descs) {
         HIP_TRY(hipMemcpy(B.depth + off, d.depth, (size_t)d.n_cells, hipMemcpyHostToDevice));
     }
     std::vector<swe::Slots> init((size_t)nscen);
-    for (auto &sl : init) std::memcpy(&sl.dtmin, &HUGE_SR, sizeof HUGE_SR);
     HIP_TRY(hipMemcpy(B.d_slots, init.data(), sizeof(swe::Slots) * init.size(),
                       hipMemcpyHostToDevice));
     B.uploaded.assign((size_t)nscen, 0);
@@ -378,6 +392,9 @@ swe::StepArgs make_args(const Batch &B, int from, int to, int forward) {
     a.scen = B.d_scen;
     a.chunk_scen = B.d_chunk_scen;
     a.slots = B.d_slots;
+    a.part_min = B.d_part_min;
+    a.part_nref = B.d_part_nref;
+    a.n_scen = B.nscen;
     a.nchunks = B.nchunks;
     a.g = B.g;
     a.inv16g = 1.0 / (16.0 * B.g);
@@ -455,10 +472,9 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
         HIP_TRY(hipMemcpyAsync(B.d_scen, B.scen.data(), sizeof(swe::Scen) * B.scen.size(),
                                hipMemcpyHostToDevice, B.stream));
     // the chain starts from dt0 (clamped as pre_traversal_grid_op does) and the
-    // caller's time; the reduction slots keep their between-launch invariant
+    // caller's time
     std::vector<swe::Slots> init((size_t)B.nscen);
     for (int32_t s = 0; s < B.nscen; ++s) {
-        std::memcpy(&init[s].dtmin, &HUGE_SR, sizeof HUGE_SR);
         init[s].dt = (dt_clamp > 0.0 && dt_clamp < dt0[s]) ? dt_clamp : dt0[s];
         init[s].time = time[s];
     }
@@ -536,14 +552,8 @@ int batch_forward_device(Batch &B, const double *eta, const double *hu, const do
     a.hv_out = hv_out;
     a.dt_host = dt_dev;
     a.masks = masks;
-    a.out_dtcfl = B.nscen == 1 ? dt_cfl_dev : nullptr;
+    a.out_dtcfl = dt_cfl_dev;
     HIP_TRY(swe::launch_step(a, B.solver, B.chunk, B.grid, st));
-    if (dt_cfl_dev && B.nscen > 1) {
-        // per-scenario CFL minima are published in the slots by the last chunk
-        for (int32_t s = 0; s < B.nscen; ++s)
-            HIP_TRY(hipMemcpyAsync(dt_cfl_dev + s, &B.d_slots[s].res_dtcfl, sizeof(double),
-                                   hipMemcpyDeviceToDevice, st));
-    }
     return SWE_OK;
 }
 

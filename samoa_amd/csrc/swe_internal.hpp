@@ -12,18 +12,14 @@ namespace swe {
 constexpr int CHUNK_SIZES[3] = {128, 256, 512};
 constexpr int DEFAULT_CHUNK = 256;
 
-// Per-scenario reduction slots.  Invariant between launches: dtmin holds the
-// bits of huge(1.0), counter and nref are 0.  The last workgroup of a
-// scenario's step restores the invariant after publishing the results.
+// Per-scenario results and device dt chain.  Written only by the finalize
+// kernel that follows each step kernel on the stream; the step kernel reads
+// dt.
 struct Slots {
-    unsigned long long dtmin;    // running min of the CFL candidates (double bits)
-    unsigned long long nref;     // running count of refinement flags
-    unsigned int counter;        // chunks of this scenario done
-    unsigned int pad;
     double dt;                   // dt of the next step (device dt chain)
     double time;                 // simulated time (device dt chain)
-    double res_dtcfl;            // published CFL minimum of the last step
-    unsigned long long res_nref; // published refinement count of the last step
+    double res_dtcfl;            // CFL minimum of the last step
+    long long res_nref;          // refinement count of the last step
 };
 
 // One scenario (mesh + constants) of a batch.  A single mesh is a batch of
@@ -53,8 +49,11 @@ struct StepArgs {
     const int32_t *chunk_scen;             // [nchunks] scenario of each chunk, or null (one scenario)
     Slots *slots;                          // [n_scen]
     const double *dt_host;                 // [n_scen] dt per scenario when device_dt == 0, or null
-    double *out_dtcfl;                     // single-scenario extra copy of the CFL minimum, or null
-    int64_t *out_nref;                     // single-scenario extra copy of the refinement count, or null
+    double *out_dtcfl;                     // [n_scen] extra copy of the CFL minima, or null
+    int64_t *out_nref;                     // [n_scen] extra copy of the refinement counts, or null
+    double *part_min;                      // [nchunks] CFL minimum of each chunk
+    int32_t *part_nref;                    // [nchunks] refinement count of each chunk
+    int32_t n_scen;
     int64_t nchunks;
     double dt;                             // dt when device_dt == 0 and dt_host is null
     double g, inv16g;                      // inv16g = 1 / (16 g), riemannsolvers.f:556
@@ -85,6 +84,8 @@ hipError_t upload_tables(const double *normal, const double *leg);
 hipError_t upload_tables_adjoint(const double *normal, const double *leg);
 hipError_t launch_adjoint(const AdjArgs &a, int solver, int chunk, int64_t grid_blocks,
                           hipStream_t s);
+// Launches the step kernel and, after it on the same stream, the finalize
+// kernel (per-scenario reduction of the chunk partials, results, dt chain).
 // grid_blocks: number of workgroups (multiple of 8) or 0 for one per chunk.
 hipError_t launch_step(const StepArgs &a, int solver, int chunk, int64_t grid_blocks,
                        hipStream_t s);

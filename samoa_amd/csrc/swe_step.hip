@@ -1,11 +1,14 @@
 // swe_step.hip -- the fused SWE Euler step for gfx950 (MI355X).
 //
-// One launch performs one traversal of SWE_euler_timestep.f90 over every
-// mesh of a batch: the GeoClaw flux on every edge (skeleton_scalar_op
+// One step kernel performs one traversal of SWE_euler_timestep.f90 over
+// every mesh of a batch: the GeoClaw flux on every edge (skeleton_scalar_op
 // :196-215 and bnd_skeleton_scalar_op :231-255), the cell update with the
-// wet/dry limiter (cell_update_op :257-284, volume_op :307-355) and, per
-// scenario, the CFL / refinement reductions and the next dt
-// (post_traversal_grid_op :111-131, pre_traversal_grid_op :89-109).
+// wet/dry limiter (cell_update_op :257-284, volume_op :307-355) and each
+// chunk's CFL / refinement partials.  A small finalize kernel then reduces
+// the partials per scenario and advances the dt chain
+// (post_traversal_grid_op :111-131, pre_traversal_grid_op :89-109).  The step
+// kernel uses no atomics: per-workgroup atomics on one per-scenario address
+// serialise across the eight XCDs.
 //
 // Mapping.  A workgroup processes chunks of CH consecutive SFC cells of one
 // scenario (one thread per cell).  It stages the chunk's state in LDS, builds
@@ -256,7 +259,8 @@ __global__ __launch_bounds__(CH) void euler_step_kernel(StepArgs a) {
             a.hv_out[gs + lc_t] = q2;
         }
 
-        // CFL minimum and refinement count (post_traversal_grid_op :117-118)
+        // CFL minimum and refinement count of the chunk (post_traversal_grid_op
+        // :117-118); the finalize kernel reduces the chunks of each scenario.
         const double wmin = wave_min(cand);
         const int wref = wave_sum_i(refined);
         if (lane == 0) {
@@ -272,37 +276,57 @@ __global__ __launch_bounds__(CH) void euler_step_kernel(StepArgs a) {
                 m = dmin2(m, s_wmin[w]);
                 rsum += s_wref[w];
             }
-            Slots *sl = &a.slots[s];
-            // non-negative doubles order like their bit patterns
-            atomicMin(&sl->dtmin, (unsigned long long)__double_as_longlong(m));
-            if (rsum) atomicAdd(&sl->nref, (unsigned long long)rsum);
-            __threadfence();  // publish this chunk's contribution before counting it
-            const unsigned int done = atomicAdd(&sl->counter, 1u);
-            if (done == (unsigned int)(sc.nchunks - 1)) {
-                // last chunk of this scenario's step: every contribution is visible
-                __threadfence();
-                const unsigned long long bits = atomicAdd(&sl->dtmin, 0ull);
-                const unsigned long long nr = atomicAdd(&sl->nref, 0ull);
-                const double mn = __longlong_as_double((long long)bits);
-                sl->res_dtcfl = mn;
-                sl->res_nref = nr;
-                if (a.out_dtcfl) *a.out_dtcfl = mn;
-                if (a.out_nref) *a.out_nref = (int64_t)nr;
-                if (a.device_dt) {
-                    // post_traversal_grid_op :115-128, next pre_traversal_grid_op :93-99
-                    sl->time = sl->time + dt;
-                    double next = sc.courant * mn;
-                    if (sc.dt_clamp > 0.0) next = dmin2(sc.dt_clamp, next);
-                    sl->dt = next;
-                }
-                atomicExch(&sl->dtmin,
-                           (unsigned long long)__double_as_longlong(1.7976931348623157e308));
-                atomicExch(&sl->nref, 0ull);
-                atomicExch(&sl->counter, 0u);
-                __threadfence();
-            }
+            a.part_min[chunk] = m;
+            a.part_nref[chunk] = rsum;
         }
         __syncthreads();  // LDS is reused by the next chunk
+    }
+}
+
+// One workgroup per scenario: reduces the chunk partials of the step that
+// just ran, publishes the results and advances the device dt chain
+// (post_traversal_grid_op :115-128 and the next pre_traversal_grid_op :93-99).
+constexpr int FIN_THREADS = 256;
+
+__global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(StepArgs a) {
+    __shared__ double s_min[FIN_THREADS / 64];
+    __shared__ long long s_ref[FIN_THREADS / 64];
+    const int s = blockIdx.x;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const Scen sc = a.scen[s];
+    double m = 1.7976931348623157e308;  // huge(1.0_SR)
+    long long r = 0;
+    for (int64_t i = t; i < sc.nchunks; i += FIN_THREADS) {
+        m = dmin2(m, a.part_min[sc.chunk_off + i]);
+        r += a.part_nref[sc.chunk_off + i];
+    }
+    m = wave_min(m);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
+    if (lane == 0) {
+        s_min[wave] = m;
+        s_ref[wave] = r;
+    }
+    __syncthreads();
+    if (t == 0) {
+        m = s_min[0];
+        r = s_ref[0];
+#pragma unroll
+        for (int w = 1; w < FIN_THREADS / 64; ++w) {
+            m = dmin2(m, s_min[w]);
+            r += s_ref[w];
+        }
+        Slots *sl = &a.slots[s];
+        sl->res_dtcfl = m;
+        sl->res_nref = r;
+        if (a.out_dtcfl) a.out_dtcfl[s] = m;
+        if (a.out_nref) a.out_nref[s] = (int64_t)r;
+        if (a.device_dt) {
+            sl->time = sl->time + sl->dt;
+            double next = sc.courant * m;
+            if (sc.dt_clamp > 0.0) next = dmin2(sc.dt_clamp, next);
+            sl->dt = next;
+        }
     }
 }
 
@@ -318,6 +342,7 @@ static void launch_t(const StepArgs &a, int64_t grid_blocks, hipStream_t s) {
     const int64_t blocks = grid_blocks > 0 && grid_blocks < nvirt ? grid_blocks : nvirt;
     hipLaunchKernelGGL((euler_step_kernel<SOLVER, CH>), dim3((unsigned)blocks), dim3(CH), 0, s,
                        a);
+    hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)a.n_scen), dim3(FIN_THREADS), 0, s, a);
 }
 
 hipError_t launch_step(const StepArgs &a, int solver, int chunk, int64_t grid_blocks,
