@@ -135,6 +135,8 @@ int pack_mesh(const swe_mesh_desc &d, const double *normal, std::vector<int32_t>
     const int64_t n = d.n_cells;
     packed.assign((size_t)(3 * n), -1);
     int ud = d.depth[0];
+    // every cell's depth and plotter type first: the topology pass reads the
+    // neighbours' plotter types to index the normal table
     for (int64_t c = 0; c < n; ++c) {
         const int dep = d.depth[c];
         const int p = d.plotter[c];
@@ -144,12 +146,15 @@ int pack_mesh(const swe_mesh_desc &d, const double *normal, std::vector<int32_t>
             return fail(SWE_ERR_ARG, "plotter[%lld] = %d", (long long)c, p);
         for (int k = 0; k < 3; ++k) {
             const int32_t o = d.nbr[3 * c + k];
-            if (o < 0) {
-                if (o != -1) return fail(SWE_ERR_ARG, "nbr[%lld] = %d", (long long)(3 * c + k), o);
-                continue;
-            }
-            if (o >= n || o == c)
+            if (o < -1 || o >= n || o == c)
                 return fail(SWE_ERR_ARG, "nbr[%lld] = %d invalid", (long long)(3 * c + k), o);
+        }
+    }
+    for (int64_t c = 0; c < n; ++c) {
+        const int p = d.plotter[c];
+        for (int k = 0; k < 3; ++k) {
+            const int32_t o = d.nbr[3 * c + k];
+            if (o < 0) continue;
             int ko = -1;
             for (int j = 0; j < 3; ++j)
                 if (d.nbr[3 * (int64_t)o + j] == (int32_t)c) ko = j;
