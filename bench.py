@@ -123,7 +123,7 @@ def ensemble(args, torch):
         "value": cells * args.ens_steps / (ms_total * 1e-3), "unit": "cell-updates/s",
         "ms_per_step": ms_step, "steps": args.ens_steps, "scenarios": n_scen, "cells": cells,
         "depths": {str(d): depths.count(d) for d in sorted(set(depths))},
-        "tuning": {"chunk": tuned["chunk"], "grid": tuned["grid"]},
+        "tuning": {"chunk": tuned["chunk"], "grid": tuned["grid"], "trials": tuned["trials"]},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK},
         "workload": "tsunami_ensemble_64_uniform_d14-20",
@@ -304,7 +304,7 @@ def main():
     import numpy as np
     import torch
 
-    from samoa_amd import host, swe
+    from samoa_amd import autotune, host, swe
 
     if args.workload == "ensemble":
         print(json.dumps({"metric": METRIC, "ensemble": ensemble(args, torch)}), flush=True)
@@ -328,6 +328,7 @@ def main():
     op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3, courant=courant,
                            solver=swe.SOLVER_AUG_RIEMANN, boundary=swe.BC_OUTFLOW)
     op.upload(*state)
+    tuned = autotune.tune(op, dt0, use_cache=False)
     op.set_stream(stream.cuda_stream)
 
     # warmup: first steps of the run (also captures the graph)
@@ -351,12 +352,13 @@ def main():
         raise RuntimeError("non-finite state after the timed steps")
 
     achieved = BYTES_PER_CELL * cells / (ms_per_step * 1e-3)
-    traffic = None
+    traffic, valu = None, None
     if os.path.exists(PROFILE_TRAFFIC):
         prof = json.load(open(PROFILE_TRAFFIC))
         key = f"euler_step_tsunami_d{d}"
         if key in prof:
             traffic = prof[key]["hbm_bytes_per_launch"]
+            valu = prof[key].get("valu_issue_frac")
 
     if args.calibrate_copy:
         a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
@@ -387,9 +389,11 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": traffic,
-            "kernel": "swe::euler_step_kernel<0>",
+            "kernel": "swe::euler_step_kernel<0> + swe::finalize_kernel (one step)",
             "bytes_per_cell_update": BYTES_PER_CELL,
+            "valu_issue_frac_profiled": valu,
         },
+        "tuning": {"chunk": tuned["chunk"], "grid": tuned["grid"], "trials": tuned["trials"]},
     }
     out["roofline"]["achieved"] = achieved / 1e9
     out["roofline"]["peak"] = HBM_PEAK / 1e9

@@ -144,6 +144,12 @@ int swe_adjoint_step_device(swe_handle *h, const double *eta, const double *hu, 
  * records on that stream.  Waits for work already queued. */
 int swe_set_stream(swe_handle *h, void *stream);
 
+/* Launch shape of the handle's step kernel, as swe_batch_set_tuning below:
+ * chunk_cells in {128, 256, 512}, grid_blocks 0 or a multiple of 8.  No
+ * counterpart in the reference; results do not depend on it. */
+int swe_set_tuning(swe_handle *h, int chunk_cells, int64_t grid_blocks);
+int swe_get_tuning(swe_handle *h, int *chunk_cells, int64_t *grid_blocks);
+
 /* Releases the handle and its device memory. NULL is accepted. */
 int swe_destroy(swe_handle *h);
 

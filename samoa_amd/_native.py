@@ -63,6 +63,8 @@ SIGNATURES = {
     "swe_batch_download": (_I, [_P, C.c_int32, _P, _P, _P, _P]),
     "swe_batch_step": (_I, [_P, _P, _I, _P, _P]),
     "swe_batch_run": (_I, [_P, _I64, _P, _I, _D, _P, _P]),
+    "swe_set_tuning": (_I, [_P, _I, _I64]),
+    "swe_get_tuning": (_I, [_P, C.POINTER(_I), C.POINTER(_I64)]),
     "swe_batch_set_tuning": (_I, [_P, _I, _I64]),
     "swe_batch_get_tuning": (_I, [_P, C.POINTER(_I), C.POINTER(_I64)]),
     "swe_batch_set_stream": (_I, [_P, _P]),

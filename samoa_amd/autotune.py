@@ -1,12 +1,12 @@
-"""Launch-shape autotuner for batched ensembles.
+"""Launch-shape autotuner for single meshes and batched ensembles.
 
 Tries every compiled chunk size (cells per workgroup, i.e. the LDS tile) and a
 few grid sizes (one workgroup per chunk, or persistent grids of k workgroups
-per CU) on the batch's own meshes and state, timing a few graph-replayed
-steps of each with HIP events on the stream the kernels run on.  The state is
-saved before and restored after, so tuning does not advance the simulation.
-Results are cached per device and batch signature in
-~/.cache/samoa_amd/autotune.json.
+per CU) on the operator's own meshes and state, timing graph-replayed steps
+of each with HIP events on the stream the kernels run on (the median of a
+few repetitions).  The state is saved before and restored after, so tuning
+does not advance the simulation.  Results are cached per device and mesh
+signature in ~/.cache/samoa_amd/autotune.json.
 """
 from __future__ import annotations
 
@@ -18,6 +18,31 @@ import numpy as np
 
 CHUNKS = (128, 256, 512)
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "samoa_amd", "autotune.json")
+
+
+class _Single:
+    """An EulerTimestep seen through the Batch interface (one scenario)."""
+
+    def __init__(self, op):
+        self.op = op
+        self.meshes = [op.mesh]
+        self.n_scen = 1
+        self.cells = op.n
+
+    def download(self, s):
+        return self.op.download()
+
+    def upload(self, s, *st):
+        self.op.upload(*st)
+
+    def set_stream(self, stream):
+        self.op.set_stream(stream)
+
+    def set_tuning(self, chunk, grid):
+        self.op.set_tuning(chunk, grid)
+
+    def run(self, n, dt0, forward):
+        return self.op.run(n, float(np.asarray(dt0).reshape(-1)[0]), forward)
 
 
 def signature(batch) -> str:
@@ -35,8 +60,15 @@ def candidates(n_cus: int):
     return out
 
 
-def tune(batch, dt0, steps: int = 6, use_cache: bool = True, verbose: bool = False) -> dict:
+def tune(batch, dt0, steps: int = 20, repeats: int = 3, use_cache: bool = True,
+         verbose: bool = False) -> dict:
+    """batch: a swe.Batch or a swe.EulerTimestep; dt0 per scenario."""
     import torch
+
+    from .swe import EulerTimestep
+
+    if isinstance(batch, EulerTimestep):
+        batch = _Single(batch)
 
     dev = torch.cuda.current_device()
     props = torch.cuda.get_device_properties(dev)
@@ -60,13 +92,17 @@ def tune(batch, dt0, steps: int = 6, use_cache: bool = True, verbose: bool = Fal
         for chunk, grid in candidates(props.multi_processor_count):
             batch.set_tuning(chunk, grid)
             batch.run(2, dt0, forward=False)  # capture + warm
-            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            torch.cuda.synchronize()
-            start.record(stream)
-            batch.run(steps, dt0, forward=False)
-            end.record(stream)
-            torch.cuda.synchronize()
-            ms = start.elapsed_time(end) / steps
+            reps = []
+            for _ in range(repeats):
+                start = torch.cuda.Event(enable_timing=True)
+                end = torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                start.record(stream)
+                batch.run(steps, dt0, forward=False)
+                end.record(stream)
+                torch.cuda.synchronize()
+                reps.append(start.elapsed_time(end) / steps)
+            ms = float(np.median(reps))
             timings.append({"chunk": chunk, "grid": grid, "ms_per_step": ms})
             if verbose:
                 print(f"autotune: chunk {chunk} grid {grid}: {ms:.4f} ms/step", flush=True)

@@ -688,6 +688,19 @@ int swe_set_stream(swe_handle *h, void *stream) {
     return batch_set_stream(h->B, stream);
 }
 
+int swe_set_tuning(swe_handle *h, int chunk_cells, int64_t grid_blocks) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_set_tuning: null handle");
+    return batch_set_tuning(h->B, chunk_cells, grid_blocks);
+}
+
+int swe_get_tuning(swe_handle *h, int *chunk_cells, int64_t *grid_blocks) {
+    if (!h || !chunk_cells || !grid_blocks)
+        return fail(SWE_ERR_ARG, "swe_get_tuning: null argument");
+    *chunk_cells = h->B.chunk;
+    *grid_blocks = h->B.grid;
+    return SWE_OK;
+}
+
 int swe_step_device(swe_handle *h, const double *eta, const double *hu, const double *hv,
                     const double *b, double *eta_out, double *hu_out, double *hv_out, double dt,
                     int forward, double *dt_cfl_dev, int64_t *n_refine_dev, int8_t *refine_dev,

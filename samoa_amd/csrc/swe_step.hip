@@ -286,7 +286,9 @@ __global__ __launch_bounds__(CH) void euler_step_kernel(StepArgs a) {
 // One workgroup per scenario: reduces the chunk partials of the step that
 // just ran, publishes the results and advances the device dt chain
 // (post_traversal_grid_op :115-128 and the next pre_traversal_grid_op :93-99).
-constexpr int FIN_THREADS = 256;
+// 1024 threads with four independent loads in flight each: the partials of
+// a 4M-cell mesh (16k chunks) take four rounds of memory latency.
+constexpr int FIN_THREADS = 1024;
 
 __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(StepArgs a) {
     __shared__ double s_min[FIN_THREADS / 64];
@@ -294,12 +296,28 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(StepArgs a) {
     const int s = blockIdx.x;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const Scen sc = a.scen[s];
-    double m = 1.7976931348623157e308;  // huge(1.0_SR)
+    const double huge = 1.7976931348623157e308;  // huge(1.0_SR)
+    double m0 = huge, m1 = huge, m2 = huge, m3 = huge;
     long long r = 0;
-    for (int64_t i = t; i < sc.nchunks; i += FIN_THREADS) {
-        m = dmin2(m, a.part_min[sc.chunk_off + i]);
-        r += a.part_nref[sc.chunk_off + i];
+    const double *pm = a.part_min + sc.chunk_off;
+    const int32_t *pr = a.part_nref + sc.chunk_off;
+    const int64_t nc = sc.nchunks;
+    int64_t i = t;
+    for (; i + 3 * FIN_THREADS < nc; i += 4 * FIN_THREADS) {
+        const double v0 = pm[i], v1 = pm[i + FIN_THREADS], v2 = pm[i + 2 * FIN_THREADS],
+                     v3 = pm[i + 3 * FIN_THREADS];
+        r += (long long)pr[i] + pr[i + FIN_THREADS] + pr[i + 2 * FIN_THREADS] +
+             pr[i + 3 * FIN_THREADS];
+        m0 = dmin2(m0, v0);
+        m1 = dmin2(m1, v1);
+        m2 = dmin2(m2, v2);
+        m3 = dmin2(m3, v3);
     }
+    for (; i < nc; i += FIN_THREADS) {
+        m0 = dmin2(m0, pm[i]);
+        r += pr[i];
+    }
+    double m = dmin2(dmin2(m0, m1), dmin2(m2, m3));
     m = wave_min(m);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);

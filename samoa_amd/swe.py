@@ -147,6 +147,14 @@ class EulerTimestep:
         """Launch on a caller's HIP stream (0 = the library's own)."""
         _native.check("swe_set_stream", self._lib.swe_set_stream(self._h, stream or None))
 
+    def set_tuning(self, chunk: int, grid: int = 0) -> None:
+        _native.check("swe_set_tuning", self._lib.swe_set_tuning(self._h, chunk, grid))
+
+    def get_tuning(self):
+        c, g = C.c_int(), C.c_int64()
+        _native.check("swe_get_tuning", self._lib.swe_get_tuning(self._h, C.byref(c), C.byref(g)))
+        return c.value, g.value
+
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.swe_destroy(self._h)
