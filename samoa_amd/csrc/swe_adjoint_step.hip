@@ -41,9 +41,15 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
     __shared__ uint16_t s_list[3 * CH];
     __shared__ int s_wsum[CH / 64];
 
+    __shared__ double s_normal[17 * 3 * 2];  // geometry tables in LDS (see swe_step.hip)
+    __shared__ double s_leg[64];
+
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const double g = a.g, inv16g = a.inv16g;
+    for (int i = t; i < 17 * 3 * 2; i += CH) s_normal[i] = c_normal_adj[i];
+    for (int i = t; i < 64; i += CH) s_leg[i] = c_leg_adj[i];
+    __syncthreads();
     const int64_t nchunks = a.nchunks;
     const int64_t q = nchunks / 8, r = nchunks % 8;
     const int64_t nvirt = 8 * ((nchunks + 7) / 8);
@@ -141,10 +147,10 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
             const int64_t cc = lbase + lc;
             const int32_t p = s_nbr[k][lc];
             const int pi = s_plot[lc] + 8;
-            const double nx = c_normal_adj[(pi * 3 + k) * 2], ny = c_normal_adj[(pi * 3 + k) * 2 + 1];
+            const double nx = s_normal[(pi * 3 + k) * 2], ny = s_normal[(pi * 3 + k) * 2 + 1];
             const double qe = s_q[0][lc], qu = s_q[1][lc], qv = s_q[2][lc], qb = s_q[3][lc];
             const int dc = s_dep[lc];
-            const double len_c = S * (k == 1 ? c_leg_adj[dc] : c_leg_adj[dc + 1]);
+            const double len_c = S * (k == 1 ? s_leg[dc] : s_leg[dc + 1]);
             const double sc0 = s_seed[0][lc] * len_c, sc1 = s_seed[1][lc] * len_c,
                          sc2 = s_seed[2][lc] * len_c;
             const bool bnd = p < 0;
@@ -169,7 +175,7 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
                 ov = s_q[2][lo];
                 ob = s_q[3][lo];
                 const int dl = s_dep[lo];
-                const double len_o = S * (ko == 1 ? c_leg_adj[dl] : c_leg_adj[dl + 1]);
+                const double len_o = S * (ko == 1 ? s_leg[dl] : s_leg[dl + 1]);
                 so0 = s_seed[0][lo] * len_o;
                 so1 = s_seed[1][lo] * len_o;
                 so2 = s_seed[2][lo] * len_o;
@@ -182,7 +188,7 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
                 const int dl = sc.depth >= 0 ? sc.depth : a.depth[gt + o];
                 const int mk = a.masks[x];
                 const double vol = (S * S) * ldexp(1.0, -(dl + 1));
-                const double f = (mk & 2) ? 0.0 : (-dt / vol) * (S * (ko == 1 ? c_leg_adj[dl] : c_leg_adj[dl + 1]));
+                const double f = (mk & 2) ? 0.0 : (-dt / vol) * (S * (ko == 1 ? s_leg[dl] : s_leg[dl + 1]));
                 so0 = f * a.leta[x];
                 so1 = f * a.lhu[x];
                 so2 = f * a.lhv[x];

@@ -65,10 +65,17 @@ __global__ __launch_bounds__(CH) void euler_step_kernel(StepArgs a) {
     __shared__ int s_wsum[NW];
     __shared__ double s_wmin[NW];
     __shared__ int s_wref[NW];
+    // geometry tables in LDS: the per-lane indexed reads would otherwise be
+    // global loads that every edge waits for
+    __shared__ double s_normal[17 * 3 * 2];
+    __shared__ double s_leg[64];
 
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const double g = a.g, inv16g = a.inv16g;
+    for (int i = t; i < 17 * 3 * 2; i += CH) s_normal[i] = c_normal[i];
+    for (int i = t; i < 64; i += CH) s_leg[i] = c_leg[i];
+    __syncthreads();
 
     // XCD-aware order: virtual block v runs on XCD v % 8 (the dispatcher deals
     // workgroups round robin and the grid is a multiple of 8); XCD x owns a
@@ -144,7 +151,7 @@ __global__ __launch_bounds__(CH) void euler_step_kernel(StepArgs a) {
             const int64_t cc = lbase + lc;
             const int32_t p = s_nbr[k][lc];
             const int pi = s_plot[lc] + 8;
-            const double nx = c_normal[(pi * 3 + k) * 2], ny = c_normal[(pi * 3 + k) * 2 + 1];
+            const double nx = s_normal[(pi * 3 + k) * 2], ny = s_normal[(pi * 3 + k) * 2 + 1];
             const double qe = s_q[0][lc], qu = s_q[1][lc], qv = s_q[2][lc], qb = s_q[3][lc];
             const bool bnd = p < 0;
             const int64_t o = bnd ? cc : (int64_t)(p >> 2);
@@ -204,7 +211,7 @@ __global__ __launch_bounds__(CH) void euler_step_kernel(StepArgs a) {
             const int d = sc.depth >= 0 ? sc.depth : (int)a.depth[gt + lc_t];
             const double S = sc.scaling;
             const double volume = (S * S) * ldexp(1.0, -(d + 1));
-            const double lleg = S * c_leg[d + 1], lhyp = S * c_leg[d];
+            const double lleg = S * s_leg[d + 1], lhyp = S * s_leg[d];
             const int k0 = a.forward ? 0 : 2, k2 = a.forward ? 2 : 0;
             double dh = 0.0, d1 = 0.0, d2 = 0.0, sv = 0.0;
             dh = dh + lleg * s_f[0][k0][t];
