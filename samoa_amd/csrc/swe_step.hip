@@ -54,8 +54,11 @@ __device__ __forceinline__ int wave_sum_i(int v) {
     return v;
 }
 
+// Four waves per SIMD (at most 128 VGPRs, a few spilled) outrun three
+// unspilled ones: the LDS of four 256-cell workgroups fits a CU.
 template <int SOLVER, int CH>
-__global__ __launch_bounds__(CH) void euler_step_kernel(StepArgs a) {
+__global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void euler_step_kernel(
+    StepArgs a) {
     constexpr int NW = CH / 64;
     __shared__ double s_q[4][CH];        // eta, hu, hv, b of the chunk
     __shared__ double s_f[4][3][CH];     // flux (h, hu, hv, s) per (edge, cell)
