@@ -20,7 +20,8 @@
  *              (fine_triangle%i_plotter_type)
  *   nbr[3n]    neighbour across edge k = 0,1,2 (reference edge index k+1,
  *              edge i opposite node i, SFC_data_types.f90:660-666), -1 on the
- *              domain boundary
+ *              domain boundary; consecutive cells of the curve share an edge
+ *              (swe_create rejects a mesh where they do not)
  *
  * Ownership: the caller owns host arrays and may free them when a call
  * returns; the library owns its device buffers.  A handle is not re-entrant:

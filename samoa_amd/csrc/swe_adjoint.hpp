@@ -228,21 +228,21 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
     const double uL1 = wetL ? huL / hL : 0.0, vL1 = wetL ? hvL / hL : 0.0;
 
     // stage B: wall ghosting (:87-115)
+    // (the decisions only: hstar carries no gradient; one code path for both
+    // sides, with the closed-form hstar of the forward solver)
     bool wallR = false, clipR = false, wallL = false, clipL = false;
-    if (hR1 <= drytol) {
-        double hstar, t1, t2;
-        const double sq = sqrt(g * hL1);
-        riemanntype(hL1, hL1, uL1, -uL1, sq, sq, drytol, g, inv16g, hstar, t1, t2);
-        const double hstartest = dmax2(hL1, hstar);
-        wallR = hstartest + bL < bR;
-        clipR = !wallR && (hL1 + bL < bR);
-    } else if (hL1 <= drytol) {
-        double hstar, t1, t2;
-        const double sq = sqrt(g * hR1);
-        riemanntype(hR1, hR1, -uR1, uR1, sq, sq, drytol, g, inv16g, hstar, t1, t2);
-        const double hstartest = dmax2(hR1, hstar);
-        wallL = hstartest + bR < bL;
-        clipL = !wallL && (hR1 + bR < bL);
+    if (hR1 <= drytol || hL1 <= drytol) {
+        const bool rdry = hR1 <= drytol;
+        const double hw = rdry ? hL1 : hR1;
+        const double bw = rdry ? bL : bR, bd = rdry ? bR : bL;
+        const double hstar =
+            wall_hstar(hw, rdry ? uL1 : -uR1, rdry ? -uL1 : uR1, sqrt(g * hw), g, inv16g);
+        const bool wall = dmax2(hw, hstar) + bw < bd;
+        const bool clip = !wall && (hw + bw < bd);
+        wallR = rdry && wall;
+        clipR = rdry && clip;
+        wallL = !rdry && wall;
+        clipL = !rdry && clip;
     }
     const double hR2 = wallR ? hL1 : hR1, huR2 = wallR ? -huL1 : huR1;
     const double bR2 = wallR ? bL : (clipR ? hL1 + bL : bR);

@@ -170,6 +170,14 @@ int pack_mesh(const swe_mesh_desc &d, const double *normal, std::vector<int32_t>
                             (long long)c, o);
             packed[3 * c + k] = (int32_t)((o << 2) | ko);
         }
+        // Sierpinski order: consecutive cells share an edge.  The kernels rely
+        // on it: a run of m consecutive cells then has at most m + 2 edges
+        // leaving it, which bounds the LDS halo of a chunk.
+        if (c + 1 < n && d.nbr[3 * c] != c + 1 && d.nbr[3 * c + 1] != c + 1 &&
+            d.nbr[3 * c + 2] != c + 1)
+            return fail(SWE_ERR_ARG,
+                        "cells %lld and %lld are consecutive but share no edge (not in "
+                        "Sierpinski order)", (long long)c, (long long)(c + 1));
     }
     *uniform_depth = ud;
     return SWE_OK;

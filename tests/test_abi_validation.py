@@ -87,6 +87,13 @@ def test_bad_topology(lib):
     cases.append((dep, plo, b, "not symmetric"))
     # flip one plotter type: the shared edges' normals no longer cancel
     p = plo.copy(); p[c] = -p[c]; cases.append((dep, p, nbr, "disagree on the normal"))
+    # swap two cells' order without touching adjacency: no longer a curve
+    perm = np.arange(len(dep))
+    perm[[1, 5]] = perm[[5, 1]]
+    inv = np.argsort(perm)
+    nb2 = nbr.reshape(-1, 3)[perm]
+    nb2 = np.where(nb2 >= 0, inv[np.maximum(nb2, 0)], -1).astype(np.int32).reshape(-1)
+    cases.append((dep[perm].copy(), plo[perm].copy(), nb2, "not in Sierpinski order"))
     for d_, p_, n_, msg in cases:
         assert create(lib, desc(d_, p_, n_)) == ERR_ARG, msg
         assert msg in last_error(lib), (msg, last_error(lib))
