@@ -54,9 +54,20 @@ def parse():
     p.add_argument("--inv-instances", type=int, default=16)
     p.add_argument("--inv-iters", type=int, default=20)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--tuning", default=None,
+                   help="CHUNK,GRID launch shape instead of autotuning (e.g. for profiling)")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="also run a 1 GiB device copy (counter calibration under rocprofv3)")
     return p.parse_args()
+
+
+def fixed_tuning(args, op):
+    """--tuning CHUNK,GRID: set the launch shape instead of autotuning."""
+    if not args.tuning:
+        return None
+    chunk, grid = (int(x) for x in args.tuning.split(","))
+    op.set_tuning(chunk, grid)
+    return {"chunk": chunk, "grid": grid, "trials": []}
 
 
 def cpu_baseline(mesh, state, dt, seconds_target=15.0):
@@ -102,7 +113,7 @@ def ensemble(args, torch):
         b.upload(s, *st)
         dt0.append(host.initial_dt(m, st, 0.45))
     dt0 = np.array(dt0)
-    tuned = autotune.tune(b, dt0, use_cache=False)
+    tuned = fixed_tuning(args, b) or autotune.tune(b, dt0, use_cache=False)
     stream = torch.cuda.Stream()
     b.set_stream(stream.cuda_stream)
     t, dt = b.run(args.warmup, dt0, forward=False)
@@ -328,7 +339,7 @@ def main():
     op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3, courant=courant,
                            solver=swe.SOLVER_AUG_RIEMANN, boundary=swe.BC_OUTFLOW)
     op.upload(*state)
-    tuned = autotune.tune(op, dt0, use_cache=False)
+    tuned = fixed_tuning(args, op) or autotune.tune(op, dt0, use_cache=False)
     op.set_stream(stream.cuda_stream)
 
     # warmup: first steps of the run (also captures the graph)

@@ -32,21 +32,14 @@ __constant__ double c_leg_adj[64];             // l(k) = sqrt(0.5)**k at k + 1
 
 template <int SOLVER, int CH>
 __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
-    // Columns [0, CH): the chunk's cells; [CH, 2 CH + 2): its halo
-    // neighbours (a run of m consecutive SFC cells has at most m + 2 edges
-    // leaving it), staged before the edge loop so that the loop reads LDS only.
-    // The register-bound occupancy (two waves per SIMD) leaves the LDS room.
-    constexpr int NQ = 2 * CH + 2;
-    __shared__ double s_q[4][NQ];        // eta, hu, hv, b at t_n
-    __shared__ double s_seed[3][NQ];     // cells: s * lam^{n+1} after the limiter;
-                                         // halo: the same times the shared edge's length
+    __shared__ double s_q[4][CH];        // eta, hu, hv, b of the chunk at t_n
+    __shared__ double s_seed[3][CH];     // s * lam^{n+1} after the limiter (per cell)
     __shared__ double s_g[4][3][CH];     // transpose-flux contributions per (edge, cell)
     __shared__ int32_t s_nbr[3][CH];
-    __shared__ uint16_t s_hslot[3][CH];  // column of a halo neighbour
     __shared__ int8_t s_plot[CH];
     __shared__ int8_t s_dep[CH];
     __shared__ uint16_t s_list[3 * CH];
-    __shared__ int s_wsum[CH / 64], s_hsum[CH / 64];
+    __shared__ int s_wsum[CH / 64];
 
     __shared__ double s_normal[17 * 3 * 2];  // geometry tables in LDS (see swe_step.hip)
     __shared__ double s_leg[64];
@@ -120,7 +113,7 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
 #pragma unroll
             for (int m = 0; m < 4; ++m) s_g[m][k][t] = 0.0;
 
-        int own = 0, halo = 0;
+        int own = 0;
         if (active) {
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -128,52 +121,24 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
                 const int64_t o = (int64_t)(p >> 2);
                 const bool inside = p >= 0 && o >= lbase && o < lbase + cnt;
                 if (!inside || o > lc_t) own |= 1 << k;
-                if (p >= 0 && !inside) halo |= 1 << k;
             }
         }
-        const int nown = __popc(own), nhalo = __popc(halo);
+        const int nown = __popc(own);
         const uint64_t b0 = __ballot(nown & 1), b1 = __ballot(nown & 2);
-        const uint64_t h0 = __ballot(nhalo & 1), h1 = __ballot(nhalo & 2);
         const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
         const int wpre = __popcll(b0 & lt) + 2 * __popcll(b1 & lt);
-        const int whpre = __popcll(h0 & lt) + 2 * __popcll(h1 & lt);
-        if (lane == 0) {
-            s_wsum[wave] = __popcll(b0) + 2 * __popcll(b1);
-            s_hsum[wave] = __popcll(h0) + 2 * __popcll(h1);
-        }
+        if (lane == 0) s_wsum[wave] = __popcll(b0) + 2 * __popcll(b1);
         __syncthreads();
-        int pre = wpre, total = 0, hpre = CH + whpre;
+        int pre = wpre, total = 0;
 #pragma unroll
         for (int w = 0; w < CH / 64; ++w) {
             const int ws = s_wsum[w];
             pre += w < wave ? ws : 0;
-            hpre += w < wave ? s_hsum[w] : 0;
             total += ws;
         }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
+        for (int k = 0; k < 3; ++k)
             if (own & (1 << k)) s_list[pre++] = (uint16_t)((t << 2) | k);
-            if (halo & (1 << k)) {
-                // halo neighbour: state and its seed on the shared edge
-                const int32_t p = nb[k];
-                const int64_t o = (int64_t)(p >> 2);
-                const int ko = p & 3;
-                const int64_t x = gs + o;
-                s_q[0][hpre] = a.eta[x];
-                s_q[1][hpre] = a.hu[x];
-                s_q[2][hpre] = a.hv[x];
-                s_q[3][hpre] = a.b[x];
-                const int dl = sc.depth >= 0 ? sc.depth : a.depth[gt + o];
-                const int mk = a.masks[x];
-                const double vol = (S * S) * ldexp(1.0, -(dl + 1));
-                const double f = (mk & 2) ? 0.0
-                                          : (-dt / vol) * (S * (ko == 1 ? s_leg[dl] : s_leg[dl + 1]));
-                s_seed[0][hpre] = f * a.leta[x];
-                s_seed[1][hpre] = f * a.lhu[x];
-                s_seed[2][hpre] = f * a.lhv[x];
-                s_hslot[k][t] = (uint16_t)hpre++;
-            }
-        }
         __syncthreads();
 
         for (int e = t; e < total; e += CH) {
@@ -215,14 +180,23 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
                 so1 = s_seed[1][lo] * len_o;
                 so2 = s_seed[2][lo] * len_o;
             } else {
-                const int ho = s_hslot[k][lc];
-                oe = s_q[0][ho];
-                ou = s_q[1][ho];
-                ov = s_q[2][ho];
-                ob = s_q[3][ho];
-                so0 = s_seed[0][ho];
-                so1 = s_seed[1][ho];
-                so2 = s_seed[2][ho];
+                const int64_t x = gs + o;
+                oe = a.eta[x];
+                ou = a.hu[x];
+                ov = a.hv[x];
+                ob = a.b[x];
+                const int dl = sc.depth >= 0 ? sc.depth : a.depth[gt + o];
+                const int mk = a.masks[x];
+                // the neighbour's seed exactly as its own chunk forms it
+                // ((s * lam) after the limiter, times the edge length), so
+                // that results do not depend on the chunking
+                const double vol = (S * S) * ldexp(1.0, -(dl + 1));
+                const double scale = -dt / vol;
+                const double len_o = S * (ko == 1 ? s_leg[dl] : s_leg[dl + 1]);
+                const bool dry = (mk & 2) != 0;
+                so0 = (scale * (dry ? 0.0 : a.leta[x])) * len_o;
+                so1 = (scale * (dry ? 0.0 : a.lhu[x])) * len_o;
+                so2 = (scale * (dry ? 0.0 : a.lhv[x])) * len_o;
             }
             const bool c_left = bnd || (a.forward ? (cc < o) : (cc > o));
             const double sx = c_left ? nx : -nx, sy = c_left ? ny : -ny;

@@ -244,3 +244,34 @@ def test_batch_adjoint_equals_single_runs():
         col += len(gl[s])
         one.close()
     batch.close()
+
+
+def test_adjoint_step_independent_of_launch_shape():
+    """Cells whose neighbour lies in another chunk get its seed formed exactly
+    as that chunk forms it, so the chunk size does not change a bit."""
+    d, S = 11, 100.0
+    mesh = make_mesh(d)
+    st = wetdry_state(mesh, 77, S)
+    dt = po.initial_dt(mesh.depth, st, 0.45) * S
+    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=DRY, boundary=1)
+    stream = torch.cuda.current_stream().cuda_stream
+    dq = [torch.tensor(a, device=DEV) for a in st]
+    out = [torch.empty_like(dq[0]) for _ in range(3)]
+    dtd = torch.tensor([dt], dtype=torch.float64, device=DEV)
+    masks = torch.empty(mesh.n, dtype=torch.int8, device=DEV)
+    rng = np.random.default_rng(5)
+    lin = [torch.tensor(rng.normal(size=mesh.n), device=DEV) for _ in range(3)]
+    results = []
+    for chunk, grid in ((256, 0), (128, 0), (512, 0), (256, 16), (128, 2048)):
+        print("case: launch shape", chunk, grid, flush=True)
+        op.set_tuning(chunk, grid)
+        op.forward_device(*map(P, dq), *map(P, out), P(dtd), True, masks=P(masks), stream=stream)
+        lout = [torch.empty_like(dq[0]) for _ in range(3)]
+        gb = torch.zeros_like(dq[0])
+        op.adjoint_device(*map(P, dq), P(masks), *map(P, lin), *map(P, lout), P(gb), P(dtd), True,
+                          stream=stream)
+        torch.cuda.synchronize()
+        results.append(torch.stack(lout + [gb]).cpu())
+    op.close()
+    for r in results[1:]:
+        assert torch.equal(r, results[0])
