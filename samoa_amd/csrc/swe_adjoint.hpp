@@ -17,8 +17,25 @@
 
 namespace swe {
 
+// Reciprocal for the reverse sweep, whose products need about 1e-15
+// relative accuracy rather than correct rounding: on the device the hardware
+// estimate refined by two Newton-Raphson steps (r <- r + r (1 - x r)), five
+// instructions instead of a correctly rounded division's eleven.  Values that
+// steer a branch keep the reference's exact division.
+SWE_HD double rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+
 // d sqrt(x)/dx with value y = sqrt(x), 0 at y == 0.
-SWE_HD double dsqrt(double y) { return y > 0.0 ? 0.5 / y : 0.0; }
+SWE_HD double dsqrt(double y) { return y > 0.0 ? 0.5 * rcp(y) : 0.0; }
 
 // Reverse of riemanntype (main call of riemann_aug_JCP): seeds gs1m, gs2m;
 // accumulates into ghL, ghR, guL, guR, gsqL, gsqR.
@@ -68,20 +85,23 @@ SWE_HD void riemanntype_adj(double hL, double hR, double uL, double uR, double s
         }
     } else if (F_max <= 0.0) {  // two shocks
         const double h0 = h_max;
-        const double iL = 1.0 / h0 + 1.0 / hL, iR = 1.0 / h0 + 1.0 / hR;
+        const double ih0 = rcp(h0), ihL = rcp(hL), ihR = rcp(hR);
+        const double iL = ih0 + ihL, iR = ih0 + ihR;
         const double gL = sqrt((0.5 * g) * iL), gR = sqrt((0.5 * g) * iR);
         const double F0 = (delu + (h0 - hL) * gL) + (h0 - hR) * gR;
         const double h02 = h0 * h0;
-        const double e1 = g * (h0 - hL), e2 = (4.0 * h02) * gL, t1 = e1 / e2;
-        const double e3 = g * (h0 - hR), e4 = (4.0 * h02) * gR, t2 = e3 / e4;
+        const double e1 = g * (h0 - hL), e2 = (4.0 * h02) * gL, ie2 = rcp(e2), t1 = e1 * ie2;
+        const double e3 = g * (h0 - hR), e4 = (4.0 * h02) * gR, ie4 = rcp(e4), t2 = e3 * ie4;
         const double dfdh = ((gL - t1) + gR) - t2;
         const double sq0 = sqrt(h0);
         const double slope = (2.0 * sq0) * dfdh;
-        const double fs = F0 / slope;
+        const double islope = rcp(slope);
+        const double fs = F0 * islope;
         const double q = sq0 - fs;
         const double hm = q * q;
-        const double sL_ = sqrt((0.5 * g) * (1.0 / hm + 1.0 / hL));
-        const double sR_ = sqrt((0.5 * g) * (1.0 / hm + 1.0 / hR));
+        const double ihm = rcp(hm);
+        const double sL_ = sqrt((0.5 * g) * (ihm + ihL));
+        const double sR_ = sqrt((0.5 * g) * (ihm + ihR));
         const double sqm = sqrt(g * hm);
         // s1m = u1m - sqm, s2m = u2m + sqm
         const double gu1m = gs1m, gu2m = gs2m;
@@ -94,8 +114,8 @@ SWE_HD void riemanntype_adj(double hL, double hR, double uL, double uR, double s
         ghR -= gu2m * sR_;
         {
             const double gjR = gu2m * (hm - hR) * dsqrt(sR_) * (0.5 * g);
-            ghm -= gjR / (hm * hm);
-            ghR -= gjR / (hR * hR);
+            ghm -= gjR * (ihm * ihm);
+            ghR -= gjR * (ihR * ihR);
         }
         // u1m = uL - (hm - hL) sL_
         guL += gu1m;
@@ -103,14 +123,14 @@ SWE_HD void riemanntype_adj(double hL, double hR, double uL, double uR, double s
         ghL += gu1m * sL_;
         {
             const double gjL = -gu1m * (hm - hL) * dsqrt(sL_) * (0.5 * g);
-            ghm -= gjL / (hm * hm);
-            ghL -= gjL / (hL * hL);
+            ghm -= gjL * (ihm * ihm);
+            ghL -= gjL * (ihL * ihL);
         }
         const double gq = 2.0 * q * ghm;
         double gsq0 = gq;
         const double gfs = -gq;
-        double gF0 = gfs / slope;
-        const double gslope = -gfs * fs / slope;
+        double gF0 = gfs * islope;
+        const double gslope = -gfs * fs * islope;
         gsq0 += gslope * 2.0 * dfdh;
         const double gdfdh = gslope * (2.0 * sq0);
         ggL += gdfdh;
@@ -118,12 +138,12 @@ SWE_HD void riemanntype_adj(double hL, double hR, double uL, double uR, double s
         const double gt1 = -gdfdh, gt2 = -gdfdh;
         double gh02 = 0.0;
         {
-            const double ge3 = gt2 / e4, ge4 = -gt2 * t2 / e4;
+            const double ge3 = gt2 * ie4, ge4 = -gt2 * t2 * ie4;
             gh0 += ge3 * g;
             ghR -= ge3 * g;
             gh02 += ge4 * 4.0 * gR;
             ggR += ge4 * (4.0 * h02);
-            const double ge1 = gt1 / e2, ge2 = -gt1 * t1 / e2;
+            const double ge1 = gt1 * ie2, ge2 = -gt1 * t1 * ie2;
             gh0 += ge1 * g;
             ghL -= ge1 * g;
             gh02 += ge2 * 4.0 * gL;
@@ -140,21 +160,21 @@ SWE_HD void riemanntype_adj(double hL, double hR, double uL, double uR, double s
         ggR += gF0 * (h0 - hR);
         {
             const double giR = ggR * dsqrt(gR) * (0.5 * g);
-            gh0 -= giR / (h0 * h0);
-            ghR -= giR / (hR * hR);
+            gh0 -= giR * (ih0 * ih0);
+            ghR -= giR * (ihR * ihR);
             const double giL = ggL * dsqrt(gL) * (0.5 * g);
-            gh0 -= giL / (h0 * h0);
-            ghL -= giL / (hL * hL);
+            gh0 -= giL * (ih0 * ih0);
+            ghL -= giL * (ihL * ihL);
         }
         ghmax += gh0;
         (void)gF0;
     } else {  // one shock, one rarefaction
         const double h0 = h_min;
-        const double F0 = (delu + 2.0 * (sqh_min - sqh_max)) +
-                          (h0 - h_min) * sqrt((0.5 * g) * (1.0 / h0 + 1.0 / h_min));
-        const double den = h_max - h_min;
-        const double slope = (F_max - F0) / den;
-        const double fs = F0 / slope;
+        const double F0 = F_min + 0.0;  // the reference's second term is an exact zero
+        const double den = h_max - h_min, iden = rcp(den);
+        const double slope = (F_max - F0) * iden;
+        const double islope = rcp(slope);
+        const double fs = F0 * islope;
         const double hm = h0 - fs;
         const double sqm = sqrt(g * hm);
         double gsqm;
@@ -172,9 +192,9 @@ SWE_HD void riemanntype_adj(double hL, double hR, double uL, double uR, double s
         const double ghm = gsqm * g * dsqrt(sqm);
         double gh0 = ghm;
         const double gfs = -ghm;
-        double gF0 = gfs / slope;
-        const double gslope = -gfs * fs / slope;
-        const double gnum = gslope / den, gden = -gslope * slope / den;
+        double gF0 = gfs * islope;
+        const double gslope = -gfs * fs * islope;
+        const double gnum = gslope * iden, gden = -gslope * slope * iden;
         const double gFmax = gnum;
         gF0 -= gnum;
         ghmax += gden;
@@ -184,7 +204,8 @@ SWE_HD void riemanntype_adj(double hL, double hR, double uL, double uR, double s
         ghmax += gFmax * sqY;
         ghmin -= gFmax * sqY;
         const double gY = gFmax * (h_max - h_min) * dsqrt(sqY);
-        const double gWn = gY / Wd, gWd = -gY * Y / Wd;
+        const double iWd = rcp(Wd);
+        const double gWn = gY * iWd, gWd = -gY * Y * iWd;
         ghmax += gWn * (0.5 * g) + gWd * h_min;
         ghmin += gWn * (0.5 * g) + gWd * h_max;
         // F0 = delu + 2 (sqmin - sqmax) (+ an exact zero)
@@ -305,8 +326,9 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
         gvL2 += gf20 * beta1;
         (void)f20;
         (void)f22;
-        const double gn2 = gb2 / dd, gn1 = gb1 / dd;
-        const double gdd = -(gb1 * beta1 + gb2 * beta2) / dd;
+        const double idd = rcp(dd);
+        const double gn2 = gb2 * idd, gn1 = gb1 * idd;
+        const double gdd = -(gb1 * beta1 + gb2 * beta2) * idd;
         double gdec = gn2 - gn1;
         gsE1 -= gn2 * delhu;
         gdelhu -= gn2 * sE1;
@@ -428,8 +450,9 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
         double gbeta0 = gf[1][0] * r20 + gf00 * l0;
         double gr20 = gf[1][0] * beta0;
         double gl0 = gf00 * beta0;
-        const double gn0 = gbeta0 / det, gn1 = gbeta1 / det, gn2 = gbeta2 / det;
-        const double gdet = -(gbeta0 * beta0 + gbeta1 * beta1 + gbeta2 * beta2) / det;
+        const double idet = rcp(det);
+        const double gn0 = gbeta0 * idet, gn1 = gbeta1 * idet, gn2 = gbeta2 * idet;
+        const double gdet = -(gbeta0 * beta0 + gbeta1 * beta1 + gbeta2 * beta2) * idet;
         double gd0 = 0, gd1 = 0, gd2 = 0;
         // n2 = -d1 + d0 l0
         gd1 -= gn2;
@@ -479,8 +502,9 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
             ghbar -= gp0 * g * delb;
             gdelb -= gp0 * (g * hbar);
         } else {  // p0 = -(Kp / s1s2bar), Kp = M s1s2tilde
-            const double gKp = -gp0 / s1s2bar;
-            gs1s2bar += gp0 * Kp / (s1s2bar * s1s2bar);
+            const double ib = rcp(s1s2bar);
+            const double gKp = -gp0 * ib;
+            gs1s2bar += gp0 * Kp * (ib * ib);
             gM += gKp * s1s2tilde;
             gs1s2tilde += gKp * M;
         }
@@ -507,20 +531,23 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
         // B2 = (H dl) / l2, B1 = (H dl) / l0 (only where a bound was taken:
         // an unused bound may divide by zero)
         if (gB2 != 0.0) {
-            gH += gB2 * dl / l2;
-            gdl += gB2 * hstarHLL / l2;
-            gl2 -= gB2 * B2 / l2;
+            const double il2 = rcp(l2);
+            gH += gB2 * dl * il2;
+            gdl += gB2 * hstarHLL * il2;
+            gl2 -= gB2 * B2 * il2;
         }
         if (gB1 != 0.0) {
-            gH += gB1 * dl / l0;
-            gdl += gB1 * hstarHLL / l0;
-            gl0 -= gB1 * B1 / l0;
+            const double il0 = rcp(l0);
+            gH += gB1 * dl * il0;
+            gdl += gB1 * hstarHLL * il0;
+            gl0 -= gB1 * B1 * il0;
         }
         if (sonic) {
             gdelb -= gdd0;
         } else {  // dd0 = M / s1s2bar
-            gM += gdd0 / s1s2bar;
-            gs1s2bar -= gdd0 * dd0 / s1s2bar;
+            const double ib = rcp(s1s2bar);
+            gM += gdd0 * ib;
+            gs1s2bar -= gdd0 * dd0 * ib;
         }
         // M = (delb g) hbar
         gdelb += gM * g * hbar;
@@ -546,8 +573,9 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
         gl0 += 2.0 * l0 * gr20;
         // hstarHLL = max(xh / dl, 0)
         if (hpos) {
-            const double gxh = gH / dl;
-            gdl -= gH * xq / dl;
+            const double idl = rcp(dl);
+            const double gxh = gH * idl;
+            gdl -= gH * xq * idl;
             ghuL2 += gxh;
             ghuR2 -= gxh;
             gl2 += gxh * hR2;
@@ -587,7 +615,8 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
         const double gcx = gchat * dsqrt(chat);
         ghR2 += gcx * (g * 0.5);
         ghL2 += gcx * (g * 0.5);
-        const double gnum = guhat / den, gden = -guhat * uhat / den;
+        const double iden = rcp(den);
+        const double gnum = guhat * iden, gden = -guhat * uhat * iden;
         gsqL += gnum * uL2 + gden;
         guL2 += gnum * sqL;
         gsqR += gnum * uR2 + gden;
@@ -667,14 +696,16 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
     }
     // stage A reverse
     if (wetR) {
-        gin[1] += ghR1 - guR1 * uR1 / hR - gvR1 * vR1 / hR;
-        gin[3] += ghuR1 + guR1 / hR;
-        gin[5] += gvR1 / hR;
+        const double ih = rcp(hR);
+        gin[1] += ghR1 - guR1 * uR1 * ih - gvR1 * vR1 * ih;
+        gin[3] += ghuR1 + guR1 * ih;
+        gin[5] += gvR1 * ih;
     }
     if (wetL) {
-        gin[0] += ghL1 - guL1 * uL1 / hL - gvL1 * vL1 / hL;
-        gin[2] += ghuL1 + guL1 / hL;
-        gin[4] += gvL1 / hL;
+        const double ih = rcp(hL);
+        gin[0] += ghL1 - guL1 * uL1 * ih - gvL1 * vL1 * ih;
+        gin[2] += ghuL1 + guL1 * ih;
+        gin[4] += gvL1 * ih;
     }
     gin[6] += gbL;
     gin[7] += gbR;

@@ -566,6 +566,7 @@ int batch_forward_device(Batch &B, const double *eta, const double *hu, const do
     a.dt_host = dt_dev;
     a.masks = masks;
     a.out_dtcfl = dt_cfl_dev;
+    a.skip_finalize = dt_cfl_dev ? 0 : 1;  // dt comes from the caller: no result is read
     HIP_TRY(swe::launch_step(a, B.solver, B.chunk, B.grid, st));
     return SWE_OK;
 }

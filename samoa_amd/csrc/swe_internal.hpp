@@ -59,6 +59,7 @@ struct StepArgs {
     double g, inv16g;                      // inv16g = 1 / (16 g), riemannsolvers.f:556
     int bc, forward;
     int device_dt;                         // 1: read dt from slots[s].dt and write the next one
+    int skip_finalize;                     // 1: nobody reads this step's CFL results
 };
 
 // Discrete adjoint of one traversal (swe_adjoint_step.hip).

@@ -370,7 +370,8 @@ static void launch_t(const StepArgs &a, int64_t grid_blocks, hipStream_t s) {
     const int64_t blocks = grid_blocks > 0 && grid_blocks < nvirt ? grid_blocks : nvirt;
     hipLaunchKernelGGL((euler_step_kernel<SOLVER, CH>), dim3((unsigned)blocks), dim3(CH), 0, s,
                        a);
-    hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)a.n_scen), dim3(FIN_THREADS), 0, s, a);
+    if (!a.skip_finalize)
+        hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)a.n_scen), dim3(FIN_THREADS), 0, s, a);
 }
 
 hipError_t launch_step(const StepArgs &a, int solver, int chunk, int64_t grid_blocks,
