@@ -13,6 +13,7 @@
 module swe_driver
     use iso_c_binding
     use swe_mi355x
+    use swe_vtu
     use swe_sfc_mesh
     use swe_scenarios
     implicit none
@@ -72,7 +73,7 @@ contains
     !> If host_steps is false and no time limit is set, the steps run on the
     !> device through swe_run.  Returns a SWE_* status.
     integer(c_int) function swe_time_loop(handle, courant, tmax, tout, nmax, host_steps, &
-                                          dt, time, forward, steps) result(rc)
+                                          dt, time, forward, steps, refine) result(rc)
         type(c_ptr), intent(in)            :: handle
         real(c_double), intent(in)         :: courant, tmax, tout
         integer(c_int64_t), intent(in)     :: nmax
@@ -80,13 +81,18 @@ contains
         real(c_double), intent(inout)      :: dt, time
         logical, intent(inout)             :: forward
         integer(c_int64_t), intent(out)    :: steps
+        !> int8 [n] or c_null_ptr: refinement flags of the last step (host steps only)
+        type(c_ptr), intent(in), optional  :: refine
+        type(c_ptr) :: rf
         real(c_double) :: dt_cfl, clamp
         integer(c_int64_t) :: nref
         integer(c_int) :: fw
 
         steps = 0
         rc = SWE_OK
-        if (.not. host_steps .and. tmax < 0.0_c_double .and. nmax >= 0) then
+        rf = c_null_ptr
+        if (present(refine)) rf = refine
+        if (.not. host_steps .and. tmax < 0.0_c_double .and. nmax >= 0 .and. .not. c_associated(rf)) then
             clamp = -1.0_c_double
             if (tout > 0.0_c_double) clamp = tout
             if (tout > 0.0_c_double) dt = min(tout, dt)
@@ -104,7 +110,7 @@ contains
             if (tmax > 0.0_c_double) dt = min(tmax, dt)
             if (tout > 0.0_c_double) dt = min(tout, dt)
             fw = merge(1_c_int, 0_c_int, forward)
-            rc = swe_step(handle, dt, fw, dt_cfl, nref, c_null_ptr)
+            rc = swe_step(handle, dt, fw, dt_cfl, nref, rf)
             if (rc /= SWE_OK) return
             time = time + dt
             dt = courant * dt_cfl
@@ -192,5 +198,42 @@ contains
         swe_host_time_loop = swe_time_loop(handle, courant, tmax, tout, nmax, host_steps /= 0, &
                                            dt, time, fw, steps)
         forward = merge(1_c_int, 0_c_int, fw)
+    end function
+    !> VTU file of a cell state (swe_vtu); path is a NUL-terminated string.
+    !> forward: 1 if the output traversal runs forward.  Returns 0 or an I/O
+    !> status.
+    integer(c_int) function swe_host_write_vtu(path, n, coords, plotter, depth, eta, b, hu, hv, &
+                                               refine, time, scaling, offset, forward) &
+            bind(c, name='swe_host_write_vtu')
+        character(kind=c_char), intent(in) :: path(*)
+        integer(c_int64_t), value          :: n
+        real(c_double), intent(in)         :: coords(2, 3, n), eta(n), b(n), hu(n), hv(n)
+        integer(c_int8_t), intent(in)      :: plotter(n), depth(n), refine(n)
+        real(c_double), value              :: time, scaling, offset
+        integer(c_int), value              :: forward
+        integer :: ierr
+        call vtu_write(c_string(path), n, coords, plotter, depth, eta, b, hu, hv, refine, time, &
+                       scaling, offset, forward /= 0, ierr)
+        swe_host_write_vtu = ierr
+    end function
+
+    !> The single-piece .pvtu master file naming `piece`.
+    integer(c_int) function swe_host_write_pvtu(path, piece) bind(c, name='swe_host_write_pvtu')
+        character(kind=c_char), intent(in) :: path(*), piece(*)
+        integer :: ierr
+        call pvtu_write(c_string(path), c_string(piece), ierr)
+        swe_host_write_pvtu = ierr
+    end function
+
+    function c_string(cs) result(s)
+        character(kind=c_char), intent(in) :: cs(*)
+        character(len=:), allocatable :: s
+        integer :: k
+        k = 0
+        do while (cs(k + 1) /= c_null_char)
+            k = k + 1
+        end do
+        allocate(character(len=k) :: s)
+        s = transfer(cs(1:k), s)
     end function
 end module swe_driver

@@ -5,28 +5,32 @@
 !   swe_mi355x -dmax 14 -nmax 100 [-dmin d] [-tmax t] [-tout t] [-nout k]
 !              [-courant c] [-drytolerance d] [-scenario dambreak|radial|tsunami]
 !              [-seed s] [-boundary outflow|wall] [-solver aug_riemann|fwave]
-!              [-dump prefix] [-hoststeps]
+!              [-dump prefix] [-hoststeps] [-xmloutput .true.] [-output_dir dir]
 !
 ! Like the reference CLI (Config.f90:213-296, M_kracken rget), -courant,
 ! -drytolerance, -tmax and -tout are read as default REAL before they are
 ! stored in double precision.  With -nout k, every k-th step is followed by an
 ! output traversal: the direction flips once more and, with -dump, the state
-! is written to <prefix>_<i>.bin (int64 n, float64 time, eta, hu, hv, b).
+! is written to <prefix>_<i>.bin (int64 n, float64 time, eta, hu, hv, b);
+! with -xmloutput .true., <output_dir>/swe_<i>_r0_s1.vtu and swe_<i>.pvtu are
+! written in the reference's VTK layout (SWE_xml_output.f90), cells in the
+! order of the output traversal.
 program swe_mi355x_main
     use iso_c_binding
     use swe_mi355x
     use swe_sfc_mesh
     use swe_scenarios
     use swe_driver
+    use swe_vtu
     implicit none
 
     integer :: dmax, dmin, nout, i, narg, ierr, depth_i
     integer(c_int64_t) :: nmax, n, seed, steps, k, nsteps_done, chunk
     real :: r_courant, r_drytol, r_tmax, r_tout
     real(c_double) :: courant, drytol, tmax, tout, scaling, offset, dt, time, amp, ratio, x0, y0
-    character(len=256) :: arg, val, scenario, dump, bcname, solvername
-    logical :: host_steps, forward
-    integer(c_int8_t), allocatable, target :: plotter(:), depth(:)
+    character(len=256) :: arg, val, scenario, dump, bcname, solvername, outdir
+    logical :: host_steps, forward, xml
+    integer(c_int8_t), allocatable, target :: plotter(:), depth(:), refine(:)
     integer(c_int32_t), allocatable, target :: nbr(:, :)
     real(c_double), allocatable, target :: coords(:, :, :), eta(:), hu(:), hv(:), b(:)
     type(swe_mesh_desc) :: desc
@@ -38,7 +42,7 @@ program swe_mi355x_main
     dmax = 14; dmin = -1; nmax = -1; nout = -1
     r_courant = 0.45; r_drytol = 0.01; r_tmax = 3600.0; r_tout = -1.0
     scenario = 'dambreak'; dump = ''; bcname = 'outflow'; solvername = 'aug_riemann'
-    seed = 0; host_steps = .false.
+    seed = 0; host_steps = .false.; xml = .false.; outdir = 'output' 
     narg = command_argument_count()
     i = 1
     do while (i <= narg)
@@ -60,6 +64,8 @@ program swe_mi355x_main
         case ('-solver'); solvername = val; i = i + 1
         case ('-dump'); dump = val; i = i + 1
         case ('-hoststeps'); host_steps = .true.
+        case ('-xmloutput'); read(val, *) xml; i = i + 1
+        case ('-output_dir'); outdir = val; i = i + 1
         case default
             print '(A, A)', 'swe_mi355x: unknown argument ', trim(arg)
             stop 2
@@ -70,7 +76,8 @@ program swe_mi355x_main
     courant = r_courant; drytol = r_drytol; tmax = r_tmax; tout = r_tout
 
     n = ishft(1_c_int64_t, dmax + 1)
-    allocate(plotter(n), depth(n), nbr(3, n), coords(2, 3, n), eta(n), hu(n), hv(n), b(n))
+    allocate(plotter(n), depth(n), nbr(3, n), coords(2, 3, n), eta(n), hu(n), hv(n), b(n), refine(n))
+    refine = 0
     call sfc_uniform_mesh(dmax, plotter, nbr, coords, ierr)
     if (ierr /= 0) then
         print '(A, I0)', 'swe_mi355x: mesh generation failed: ', ierr
@@ -122,6 +129,7 @@ program swe_mi355x_main
     iout = 0
     if (nout > 0) then
         if (len_trim(dump) > 0) call write_dump(iout)
+        if (xml) call write_xml(iout)
         iout = iout + 1
         forward = .not. forward
     end if
@@ -139,16 +147,23 @@ program swe_mi355x_main
                 chunk = min(chunk, nmax - nsteps_done)
             end if
         end if
-        rc = swe_time_loop(h, courant, tmax, tout, chunk, host_steps .or. tmax >= 0.0_c_double, &
-                           dt, time, forward, steps)
+        if (xml .and. dmin < dmax) then
+            ! the output carries the refinement flags of the last step
+            rc = swe_time_loop(h, courant, tmax, tout, chunk, .true., dt, time, forward, steps, &
+                               c_loc(refine))
+        else
+            rc = swe_time_loop(h, courant, tmax, tout, chunk, host_steps .or. tmax >= 0.0_c_double, &
+                               dt, time, forward, steps)
+        end if
         call check(rc, 'swe_time_loop')
         nsteps_done = nsteps_done + steps
         if (nout > 0 .and. mod(nsteps_done, int(nout, c_int64_t)) == 0) then
-            if (len_trim(dump) > 0) then
+            if (len_trim(dump) > 0 .or. xml) then
                 rc = swe_download(h, c_loc(eta), c_loc(hu), c_loc(hv), c_loc(b))
                 call check(rc, 'swe_download')
-                call write_dump(iout)
             end if
+            if (len_trim(dump) > 0) call write_dump(iout)
+            if (xml) call write_xml(iout)
             iout = iout + 1
             forward = .not. forward
         end if
@@ -182,6 +197,22 @@ contains
         open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
         write(u) n, time, eta, hu, hv, b
         close(u)
+    end subroutine
+
+    subroutine write_xml(idx)
+        integer, intent(in) :: idx
+        character(len=512) :: piece, fname
+        integer :: ierr2
+        write(piece, '(A, I0, A)') 'swe_', idx, '_r0_s1.vtu'
+        call vtu_write(trim(outdir) // '/' // trim(piece), n, coords, plotter, depth, eta, b, hu, hv, &
+                       refine, time, scaling, offset, forward, ierr2)
+        if (ierr2 /= 0) then
+            print '(A, A)', 'swe_mi355x: cannot write ', trim(outdir) // '/' // trim(piece)
+            stop 1
+        end if
+        write(fname, '(A, A, I0, A)') trim(outdir), '/swe_', idx, '.pvtu'
+        call pvtu_write(trim(fname), trim(piece), ierr2)
+        if (ierr2 /= 0) stop 1
     end subroutine
 
     subroutine swe_host_reference_dam_break_f(nc, x, e, u, v, bb)

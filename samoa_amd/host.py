@@ -48,6 +48,11 @@ def load():
         L.swe_host_time_loop.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int64,
                                          C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                          C.POINTER(C.c_int), C.POINTER(C.c_int64)]
+        L.swe_host_write_vtu.restype = C.c_int
+        L.swe_host_write_vtu.argtypes = [C.c_char_p, C.c_int64, _dp, _i8p, _i8p, _dp, _dp, _dp, _dp,
+                                         _i8p, C.c_double, C.c_double, C.c_double, C.c_int]
+        L.swe_host_write_pvtu.restype = C.c_int
+        L.swe_host_write_pvtu.argtypes = [C.c_char_p, C.c_char_p]
         _lib = L
     return _lib
 
@@ -119,3 +124,24 @@ def time_loop(op, *, dt: float, time: float = 0.0, forward: bool = False, nmax: 
                                    C.byref(d), C.byref(t), C.byref(fw), C.byref(steps))
     _native.check("swe_host_time_loop", rc)
     return d.value, t.value, bool(fw.value), steps.value
+
+
+def write_vtu(path: str, mesh: Mesh, state, *, time: float, scaling: float, offset: float,
+              forward: bool, refine=None) -> None:
+    """The reference's VTK layout (fortran/swe_vtu.f90): cells in the order of
+    an output traversal running forward or backward along the curve."""
+    eta, hu, hv, b = (np.ascontiguousarray(a, dtype=np.float64) for a in state)
+    rf = np.zeros(mesh.n, np.int8) if refine is None else np.ascontiguousarray(refine, np.int8)
+    rc = load().swe_host_write_vtu(os.fsencode(path), mesh.n,
+                                   np.ascontiguousarray(mesh.coords.reshape(-1), np.float64),
+                                   np.ascontiguousarray(mesh.plotter, np.int8),
+                                   np.ascontiguousarray(mesh.depth, np.int8), eta, b, hu, hv, rf,
+                                   float(time), float(scaling), float(offset), int(bool(forward)))
+    if rc != 0:
+        raise OSError(f"swe_host_write_vtu({path}) failed with I/O status {rc}")
+
+
+def write_pvtu(path: str, piece: str) -> None:
+    rc = load().swe_host_write_pvtu(os.fsencode(path), piece.encode())
+    if rc != 0:
+        raise OSError(f"swe_host_write_pvtu({path}) failed with I/O status {rc}")

@@ -6,7 +6,8 @@ Runs only in a container that holds the reference tree and the binaries that
 library against them, so neither the reference nor its binaries are needed at
 test time.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py        (all fixtures)
+    python tests/golden/make_golden.py vtu    (the VTK output files only)
 
 Fixtures:
   kat_geoclaw.npz      inputs (n,10) + reference outputs of c_bind_geoclaw_solver
@@ -16,6 +17,7 @@ Fixtures:
   tables.npz           bit patterns of r_leg_sizes / r_volumes (SFC_data_types.f90:524,557)
   run_*.npz            full SWE runs of the reference binary: mesh point order,
                        per-output state (forward SFC order), time, refinement flags
+  vtu/                 the reference's own VTK output files of two small runs
 """
 from __future__ import annotations
 
@@ -180,9 +182,36 @@ def write_run(name: str, binary: str, depth: int, steps: int, nout: int, bc: int
     print(f"run_{name}: {len(outs)} outputs, n={n}")
 
 
+def write_vtu_fixtures() -> None:
+    """The reference's own VTK output files (SWE_xml_output.f90), unmodified:
+    dam break at depths 3 and 4, 4 steps, an output every 2 steps (so both
+    traversal directions occur).  Stored as vtu/d<d>_<i>.vtu plus one .pvtu."""
+    vdir = os.path.join(HERE, "vtu")
+    os.makedirs(vdir, exist_ok=True)
+    for d in (3, 4):
+        od = tempfile.mkdtemp(prefix="swe_vtu_")
+        try:
+            subprocess.run([os.path.join(REF, "samoa_swe"), "-threads", "1", "-sections", "1",
+                            "-dstart", str(d), "-dmin", str(d), "-dmax", str(d), "-nadapt", "-1",
+                            "-tmax", "-1", "-nmax", "4", "-xmloutput", ".true.", "-nout", "2",
+                            "-output_dir", od], check=True, capture_output=True)
+            for f in glob.glob(os.path.join(od, "*_r0_s1.vtu")):
+                i = int(f.split("_")[-3])
+                shutil.copy(f, os.path.join(vdir, f"d{d}_{i}.vtu"))
+            if d == 3:
+                p0 = sorted(glob.glob(os.path.join(od, "*_0.pvtu")))[0]
+                shutil.copy(p0, os.path.join(vdir, "d3_0.pvtu"))
+        finally:
+            shutil.rmtree(od, ignore_errors=True)
+    print("vtu fixtures:", sorted(os.listdir(vdir)))
+
+
 def main() -> None:
     if not os.path.exists(os.path.join(REF, "ref_kat")):
         sys.exit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["vtu"]:
+        write_vtu_fixtures()
+        return
     kin = kat_inputs()
     aug = run_kat("geoclaw", kin, "3", nout=7)
     fwave = run_kat("geoclaw", kin, "1", nout=7)
@@ -209,6 +238,7 @@ def main() -> None:
     write_run("bath_d6_n1", "samoa_swe_bath", 6, 30, 1, 1)
     write_run("bath_d7_n2", "samoa_swe_bath", 7, 40, 2, 1)
     write_run("bath_d9_n7", "samoa_swe_bath", 9, 35, 7, 1)
+    write_vtu_fixtures()
 
 
 if __name__ == "__main__":

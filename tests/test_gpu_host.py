@@ -41,6 +41,24 @@ def test_cli_matches_reference_run(name, hoststeps):
             assert np.array_equal(arr, z["states"][i]), f"output {i}"
 
 
+@pytest.mark.parametrize("d", [3, 4])
+def test_cli_vtk_output_identical_to_reference_files(d):
+    """The drop-in end to end: the CLI on the GPU writes the same VTK files,
+    byte for byte, as the reference binary did for the same run (dam break,
+    4 steps, output every 2 steps, tests/golden/vtu)."""
+    vdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "vtu")
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [host.CLI, "-scenario", "dambreak", "-dmax", str(d), "-nmax", "4", "-nout", "2",
+               "-tmax", "-1", "-xmloutput", ".true.", "-output_dir", td]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for i in range(3):
+            mine = open(os.path.join(td, f"swe_{i}_r0_s1.vtu"), "rb").read()
+            ref = open(os.path.join(vdir, f"d{d}_{i}.vtu"), "rb").read()
+            assert mine == ref, f"output {i}"
+            assert os.path.exists(os.path.join(td, f"swe_{i}.pvtu"))
+
+
 def test_fortran_time_loop_vs_oracle():
     d, S = 12, 1.0e6
     mesh = host.mesh_uniform(d)
