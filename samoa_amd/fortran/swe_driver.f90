@@ -14,6 +14,7 @@ module swe_driver
     use iso_c_binding
     use swe_mi355x
     use swe_vtu
+    use swe_gauges
     use swe_sfc_mesh
     use swe_scenarios
     implicit none
@@ -223,6 +224,30 @@ contains
         integer :: ierr
         call pvtu_write(c_string(path), c_string(piece), ierr)
         swe_host_write_pvtu = ierr
+    end function
+
+    !> Probe values of a cell state (swe_gauges): out(8, npt) =
+    !> x_unit, y_unit, p(1), p(2), h, b, centroid distance, time in days.
+    subroutine swe_host_gauges_sample(n, coords, eta, hu, hv, b, npt, px, py, scaling, offset, &
+                                      time, out) bind(c, name='swe_host_gauges_sample')
+        integer(c_int64_t), value      :: n
+        real(c_double), intent(in)     :: coords(2, 3, n), eta(n), hu(n), hv(n), b(n)
+        integer(c_int), value          :: npt
+        real(c_double), intent(in)     :: px(npt), py(npt)
+        real(c_double), value          :: scaling, offset, time
+        real(c_double), intent(out)    :: out(8, npt)
+        call gauges_sample(n, coords, eta, hu, hv, b, int(npt), px, py, scaling, offset, time, out)
+    end subroutine
+
+    !> The probe file; full /= 0 writes 17 significant digits instead of F0.15.
+    integer(c_int) function swe_host_gauges_write(path, npt, px, py, out, full) &
+            bind(c, name='swe_host_gauges_write')
+        character(kind=c_char), intent(in) :: path(*)
+        integer(c_int), value          :: npt, full
+        real(c_double), intent(in)     :: px(npt), py(npt), out(8, npt)
+        integer :: ierr
+        call gauges_write(c_string(path), int(npt), px, py, out, full /= 0, ierr)
+        swe_host_gauges_write = ierr
     end function
 
     function c_string(cs) result(s)

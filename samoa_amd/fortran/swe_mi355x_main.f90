@@ -6,6 +6,7 @@
 !              [-courant c] [-drytolerance d] [-scenario dambreak|radial|tsunami]
 !              [-seed s] [-boundary outflow|wall] [-solver aug_riemann|fwave]
 !              [-dump prefix] [-hoststeps] [-xmloutput .true.] [-output_dir dir]
+!              [-stestpoints "x y,x y,..."] [-pointoutput_full .true.]
 !
 ! Like the reference CLI (Config.f90:213-296, M_kracken rget), -courant,
 ! -drytolerance, -tmax and -tout are read as default REAL before they are
@@ -14,7 +15,11 @@
 ! is written to <prefix>_<i>.bin (int64 n, float64 time, eta, hu, hv, b);
 ! with -xmloutput .true., <output_dir>/swe_<i>_r0_s1.vtu and swe_<i>.pvtu are
 ! written in the reference's VTK layout (SWE_xml_output.f90), cells in the
-! order of the output traversal.
+! order of the output traversal; with -stestpoints, the probe file
+! <output_dir>/swe_d<dmax>_cou<c>_dry<d>_pointoutput_<i>.txt
+! (SWE_point_output.f90; 17 significant digits with -pointoutput_full).  As
+! in SWE.f90:405-418, every output traversal flips the direction: the VTK (or
+! -dump) output once, the probe output once more.
 program swe_mi355x_main
     use iso_c_binding
     use swe_mi355x
@@ -22,6 +27,7 @@ program swe_mi355x_main
     use swe_scenarios
     use swe_driver
     use swe_vtu
+    use swe_gauges
     implicit none
 
     integer :: dmax, dmin, nout, i, narg, ierr, depth_i
@@ -29,7 +35,10 @@ program swe_mi355x_main
     real :: r_courant, r_drytol, r_tmax, r_tout
     real(c_double) :: courant, drytol, tmax, tout, scaling, offset, dt, time, amp, ratio, x0, y0
     character(len=256) :: arg, val, scenario, dump, bcname, solvername, outdir
-    logical :: host_steps, forward, xml
+    character(len=4096) :: probes
+    logical :: host_steps, forward, xml, probe_full
+    real(c_double), allocatable :: px(:), py(:), pvals(:, :)
+    integer :: npt
     integer(c_int8_t), allocatable, target :: plotter(:), depth(:), refine(:)
     integer(c_int32_t), allocatable, target :: nbr(:, :)
     real(c_double), allocatable, target :: coords(:, :, :), eta(:), hu(:), hv(:), b(:)
@@ -42,7 +51,8 @@ program swe_mi355x_main
     dmax = 14; dmin = -1; nmax = -1; nout = -1
     r_courant = 0.45; r_drytol = 0.01; r_tmax = 3600.0; r_tout = -1.0
     scenario = 'dambreak'; dump = ''; bcname = 'outflow'; solvername = 'aug_riemann'
-    seed = 0; host_steps = .false.; xml = .false.; outdir = 'output' 
+    seed = 0; host_steps = .false.; xml = .false.; outdir = 'output'; probes = ''
+    probe_full = .false.
     narg = command_argument_count()
     i = 1
     do while (i <= narg)
@@ -66,6 +76,8 @@ program swe_mi355x_main
         case ('-hoststeps'); host_steps = .true.
         case ('-xmloutput'); read(val, *) xml; i = i + 1
         case ('-output_dir'); outdir = val; i = i + 1
+        case ('-stestpoints'); probes = val; i = i + 1
+        case ('-pointoutput_full'); read(val, *) probe_full; i = i + 1
         case default
             print '(A, A)', 'swe_mi355x: unknown argument ', trim(arg)
             stop 2
@@ -74,6 +86,7 @@ program swe_mi355x_main
     end do
     if (dmin < 0) dmin = dmax
     courant = r_courant; drytol = r_drytol; tmax = r_tmax; tout = r_tout
+    call parse_probes(probes)
 
     n = ishft(1_c_int64_t, dmax + 1)
     allocate(plotter(n), depth(n), nbr(3, n), coords(2, 3, n), eta(n), hu(n), hv(n), b(n), refine(n))
@@ -128,10 +141,8 @@ program swe_mi355x_main
     dt = initial_dt(n, depth, eta, hu, hv, b, 9.80665_c_double, courant)
     iout = 0
     if (nout > 0) then
-        if (len_trim(dump) > 0) call write_dump(iout)
-        if (xml) call write_xml(iout)
+        call output(iout)
         iout = iout + 1
-        forward = .not. forward
     end if
 
     call system_clock(t0, rate)
@@ -158,14 +169,12 @@ program swe_mi355x_main
         call check(rc, 'swe_time_loop')
         nsteps_done = nsteps_done + steps
         if (nout > 0 .and. mod(nsteps_done, int(nout, c_int64_t)) == 0) then
-            if (len_trim(dump) > 0 .or. xml) then
+            if (len_trim(dump) > 0 .or. xml .or. npt > 0) then
                 rc = swe_download(h, c_loc(eta), c_loc(hu), c_loc(hv), c_loc(b))
                 call check(rc, 'swe_download')
             end if
-            if (len_trim(dump) > 0) call write_dump(iout)
-            if (xml) call write_xml(iout)
+            call output(iout)
             iout = iout + 1
-            forward = .not. forward
         end if
         if (steps == 0) exit
     end do
@@ -197,6 +206,54 @@ contains
         open(newunit=u, file=trim(fname), access='stream', form='unformatted', status='replace')
         write(u) n, time, eta, hu, hv, b
         close(u)
+    end subroutine
+
+    ! One output phase (SWE.f90:405-418): the VTK output traversal (or the
+    ! -dump file standing for it), then the probe traversal; each flips the
+    ! traversal direction.
+    subroutine output(idx)
+        integer, intent(in) :: idx
+        integer :: ierr3
+        if (len_trim(dump) > 0) call write_dump(idx)
+        if (xml) call write_xml(idx)
+        if (len_trim(dump) > 0 .or. xml) forward = .not. forward
+        if (npt > 0) then
+            call gauges_sample(n, coords, eta, hu, hv, b, npt, px, py, scaling, offset, time, pvals)
+            call gauges_write(gauges_file_name(trim(outdir) // '/swe', dmax, courant, drytol, idx), &
+                              npt, px, py, pvals, probe_full, ierr3)
+            if (ierr3 /= 0) then
+                print '(A)', 'swe_mi355x: cannot write the probe file'
+                stop 1
+            end if
+            forward = .not. forward
+        end if
+    end subroutine
+
+    ! "x1 y1,x2 y2,..." (Config.f90:605-734)
+    subroutine parse_probes(str)
+        character(*), intent(in) :: str
+        integer :: p0, p1, k
+        npt = 0
+        if (len_trim(str) == 0) then
+            allocate(px(0), py(0), pvals(8, 0))
+            return
+        end if
+        npt = 1
+        do k = 1, len_trim(str)
+            if (str(k:k) == ',') npt = npt + 1
+        end do
+        allocate(px(npt), py(npt), pvals(8, npt))
+        p0 = 1
+        do k = 1, npt
+            p1 = index(str(p0:), ',')
+            if (p1 == 0) then
+                p1 = len_trim(str) + 1
+            else
+                p1 = p0 + p1 - 1
+            end if
+            read(str(p0:p1 - 1), *) px(k), py(k)
+            p0 = p1 + 1
+        end do
     end subroutine
 
     subroutine write_xml(idx)

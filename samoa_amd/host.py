@@ -53,6 +53,11 @@ def load():
                                          _i8p, C.c_double, C.c_double, C.c_double, C.c_int]
         L.swe_host_write_pvtu.restype = C.c_int
         L.swe_host_write_pvtu.argtypes = [C.c_char_p, C.c_char_p]
+        L.swe_host_gauges_sample.restype = None
+        L.swe_host_gauges_sample.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_int, _dp, _dp,
+                                             C.c_double, C.c_double, C.c_double, _dp]
+        L.swe_host_gauges_write.restype = C.c_int
+        L.swe_host_gauges_write.argtypes = [C.c_char_p, C.c_int, _dp, _dp, _dp, C.c_int]
         _lib = L
     return _lib
 
@@ -145,3 +150,26 @@ def write_pvtu(path: str, piece: str) -> None:
     rc = load().swe_host_write_pvtu(os.fsencode(path), piece.encode())
     if rc != 0:
         raise OSError(f"swe_host_write_pvtu({path}) failed with I/O status {rc}")
+
+
+def gauges_sample(mesh: Mesh, state, points, *, time: float, scaling: float,
+                  offset: float) -> np.ndarray:
+    """Probe values (fortran/swe_gauges.f90) at world points [(x, y), ...]:
+    rows x_unit, y_unit, p(1), p(2), h, b, centroid distance, time in days."""
+    eta, hu, hv, b = (np.ascontiguousarray(a, dtype=np.float64) for a in state)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    out = np.zeros((len(pts), 8))
+    load().swe_host_gauges_sample(mesh.n, np.ascontiguousarray(mesh.coords.reshape(-1), np.float64),
+                                  eta, hu, hv, b, len(pts), np.ascontiguousarray(pts[:, 0]),
+                                  np.ascontiguousarray(pts[:, 1]), float(scaling), float(offset),
+                                  float(time), out)
+    return out
+
+
+def gauges_write(path: str, points, values: np.ndarray, full: bool = False) -> None:
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 2)
+    rc = load().swe_host_gauges_write(os.fsencode(path), len(pts), np.ascontiguousarray(pts[:, 0]),
+                                      np.ascontiguousarray(pts[:, 1]),
+                                      np.ascontiguousarray(values, np.float64), int(bool(full)))
+    if rc != 0:
+        raise OSError(f"swe_host_gauges_write({path}) failed with I/O status {rc}")

@@ -182,27 +182,43 @@ def write_run(name: str, binary: str, depth: int, steps: int, nout: int, bc: int
     print(f"run_{name}: {len(outs)} outputs, n={n}")
 
 
+PROBES = "0.3 0.4,0.71 0.2,-0.5 0.25,-4.9 4.95,2.5 -1.25"
+
+
 def write_vtu_fixtures() -> None:
-    """The reference's own VTK output files (SWE_xml_output.f90), unmodified:
-    dam break at depths 3 and 4, 4 steps, an output every 2 steps (so both
-    traversal directions occur).  Stored as vtu/d<d>_<i>.vtu plus one .pvtu."""
+    """The reference's own VTK and probe output files (SWE_xml_output.f90,
+    SWE_point_output.f90), unmodified: dam break at depths 3 and 4, 4 steps,
+    an output every 2 steps.  Run A writes VTK files only, so the output
+    traversals alternate in direction (vtu/d<d>_<i>.vtu, one .pvtu); run B
+    adds the probes PROBES, whose traversal flips the direction once more per
+    output (vtu/p<d>_<i>.vtu and vtu/p<d>_<i>.points.txt)."""
     vdir = os.path.join(HERE, "vtu")
     os.makedirs(vdir, exist_ok=True)
     for d in (3, 4):
-        od = tempfile.mkdtemp(prefix="swe_vtu_")
-        try:
-            subprocess.run([os.path.join(REF, "samoa_swe"), "-threads", "1", "-sections", "1",
-                            "-dstart", str(d), "-dmin", str(d), "-dmax", str(d), "-nadapt", "-1",
-                            "-tmax", "-1", "-nmax", "4", "-xmloutput", ".true.", "-nout", "2",
-                            "-output_dir", od], check=True, capture_output=True)
-            for f in glob.glob(os.path.join(od, "*_r0_s1.vtu")):
-                i = int(f.split("_")[-3])
-                shutil.copy(f, os.path.join(vdir, f"d{d}_{i}.vtu"))
-            if d == 3:
-                p0 = sorted(glob.glob(os.path.join(od, "*_0.pvtu")))[0]
-                shutil.copy(p0, os.path.join(vdir, "d3_0.pvtu"))
-        finally:
-            shutil.rmtree(od, ignore_errors=True)
+        for tag, extra in (("d", []), ("p", ["-stestpoints", PROBES])):
+            od = tempfile.mkdtemp(prefix="swe_vtu_")
+            try:
+                subprocess.run([os.path.join(REF, "samoa_swe"), "-threads", "1", "-sections", "1",
+                                "-dstart", str(d), "-dmin", str(d), "-dmax", str(d), "-nadapt",
+                                "-1", "-tmax", "-1", "-nmax", "4", "-xmloutput", ".true.", "-nout",
+                                "2", *extra, "-output_dir", od], check=True, capture_output=True)
+                for f in glob.glob(os.path.join(od, "*_r0_s1.vtu")):
+                    i = int(f.split("_")[-3])
+                    shutil.copy(f, os.path.join(vdir, f"{tag}{d}_{i}.vtu"))
+                for f in glob.glob(os.path.join(od, "*_pointoutput_*.txt")):
+                    i = int(f.rsplit("_", 1)[1].split(".")[0])
+                    shutil.copy(f, os.path.join(vdir, f"{tag}{d}_{i}.points.txt"))
+                if d == 3 and tag == "d":
+                    p0 = sorted(glob.glob(os.path.join(od, "*_0.pvtu")))[0]
+                    shutil.copy(p0, os.path.join(vdir, "d3_0.pvtu"))
+                if d == 3 and tag == "p":
+                    name = os.path.basename(glob.glob(os.path.join(od, "*_pointoutput_0.txt"))[0])
+                    with open(os.path.join(vdir, "points_name.txt"), "w") as fh:
+                        fh.write(name[name.index("_d"):] + "\n")
+            finally:
+                shutil.rmtree(od, ignore_errors=True)
+    for f in os.listdir(vdir):
+        os.chmod(os.path.join(vdir, f), 0o644)
     print("vtu fixtures:", sorted(os.listdir(vdir)))
 
 

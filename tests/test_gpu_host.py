@@ -59,6 +59,33 @@ def test_cli_vtk_output_identical_to_reference_files(d):
             assert os.path.exists(os.path.join(td, f"swe_{i}.pvtu"))
 
 
+@pytest.mark.parametrize("d", [3, 4])
+def test_cli_probe_output_matches_reference_files(d):
+    """With probes the reference runs one more output traversal per output,
+    which flips the direction once more: the VTK files and the probe files
+    of the CLI match the reference run with -stestpoints (the centroid
+    distance to its last bit, the rest exactly)."""
+    vdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "vtu")
+    suffix = open(os.path.join(vdir, "points_name.txt")).read().strip()
+    probes = "0.3 0.4,0.71 0.2,-0.5 0.25,-4.9 4.95,2.5 -1.25"
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [host.CLI, "-scenario", "dambreak", "-dmax", str(d), "-nmax", "4", "-nout", "2",
+               "-tmax", "-1", "-xmloutput", ".true.", "-stestpoints", probes, "-output_dir", td]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for i in range(3):
+            mine = open(os.path.join(td, f"swe_{i}_r0_s1.vtu"), "rb").read()
+            assert mine == open(os.path.join(vdir, f"p{d}_{i}.vtu"), "rb").read(), f"output {i}"
+            name = "swe" + suffix.replace("_d3_", f"_d{d}_").replace("_pointoutput_0", f"_pointoutput_{i}")
+            got = open(os.path.join(td, name)).read().splitlines()
+            ref = open(os.path.join(vdir, f"p{d}_{i}.points.txt")).read().splitlines()
+            assert got[0] == ref[0] and len(got) == len(ref)
+            for a, b in zip(got[1:], ref[1:]):
+                fa, fb = a.split(","), b.split(",")
+                assert fa[:7] == fb[:7] and fa[8] == fb[8], (a, b)
+                assert abs(float(fa[7]) - float(fb[7])) <= 2e-15, (a, b)
+
+
 def test_fortran_time_loop_vs_oracle():
     d, S = 12, 1.0e6
     mesh = host.mesh_uniform(d)
