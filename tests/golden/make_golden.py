@@ -8,6 +8,7 @@ test time.
 
     python tests/golden/make_golden.py        (all fixtures)
     python tests/golden/make_golden.py vtu    (the VTK output files only)
+    python tests/golden/make_golden.py adapt  (the adaptive-mesh runs only)
 
 Fixtures:
   kat_geoclaw.npz      inputs (n,10) + reference outputs of c_bind_geoclaw_solver
@@ -18,6 +19,8 @@ Fixtures:
   run_*.npz            full SWE runs of the reference binary: mesh point order,
                        per-output state (forward SFC order), time, refinement flags
   vtu/                 the reference's own VTK output files of two small runs
+  adapt_*.npz          runs on the reference's adaptively refined (then frozen)
+                       meshes: cell corners, depths, states, times, flags
 """
 from __future__ import annotations
 
@@ -222,11 +225,52 @@ def write_vtu_fixtures() -> None:
     print("vtu fixtures:", sorted(os.listdir(vdir)))
 
 
+def write_adaptive_run(name: str, binary: str, dstart: int, dmin: int, dmax: int, steps: int,
+                       bc: int) -> None:
+    """A run on an adaptive mesh: the reference's initial refinement from
+    depth dstart up to dmax (SWE_initialize.f90), then frozen (-nadapt -1),
+    an output after every step.  The cells are stored in forward curve order
+    (the reference writes them in the order of its output traversal, which
+    runs backward here); the mesh is rebuilt from the corners by the test."""
+    od = tempfile.mkdtemp(prefix="swe_golden_")
+    try:
+        cmd = [os.path.join(REF, binary), "-threads", "1", "-sections", "1", "-dstart", str(dstart),
+               "-dmin", str(dmin), "-dmax", str(dmax), "-nadapt", "-1", "-tmax", "-1", "-nmax",
+               str(steps), "-xmloutput", ".true.", "-nout", "1", "-output_dir", od]
+        subprocess.run(cmd, check=True, capture_output=True)
+        files = sorted(glob.glob(os.path.join(od, "*_r0_s1.vtu")),
+                       key=lambda f: int(f.split("_")[-3]))
+        outs = [po.read_vtu(f) for f in files]
+    finally:
+        shutil.rmtree(od, ignore_errors=True)
+    n = len(outs[0]["depth"])
+    pts = outs[0]["Point"].reshape(n, 3, 3)[:, :, :2]
+    for v in outs[1:]:  # frozen mesh, same output order
+        assert np.array_equal(v["Point"].reshape(n, 3, 3)[:, :, :2], pts)
+    rev = np.arange(n)[::-1]
+    states, times, refine = [], [], []
+    for v in outs:
+        m = v["momentum"].reshape(n, 3)
+        states.append(np.stack([v["water height"], m[:, 0], m[:, 1], v["bathymetry"]])[:, rev])
+        times.append(v["time"][0])
+        refine.append(v["refinement flag"][rev])
+    meta = dict(binary=binary, dstart=dstart, dmin=dmin, dmax=dmax, steps=steps, nout=1, bc=bc,
+                scaling=10.0, offset=-5.0, courant=F32_COURANT, drytol=F32_DRYTOL,
+                cmd=" ".join(cmd[1:]).replace(od, "<dir>"))
+    np.savez_compressed(os.path.join(HERE, f"adapt_{name}.npz"), corners=pts[rev],
+                        depth=outs[0]["depth"][rev], states=np.array(states), times=np.array(times),
+                        refine=np.array(refine), meta=json.dumps(meta))
+    print(f"adapt_{name}: {len(outs)} outputs, n={n}, depths {np.bincount(outs[0]['depth'])}")
+
+
 def main() -> None:
     if not os.path.exists(os.path.join(REF, "ref_kat")):
         sys.exit("build the reference first: make -C oracle ref")
     if sys.argv[1:] == ["vtu"]:
         write_vtu_fixtures()
+        return
+    if sys.argv[1:] == ["adapt"]:
+        write_adaptive_fixtures()
         return
     kin = kat_inputs()
     aug = run_kat("geoclaw", kin, "3", nout=7)
@@ -255,6 +299,12 @@ def main() -> None:
     write_run("bath_d7_n2", "samoa_swe_bath", 7, 40, 2, 1)
     write_run("bath_d9_n7", "samoa_swe_bath", 9, 35, 7, 1)
     write_vtu_fixtures()
+    write_adaptive_fixtures()
+
+
+def write_adaptive_fixtures() -> None:
+    write_adaptive_run("plain_d2_9", "samoa_swe", 2, 2, 9, 12, 0)
+    write_adaptive_run("bath_d3_11", "samoa_swe_bath", 3, 3, 11, 15, 1)
 
 
 if __name__ == "__main__":
