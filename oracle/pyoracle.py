@@ -60,6 +60,10 @@ def lib():
         L.swo_step_omp.restype = C.c_double
         L.swo_initial_dt.argtypes = [C.c_int64, _i8p, _dp, _dp, _dp, _dp, C.c_double, C.c_double]
         L.swo_initial_dt.restype = C.c_double
+        L.swo_adapt.argtypes = [C.c_int64, _i8p, _i8p, _i32p, C.c_void_p, _i8p, C.c_int, C.c_int,
+                                _dp, _dp, _dp, _dp, C.c_int64, _i8p, _i8p, C.c_void_p,
+                                C.c_void_p, _dp, _dp, _dp, _dp, C.POINTER(C.c_int)]
+        L.swo_adapt.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -129,6 +133,36 @@ def step(depth, plotter, nbr, state, dt, *, scaling=1.0, drytol=0.01, dmin=None,
               dmin, dmax, bc, solver, int(bool(forward)), dt, eta, hu, hv, b, eo, ho, vo, rf,
               C.byref(nref))
     return eo, ho, vo, rf, cand, nref.value
+
+
+def adapt(depth, plotter, nbr, flags, state, *, dmin, dmax, coords=None):
+    """One remeshing step (swe_oracle_adapt.c).  Returns (depth', plotter',
+    state'[4, n'], nbr'[n', 3] or None, coords'[n', 3, 2] or None, sweeps);
+    the neighbour table and corners only when coords are given."""
+    depth = np.ascontiguousarray(depth, dtype=np.int8)
+    n = depth.shape[0]
+    cap = 4 * n
+    st = [np.ascontiguousarray(a, dtype=np.float64) for a in state]
+    d_o, p_o = np.zeros(cap, np.int8), np.zeros(cap, np.int8)
+    s_o = [np.zeros(cap) for _ in range(4)]
+    nb_o = c_o = None
+    cptr = None
+    if coords is not None:
+        cin = np.ascontiguousarray(coords, dtype=np.float64).reshape(-1)
+        nb_o = np.zeros(3 * cap, np.int32)
+        c_o = np.zeros(6 * cap)
+        cptr = cin.ctypes.data
+    sweeps = C.c_int()
+    m = lib().swo_adapt(n, depth, np.ascontiguousarray(plotter, dtype=np.int8),
+                        np.ascontiguousarray(nbr, dtype=np.int32).reshape(-1), cptr,
+                        np.ascontiguousarray(flags, dtype=np.int8), dmin, dmax, *st, cap, d_o, p_o,
+                        None if nb_o is None else nb_o.ctypes.data,
+                        None if c_o is None else c_o.ctypes.data, *s_o, C.byref(sweeps))
+    if m < 0:
+        raise RuntimeError(f"swo_adapt failed: {m}")
+    return (d_o[:m].copy(), p_o[:m].copy(), np.stack([a[:m] for a in s_o]),
+            None if nb_o is None else nb_o[:3 * m].reshape(m, 3).copy(),
+            None if c_o is None else c_o[:6 * m].reshape(m, 3, 2).copy(), sweeps.value)
 
 
 def initial_dt(depth, state, courant=0.45, g=G):

@@ -17,6 +17,10 @@
 #   SWE_initialize.f90:185   bathymetry = 0  ->  b = -0.5 + 0.25 x + 0.1 cos(2 y)
 #                            (a beach rising towards +x, dry for x > ~2)
 #   SWE_euler_timestep.f90:248 OUTFLOW copy -> mirror ghost p - 2 (p.n) n
+# VARIANT=dam raises the dam break of SWE_initialize.f90:354 from (1 m | dry)
+# to (10 m | 5 m), a wet shock strong enough to trigger refinement while the
+# mesh adapts every step (-nadapt 1):
+#   SWE_initialize.f90:354   hL = 1, hR = 0  ->  hL = 10, hR = 5
 # Only the linked binary is written to $OUT (oracle/_ref/).
 set -euo pipefail
 : "${REF:?}" "${REFTMP:?}" "${OUT:?}"
@@ -30,6 +34,9 @@ if [ "$VARIANT" = "bath" ]; then
     sed -i '248s/^\(\s*\)bnd_rep = rep%Q(1)$/\1bnd_rep = t_state(rep%Q(1)%h, rep%Q(1)%p - 2.0_GRID_SR * dot_product(rep%Q(1)%p, edge%transform_data%normal) * edge%transform_data%normal, rep%Q(1)%b)/' "$REFTMP/src/SWE/SWE_euler_timestep.f90"
     grep -q "0.1_SR \* cos(2.0_SR" "$REFTMP/src/SWE/SWE_initialize.f90"
     grep -q "2.0_GRID_SR \* dot_product" "$REFTMP/src/SWE/SWE_euler_timestep.f90"
+elif [ "$VARIANT" = "dam" ]; then
+    sed -i '354s/hL = 1.0_SR, hR = 0.0_SR/hL = 10.0_SR, hR = 5.0_SR/' "$REFTMP/src/SWE/SWE_initialize.f90"
+    grep -q "hL = 10.0_SR, hR = 5.0_SR" "$REFTMP/src/SWE/SWE_initialize.f90"
 elif [ "$VARIANT" != "plain" ]; then
     echo "build_samoa.sh: unknown VARIANT $VARIANT" >&2
     exit 1

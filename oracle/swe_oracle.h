@@ -83,6 +83,22 @@ double swo_step_omp(int64_t n, const int8_t *depth, const int8_t *plotter,
                     const double *b, double *eta_out, double *hu_out,
                     double *hv_out, int8_t *refine, int64_t *n_refine);
 
+/* One remeshing step (swe_oracle_adapt.c): conformity of the refinement
+ * requests flags[n] (-1/0/1, Conformity.f90:172-208,1244-1317), split and
+ * merge in curve order (SFC_generic_adaptive_traversal.f90) and the state
+ * transfer of SWE_adapt.f90:95-169.  coords[6n] (corners in the unit square)
+ * may be NULL; with coords, coords_out and nbr_out the new corners and the new
+ * neighbour table (matched through edge midpoints) are written too.  Writes
+ * at most cap cells; returns the new cell count, or a negative code (-2 out of
+ * memory, -3 asymmetric neighbours, -4 cap too small, -5 inconsistent merge).
+ * sweeps (nullable): conformity traversals done, the reference's count. */
+int64_t swo_adapt(int64_t n, const int8_t *depth, const int8_t *plotter, const int32_t *nbr,
+                  const double *coords, const int8_t *flags, int dmin, int dmax,
+                  const double *eta, const double *hu, const double *hv, const double *b,
+                  int64_t cap, int8_t *depth_out, int8_t *plotter_out, int32_t *nbr_out,
+                  double *coords_out, double *eta_out, double *hu_out, double *hv_out,
+                  double *b_out, int *sweeps);
+
 #ifdef __cplusplus
 }
 #endif

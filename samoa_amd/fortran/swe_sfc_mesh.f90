@@ -11,7 +11,7 @@ module swe_sfc_mesh
     use iso_c_binding
     implicit none
     private
-    public :: sfc_uniform_mesh, swe_host_mesh_uniform
+    public :: sfc_uniform_mesh, swe_host_mesh_uniform, sfc_mesh_from_depths, swe_host_mesh_from_depths
 
     integer(c_int8_t), parameter :: plotter_child(-8:8) = &
         int([3, 2, 1, 8, 7, 6, 5, 4, 0, -6, -7, -8, -1, -2, -3, -4, -5], c_int8_t)
@@ -125,5 +125,81 @@ contains
         else
             swe_host_mesh_uniform = n
         end if
+    end function
+
+    !> Plotter types and corners of an adaptive mesh from its depth sequence
+    !> alone.  The leaves of the two root trees, read in curve order, fix the
+    !> tree: a node whose depth is below the next leaf's depth was split.
+    !> ierr = 1 if the sequence is not the leaf sequence of the two roots
+    !> (a leaf shallower than its place in the tree, too few or too many cells).
+    subroutine sfc_mesh_from_depths(n, depth, plotter, coords, ierr)
+        integer(c_int64_t), intent(in)  :: n
+        integer(c_int8_t), intent(in)   :: depth(n)
+        integer(c_int8_t), intent(out)  :: plotter(n)
+        real(c_double), intent(out)     :: coords(2, 3, n)
+        integer, intent(out)            :: ierr
+        real(c_double), parameter :: root1(2, 3) = reshape([0d0, 0d0, 1d0, 0d0, 1d0, 1d0], [2, 3])
+        real(c_double), parameter :: root2(2, 3) = reshape([1d0, 1d0, 0d0, 1d0, 0d0, 0d0], [2, 3])
+        ! a depth-first stack holds at most one pending sibling per level
+        integer, parameter :: MAXD = 62
+        real(c_double) :: sx(2, 3, 2 * MAXD + 4), x(2, 3), m(2)
+        integer :: sp(2 * MAXD + 4), sd(2 * MAXD + 4), top, p, d
+        integer(c_int64_t) :: next
+
+        ierr = 0
+        top = 2
+        sx(:, :, 1) = root2
+        sp(1) = -6
+        sd(1) = 0
+        sx(:, :, 2) = root1
+        sp(2) = -2
+        sd(2) = 0
+        next = 0
+        do while (top > 0)
+            x = sx(:, :, top)
+            p = sp(top)
+            d = sd(top)
+            top = top - 1
+            if (next >= n) then
+                ierr = 1
+                return
+            end if
+            if (depth(next + 1) == d) then
+                next = next + 1
+                plotter(next) = int(p, c_int8_t)
+                coords(:, :, next) = x
+            else if (depth(next + 1) > d .and. d < MAXD) then
+                m = 0.5_c_double * (x(:, 1) + x(:, 3))
+                top = top + 1
+                sx(:, 1, top) = x(:, 2); sx(:, 2, top) = m; sx(:, 3, top) = x(:, 3)
+                sp(top) = -int(plotter_child(-p))
+                sd(top) = d + 1
+                top = top + 1
+                sx(:, 1, top) = x(:, 1); sx(:, 2, top) = m; sx(:, 3, top) = x(:, 2)
+                sp(top) = int(plotter_child(p))
+                sd(top) = d + 1
+            else
+                ierr = 1
+                return
+            end if
+        end do
+        if (next /= n) ierr = 1
+    end subroutine
+
+    !> C entry of sfc_mesh_from_depths: plotter [n], coords [6n].  Returns 0,
+    !> or 1 if depth[] is not a leaf sequence of the two root triangles.
+    integer(c_int) function swe_host_mesh_from_depths(n, depth, plotter, coords) &
+            bind(c, name='swe_host_mesh_from_depths')
+        integer(c_int64_t), value       :: n
+        integer(c_int8_t), intent(in)   :: depth(*)
+        integer(c_int8_t), intent(out)  :: plotter(*)
+        real(c_double), intent(out)     :: coords(2, 3, *)
+        integer :: ierr
+        if (n <= 0) then
+            swe_host_mesh_from_depths = 1
+            return
+        end if
+        call sfc_mesh_from_depths(n, depth(1:n), plotter(1:n), coords(:, :, 1:n), ierr)
+        swe_host_mesh_from_depths = ierr
     end function
 end module swe_sfc_mesh

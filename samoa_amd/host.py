@@ -58,6 +58,8 @@ def load():
                                              C.c_double, C.c_double, C.c_double, _dp]
         L.swe_host_gauges_write.restype = C.c_int
         L.swe_host_gauges_write.argtypes = [C.c_char_p, C.c_int, _dp, _dp, _dp, C.c_int]
+        L.swe_host_mesh_from_depths.restype = C.c_int
+        L.swe_host_mesh_from_depths.argtypes = [C.c_int64, _i8p, _i8p, _dp]
         _lib = L
     return _lib
 
@@ -71,6 +73,39 @@ def mesh_uniform(depth: int) -> Mesh:
     if got != n:
         raise RuntimeError(f"swe_host_mesh_uniform({depth}) failed: {got}")
     return Mesh(np.full(n, depth, np.int8), plotter, nbr.reshape(n, 3), coords.reshape(n, 3, 2))
+
+
+def neighbours_from_corners(coords: np.ndarray) -> np.ndarray:
+    """nbr[n, 3] of a conforming mesh from its corners[n, 3, 2]: two cells are
+    neighbours across edge k (opposite corner k) when the edges share their
+    midpoint, a dyadic point (exact in the integer key below for depth <= 60)."""
+    c = np.asarray(coords, np.float64)
+    n = c.shape[0]
+    a, b = c[:, [1, 2, 0]], c[:, [2, 0, 1]]          # the ends of edges 0, 1, 2
+    mid = np.rint(0.5 * (a + b) * 2.0 ** 33).astype(np.int64).reshape(3 * n, 2)
+    order = np.lexsort((mid[:, 1], mid[:, 0]))
+    sm = mid[order]
+    same = np.all(sm[1:] == sm[:-1], axis=1)
+    i = np.nonzero(same)[0]
+    nbr = np.full(3 * n, -1, np.int32)
+    s0, s1 = order[i], order[i + 1]
+    nbr[s0] = s1 // 3
+    nbr[s1] = s0 // 3
+    return nbr.reshape(n, 3)
+
+
+def mesh_from_depths(depth) -> Mesh:
+    """The mesh whose cells, in forward curve order, have these depths: the
+    leaves of the two root triangles (fortran/swe_sfc_mesh.f90
+    sfc_mesh_from_depths), with plotter types, corners and neighbours."""
+    depth = np.ascontiguousarray(depth, np.int8)
+    n = depth.shape[0]
+    plotter = np.empty(n, np.int8)
+    coords = np.empty(6 * n)
+    if load().swe_host_mesh_from_depths(n, depth, plotter, coords) != 0:
+        raise ValueError("depth sequence is not the leaf sequence of the two root triangles")
+    coords = coords.reshape(n, 3, 2)
+    return Mesh(depth.copy(), plotter, neighbours_from_corners(coords), coords)
 
 
 def _state(mesh, fn):

@@ -9,6 +9,7 @@ test time.
     python tests/golden/make_golden.py        (all fixtures)
     python tests/golden/make_golden.py vtu    (the VTK output files only)
     python tests/golden/make_golden.py adapt  (the adaptive-mesh runs only)
+    python tests/golden/make_golden.py remesh (the runs that remesh every step)
 
 Fixtures:
   kat_geoclaw.npz      inputs (n,10) + reference outputs of c_bind_geoclaw_solver
@@ -21,6 +22,10 @@ Fixtures:
   vtu/                 the reference's own VTK output files of two small runs
   adapt_*.npz          runs on the reference's adaptively refined (then frozen)
                        meshes: cell corners, depths, states, times, flags
+  remesh_*.npz         runs that remesh before every step (-nadapt 1) with the
+                       dam-break variant of the reference (10 m | 5 m): per
+                       output the depth sequence of the mesh (which fixes it),
+                       state, flags, time and output direction
 """
 from __future__ import annotations
 
@@ -38,6 +43,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF = os.path.join(ROOT, "oracle", "_ref")
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, ROOT)
 import pyoracle as po  # noqa: E402
 
 G = 9.80665
@@ -272,6 +278,9 @@ def main() -> None:
     if sys.argv[1:] == ["adapt"]:
         write_adaptive_fixtures()
         return
+    if sys.argv[1:] == ["remesh"]:
+        write_remesh_fixtures()
+        return
     kin = kat_inputs()
     aug = run_kat("geoclaw", kin, "3", nout=7)
     fwave = run_kat("geoclaw", kin, "1", nout=7)
@@ -300,6 +309,60 @@ def main() -> None:
     write_run("bath_d9_n7", "samoa_swe_bath", 9, 35, 7, 1)
     write_vtu_fixtures()
     write_adaptive_fixtures()
+
+
+def write_remesh_run(name: str, binary: str, dstart: int, dmin: int, dmax: int, steps: int,
+                     bc: int) -> None:
+    """A run that remeshes before every time step (SWE.f90:387-390 with
+    -nadapt 1), an output after every step.  The output traversal alternates
+    its direction; each output is stored in forward curve order (a forward
+    output starts in the root triangle below the diagonal).  The mesh of an
+    output is stored as its depth sequence; the corners the reference wrote
+    are checked here against the mesh that sequence fixes."""
+    from samoa_amd import host
+
+    od = tempfile.mkdtemp(prefix="swe_golden_")
+    try:
+        cmd = [os.path.join(REF, binary), "-threads", "1", "-sections", "1", "-dstart", str(dstart),
+               "-dmin", str(dmin), "-dmax", str(dmax), "-nadapt", "1", "-tmax", "-1", "-nmax",
+               str(steps), "-xmloutput", ".true.", "-nout", "1", "-output_dir", od]
+        subprocess.run(cmd, check=True, capture_output=True)
+        files = sorted(glob.glob(os.path.join(od, "*_r0_s1.vtu")),
+                       key=lambda f: int(f.split("_")[-3]))
+        outs = [po.read_vtu(f) for f in files]
+    finally:
+        shutil.rmtree(od, ignore_errors=True)
+    depth, states, refine, times, fwd, offsets = [], [], [], [], [], [0]
+    for v in outs:
+        n = len(v["depth"])
+        pts = v["Point"].reshape(n, 3, 3)[:, :, :2]
+        cen = pts.mean(axis=1)
+        forward = bool(cen[0, 1] < cen[0, 0])
+        idx = np.arange(n) if forward else np.arange(n)[::-1]
+        mesh = host.mesh_from_depths(v["depth"][idx])
+        ours = (10.0 * mesh.coords - 5.0)
+        ref = np.where((mesh.plotter > 0)[:, None, None], pts[idx], pts[idx][:, ::-1])
+        assert np.array_equal(ours, ref), "corners differ from the depth sequence's mesh"
+        m = v["momentum"].reshape(n, 3)
+        depth.append(v["depth"][idx])
+        states.append(np.stack([v["water height"], m[:, 0], m[:, 1], v["bathymetry"]])[:, idx])
+        refine.append(v["refinement flag"][idx])
+        times.append(v["time"][0])
+        fwd.append(forward)
+        offsets.append(offsets[-1] + n)
+    meta = dict(binary=binary, dstart=dstart, dmin=dmin, dmax=dmax, steps=steps, nout=1, bc=bc,
+                scaling=10.0, offset=-5.0, courant=F32_COURANT, drytol=F32_DRYTOL,
+                cmd=" ".join(cmd[1:]).replace(od, "<dir>"))
+    np.savez_compressed(os.path.join(HERE, f"remesh_{name}.npz"), depth=np.concatenate(depth),
+                        offsets=np.array(offsets), states=np.concatenate(states, axis=1),
+                        refine=np.concatenate(refine), times=np.array(times),
+                        forward=np.array(fwd), meta=json.dumps(meta))
+    print(f"remesh_{name}: {len(outs)} outputs, cells {[offsets[i + 1] - offsets[i] for i in range(len(outs))]}")
+
+
+def write_remesh_fixtures() -> None:
+    write_remesh_run("dam_d3_12", "samoa_swe_dam", 3, 3, 12, 30, 0)
+    write_remesh_run("dam_d2_9", "samoa_swe_dam", 2, 2, 9, 30, 0)
 
 
 def write_adaptive_fixtures() -> None:
