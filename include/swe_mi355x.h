@@ -140,6 +140,42 @@ int swe_adjoint_step_device(swe_handle *h, const double *eta, const double *hu, 
                             double *lhu_out, double *lhv_out, double *gb, const double *dt_dev,
                             int forward, void *stream);
 
+/* ---- remeshing --------------------------------------------------------- */
+
+/* Replaces one call of swe%adaption%traverse(grid) (SWE.f90:387-390): the
+ * conformity check of Conformity/Conformity.f90:172-208 (mark_edges and
+ * mark_edges_integrity, :1244-1317) on the refinement requests, then the
+ * adaptive traversal of SFC_generic_adaptive_traversal.f90 with the state
+ * transfer of SWE_adapt.f90:95-169.  A cell whose hypotenuse is marked
+ * splits (a marked leg splits that child again); a pair of sibling cells
+ * whose legs are marked merges; the others stay.  A split copies eta, hu, hv
+ * and b to the children; a merge averages all four, 0.5 (Q1 + Q2).  (The
+ * reference evaluates the scenario's bathymetry for a split cell; for a flat
+ * or cell-wise constant bed that is the parent's value.)
+ *
+ * refine: the requests (-1 coarsen, 0 keep, 1 refine) per cell in curve
+ * order, as swe_step writes them; NULL takes the flags the handle's last
+ * step left on the device (swe_keep_refinement_flags must be on).  Requests
+ * beyond [min_depth, max_depth] are dropped, as volume_op never issues them.
+ * The mesh must be whole (the leaf sequence of the two root triangles).
+ * Writes the new cell count to *n_cells and the number of conformity sweeps
+ * to *sweeps (both nullable).  The traversal direction of the reference flips
+ * once per adaption: the caller flips `forward` once more.  Single-mesh
+ * handles only.  Device memory grows as the mesh does; it is not returned
+ * when the mesh shrinks. */
+int swe_adapt(swe_handle *h, const int8_t *refine, int64_t *n_cells, int *sweeps);
+
+/* on != 0: every step of swe_step and swe_run leaves its refinement flags on
+ * the device for swe_adapt(h, NULL, ...). */
+int swe_keep_refinement_flags(swe_handle *h, int on);
+
+/* Cells of the handle's current mesh (changes with swe_adapt); -1 for NULL. */
+int64_t swe_cells(swe_handle *h);
+
+/* The current mesh in swe_mesh_desc's layout (each argument nullable):
+ * depth[n], plotter[n], nbr[3n] (neighbour cell or -1). */
+int swe_download_mesh(swe_handle *h, int8_t *depth, int8_t *plotter, int32_t *nbr);
+
 /* Launch subsequent work of this handle on `stream` (a hipStream_t; NULL
  * restores the library's own stream), e.g. to time it with events the caller
  * records on that stream.  Waits for work already queued. */

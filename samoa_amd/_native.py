@@ -14,6 +14,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 HIP_LIB = os.path.join(LIB_DIR, "libswe_mi355x.so")
 
 SWE_OK = 0
+SWE_ERR_ARG, SWE_ERR_HIP, SWE_ERR_NOMEM, SWE_ERR_STATE = -1, -2, -3, -4
 STATUS = {-1: "SWE_ERR_ARG", -2: "SWE_ERR_HIP", -3: "SWE_ERR_NOMEM", -4: "SWE_ERR_STATE"}
 
 
@@ -55,6 +56,10 @@ SIGNATURES = {
     "swe_run": (_I, [_P, _I64, _D, _I, _D, C.POINTER(_D), C.POINTER(_D)]),
     "swe_step_device": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P]),
     "swe_set_stream": (_I, [_P, _P]),
+    "swe_adapt": (_I, [_P, _P, C.POINTER(_I64), C.POINTER(_I)]),
+    "swe_keep_refinement_flags": (_I, [_P, _I]),
+    "swe_cells": (_I64, [_P]),
+    "swe_download_mesh": (_I, [_P, _P, _P, _P]),
     "swe_destroy": (_I, [_P]),
     "swe_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
     "swe_chunk_cells": (_I, []),

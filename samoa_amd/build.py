@@ -18,7 +18,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "lib")
 
-HIP_SOURCES = ["swe_step.hip", "swe_adjoint_step.hip", "swe_api.hip"]
+HIP_SOURCES = ["swe_step.hip", "swe_adjoint_step.hip", "swe_adapt.hip", "swe_api.hip"]
 HIPCC_FLAGS = [
     "-O3", "--offload-arch=gfx950", "-fPIC", "-shared", "-std=c++17",
     # every operation rounds as written: bit-identical to the reference

@@ -1,0 +1,440 @@
+// swe_adapt.hip -- remeshing on the device: the reference's adaption step
+// (SWE.f90:387-390) for a mesh held in forward curve order.
+//
+//   1. conformity (Conformity/Conformity.f90:172-208): every cell-edge holds a
+//      2-bit word (bit 0 refine, bit 1 coarsen); an edge's value is the OR of
+//      its two refine bits and the AND of its two coarsen bits, which is what
+//      the reference's single shared edge accumulates.  The requests set the
+//      words (mark_edges, :1244-1268); sweeps of mark_edges_integrity
+//      (:1271-1317) run until no word changes.  All updates are monotone, so
+//      the fixpoint does not depend on the order of the sweeps.
+//   2. tree position: cell i covers [off_i, off_i + 2^(60 - depth_i)) of the
+//      two roots [0, 2^61); a first child starts on a multiple of its
+//      parent's size.  off is an exclusive scan of the cell sizes.
+//   3. counts of new cells (split 2..4, merge 1 for the first child of a
+//      pair and 0 for the second, else 1), an exclusive scan of them.
+//   4. emit: depth, plotter type and state of every new cell (a split copies
+//      eta, hu, hv and b, a merge averages all four, SWE_adapt.f90:95-169),
+//      the neighbours inside each old cell, and for every old cell-edge the
+//      new cell-edge that lies on each half of it.
+//   5. link: a new edge on half s of old edge k of cell A meets the new edge on
+//      the same half of the same old edge in A's neighbour B; the same end of
+//      the shared edge is half s in B when A and B run in opposite senses (the
+//      plotter types differ in sign) and half 1-s when they run in the same.
+//
+// Cell corners are (entry, right angle, exit); a cell (e, r, x) splits into
+// (e, m, r) and (r, m, x), m the midpoint of the hypotenuse, with the plotter
+// types child(p), -child(-p) of SFC_node_traversal.f90:257,270.  Edge k is
+// opposite corner k; edge 1 is the hypotenuse; half 0 of edge k is the half
+// at corner (k+1) mod 3, half 1 the half at corner (k+2) mod 3.
+#include <hip/hip_runtime.h>
+
+#include "swe_internal.hpp"
+
+namespace swe {
+namespace {
+
+constexpr int AT = 256;           // threads per workgroup of the per-cell kernels
+constexpr int SCAN_T = 256;       // scan: threads per tile
+constexpr int SCAN_ITEMS = 8;     // scan: consecutive items per thread
+constexpr int SCAN_TILE = SCAN_T * SCAN_ITEMS;
+constexpr int TOP_T = 1024;       // scan of the tile sums: one workgroup
+
+// child(p) of SFC_node_traversal.f90:257,270.  The map is an involution, so
+// the same table gives the parent type of a first child: child(child(p)) = p.
+__constant__ int8_t c_child[17] = {3, 2, 1, 8, 7, 6, 5, 4, 0, -6, -7, -8, -1, -2, -3, -4, -5};
+
+// New cells of a split: pattern 0 hypotenuse only, 1 with leg 2 (the first
+// child splits again), 2 with leg 0 (the second child splits again), 3 both.
+// Per new cell: depth increment, type rule (0: child(p), 1: -child(-p),
+// 2/3: the same rules applied to the first child's type, 4/5: to the second
+// child's) and per edge either 0x10 | (old edge << 2) | half (half 2: the
+// whole old edge) or 0x20 | (cell of the pattern << 2) | its edge.
+struct SplitCell {
+    int8_t dd, rule;
+    uint8_t e[3];
+};
+__constant__ SplitCell c_split[4][4] = {
+    {{1, 0, {0x20 | (1 << 2) | 2, 0x10 | (2 << 2) | 2, 0x10 | (1 << 2) | 1}},
+     {1, 1, {0x10 | (1 << 2) | 0, 0x10 | (0 << 2) | 2, 0x20 | (0 << 2) | 0}},
+     {0, 0, {0, 0, 0}},
+     {0, 0, {0, 0, 0}}},
+    {{2, 2, {0x20 | (1 << 2) | 2, 0x10 | (1 << 2) | 1, 0x10 | (2 << 2) | 0}},
+     {2, 3, {0x10 | (2 << 2) | 1, 0x20 | (2 << 2) | 2, 0x20 | (0 << 2) | 0}},
+     {1, 1, {0x10 | (1 << 2) | 0, 0x10 | (0 << 2) | 2, 0x20 | (1 << 2) | 1}},
+     {0, 0, {0, 0, 0}}},
+    {{1, 0, {0x20 | (1 << 2) | 1, 0x10 | (2 << 2) | 2, 0x10 | (1 << 2) | 1}},
+     {2, 4, {0x20 | (2 << 2) | 2, 0x20 | (0 << 2) | 0, 0x10 | (0 << 2) | 0}},
+     {2, 5, {0x10 | (0 << 2) | 1, 0x10 | (1 << 2) | 0, 0x20 | (1 << 2) | 0}},
+     {0, 0, {0, 0, 0}}},
+    {{2, 2, {0x20 | (1 << 2) | 2, 0x10 | (1 << 2) | 1, 0x10 | (2 << 2) | 0}},
+     {2, 3, {0x10 | (2 << 2) | 1, 0x20 | (2 << 2) | 1, 0x20 | (0 << 2) | 0}},
+     {2, 4, {0x20 | (3 << 2) | 2, 0x20 | (1 << 2) | 1, 0x10 | (0 << 2) | 0}},
+     {2, 5, {0x10 | (0 << 2) | 1, 0x10 | (1 << 2) | 0, 0x20 | (2 << 2) | 0}}},
+};
+
+// codes of the count kernel
+constexpr int8_t CODE_KEEP = 0, CODE_MERGE_FIRST = 1, CODE_MERGE_SECOND = 2, CODE_SPLIT = 4;
+
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Value (bit 0 refine, bit 1 coarsen) of edge k of cell c.
+__device__ __forceinline__ uint32_t edge_value(const uint32_t *slots, const int32_t *nbr, int64_t c,
+                                               uint32_t own, int k) {
+    const uint32_t s = (own >> (2 * k)) & 3u;
+    const int32_t q = nbr[3 * c + k];
+    if (q < 0) return s;
+    const uint32_t t = (ld_relaxed(&slots[q >> 2]) >> (2 * (q & 3))) & 3u;
+    return ((s | t) & 1u) | (s & t & 2u);
+}
+
+// mark_edges (Conformity.f90:1244-1268): the hypotenuse is never coarsened,
+// a cell that keeps its depth clears its legs' coarsen bits, a refinement
+// request marks the hypotenuse.  Requests beyond [dmin, dmax] are dropped as
+// volume_op (SWE_euler_timestep.f90:339-342) never issues them.
+__global__ __launch_bounds__(AT) void conf_init_kernel(int64_t n, const int8_t *depth,
+                                                       const int8_t *flags, int dmin, int dmax,
+                                                       uint32_t *slots, uint32_t *status) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (c == 0) {
+        status[0] = 0u;
+        status[1] = 0u;
+    }
+    if (c >= n) return;
+    int f = flags[c];
+    const int d = depth[c];
+    if (f > 0 && d >= dmax) f = 0;
+    if (f < 0 && d <= dmin) f = 0;
+    const uint32_t legs = f != 0 ? 2u : 0u;
+    const uint32_t hyp = f > 0 ? 1u : 0u;
+    slots[c] = legs | (hyp << 2) | (legs << 4);
+}
+
+// mark_edges_integrity (Conformity.f90:1271-1317), in place.
+__global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t *depth,
+                                                        const int32_t *nbr, uint32_t *slots,
+                                                        uint32_t *changed, uint32_t sweep) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    bool diff = false;
+    if (c < n) {
+        const uint32_t own = ld_relaxed(&slots[c]);
+        const uint32_t v0 = edge_value(slots, nbr, c, own, 0);
+        const uint32_t v1 = edge_value(slots, nbr, c, own, 1);
+        const uint32_t v2 = edge_value(slots, nbr, c, own, 2);
+        const uint32_t rh = (v0 | v1 | v2) & 1u;
+        const uint32_t cc = (!rh && (v0 & v2 & 2u) && depth[c] > 0) ? 2u : 0u;
+        const uint32_t nw = cc | (rh << 2) | (cc << 4);
+        if (nw != own) {
+            st_relaxed(&slots[c], nw);
+            diff = true;
+        }
+    }
+    if (__any(diff) && (threadIdx.x & 63) == 0) st_relaxed(changed, sweep);
+}
+
+__device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int64_t u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// Exclusive scan over the workgroup (NT threads); *total gets the sum.
+template <int NT>
+__device__ int64_t block_excl_scan(int64_t v, int64_t *lds, int64_t *total) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t inc = wave_incl_scan(v);
+    if (lane == 63) lds[wave] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t run = 0;
+        for (int w = 0; w < NT / 64; ++w) {
+            const int64_t t = lds[w];
+            lds[w] = run;
+            run += t;
+        }
+        lds[NT / 64] = run;
+    }
+    __syncthreads();
+    const int64_t out = lds[wave] + inc - v;
+    *total = lds[NT / 64];
+    __syncthreads();
+    return out;
+}
+
+__global__ __launch_bounds__(SCAN_T) void scan_reduce_kernel(const int64_t *in, int64_t n,
+                                                             int64_t *bsum) {
+    __shared__ int64_t lds[SCAN_T / 64 + 1];
+    const int64_t base = blockIdx.x * (int64_t)SCAN_TILE + threadIdx.x * (int64_t)SCAN_ITEMS;
+    int64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j)
+        if (base + j < n) s += in[base + j];
+    int64_t total;
+    (void)block_excl_scan<SCAN_T>(s, lds, &total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// Exclusive scan of the nb tile sums in place; bsum[nb] = the grand total.
+__global__ __launch_bounds__(TOP_T) void scan_top_kernel(int64_t *bsum, int64_t nb) {
+    __shared__ int64_t lds[TOP_T / 64 + 1];
+    int64_t carry = 0;
+    for (int64_t start = 0; start < nb; start += TOP_T) {
+        const int64_t i = start + threadIdx.x;
+        const int64_t v = i < nb ? bsum[i] : 0;
+        int64_t total;
+        const int64_t ex = block_excl_scan<TOP_T>(v, lds, &total);
+        if (i < nb) bsum[i] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+// out[i] = exclusive prefix of in (in place allowed: each thread reads its
+// items before it writes them).
+__global__ __launch_bounds__(SCAN_T) void scan_apply_kernel(const int64_t *in, int64_t *out,
+                                                            int64_t n, const int64_t *bsum) {
+    __shared__ int64_t lds[SCAN_T / 64 + 1];
+    const int64_t base = blockIdx.x * (int64_t)SCAN_TILE + threadIdx.x * (int64_t)SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    int64_t s = 0;
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        v[j] = base + j < n ? in[base + j] : 0;
+        s += v[j];
+    }
+    int64_t total;
+    int64_t run = bsum[blockIdx.x] + block_excl_scan<SCAN_T>(s, lds, &total);
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j)
+        if (base + j < n) {
+            out[base + j] = run;
+            run += v[j];
+        }
+}
+
+__global__ __launch_bounds__(AT) void tree_size_kernel(int64_t n, const int8_t *depth,
+                                                       int64_t *size) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (c < n) size[c] = (int64_t)1 << (60 - depth[c]);
+}
+
+// Decision and new-cell count of every cell.  status bit 0: a coarsening cell
+// whose sibling is not a coarsening leaf (cannot happen after conformity).
+__global__ __launch_bounds__(AT) void count_kernel(int64_t n, const int8_t *depth,
+                                                   const int32_t *nbr, const uint32_t *slots,
+                                                   const int64_t *off, int8_t *code,
+                                                   int64_t *count, uint32_t *status) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t own = slots[c];
+    const uint32_t v0 = edge_value(slots, nbr, c, own, 0);
+    const uint32_t v1 = edge_value(slots, nbr, c, own, 1);
+    const uint32_t v2 = edge_value(slots, nbr, c, own, 2);
+    const int d = depth[c];
+    int8_t k = CODE_KEEP;
+    int64_t m = 1;
+    if (v1 & 1u) {
+        const int pat = (int)(v2 & 1u) | ((int)(v0 & 1u) << 1);
+        k = (int8_t)(CODE_SPLIT + pat);
+        m = 2 + (v2 & 1u) + (v0 & 1u);
+    } else if (v0 & 2u) {
+        // d > 0 here: the sweep clears the coarsen bits of root cells
+        const int64_t parent = (int64_t)1 << (61 - d);
+        const bool first = off[c] % parent == 0;
+        const int64_t sib = first ? c + 1 : c - 1;
+        bool ok = sib >= 0 && sib < n && depth[sib] == d;
+        if (ok) {
+            const uint32_t so = slots[sib];
+            ok = (edge_value(slots, nbr, sib, so, 0) & 2u) && !(edge_value(slots, nbr, sib, so, 1) & 1u);
+        }
+        if (!ok) atomicOr(status, 1u);
+        k = first ? CODE_MERGE_FIRST : CODE_MERGE_SECOND;
+        m = first ? 1 : 0;
+    }
+    code[c] = k;
+    count[c] = m;
+}
+
+__device__ __forceinline__ int split_type(int p, int rule) {
+    const int c1 = c_child[p + 8], c2 = -c_child[-p + 8];
+    switch (rule) {
+        case 0: return c1;
+        case 1: return c2;
+        case 2: return c_child[c1 + 8];
+        case 3: return -c_child[-c1 + 8];
+        case 4: return c_child[c2 + 8];
+        default: return -c_child[-c2 + 8];
+    }
+}
+
+__device__ __forceinline__ void put_slot(int32_t *eslot, int64_t c, int k, int half, int32_t v) {
+    if (half == 2) {
+        eslot[6 * c + 2 * k] = v;
+        eslot[6 * c + 2 * k + 1] = v;
+    } else {
+        eslot[6 * c + 2 * k + half] = v;
+    }
+}
+
+__global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (c >= a.n) return;
+    const int8_t k = a.code[c];
+    const int64_t b0 = a.base[c];
+    const int d = a.depth[c];
+    const int p = a.plotter[c];
+    if (k == CODE_KEEP) {
+        a.depth_out[b0] = (int8_t)d;
+        a.plotter_out[b0] = (int8_t)p;
+        a.eta_out[b0] = a.eta[c];
+        a.hu_out[b0] = a.hu[c];
+        a.hv_out[b0] = a.hv[c];
+        a.b_out[b0] = a.b[c];
+        for (int e = 0; e < 3; ++e) put_slot(a.eslot, c, e, 2, (int32_t)((b0 << 2) | e));
+    } else if (k == CODE_MERGE_FIRST) {
+        a.depth_out[b0] = (int8_t)(d - 1);
+        a.plotter_out[b0] = c_child[p + 8];  // the parent's type
+        a.eta_out[b0] = 0.5 * (a.eta[c] + a.eta[c + 1]);
+        a.hu_out[b0] = 0.5 * (a.hu[c] + a.hu[c + 1]);
+        a.hv_out[b0] = 0.5 * (a.hv[c] + a.hv[c + 1]);
+        a.b_out[b0] = 0.5 * (a.b[c] + a.b[c + 1]);
+        // the first child's hypotenuse is the parent's leg 2, its outer leg
+        // half of the parent's hypotenuse; its leg 0 disappears
+        put_slot(a.eslot, c, 1, 2, (int32_t)((b0 << 2) | 2));
+        put_slot(a.eslot, c, 2, 2, (int32_t)((b0 << 2) | 1));
+        put_slot(a.eslot, c, 0, 2, -1);
+    } else if (k == CODE_MERGE_SECOND) {
+        const int64_t par = b0 - 1;  // emitted by the first child
+        put_slot(a.eslot, c, 1, 2, (int32_t)((par << 2) | 0));
+        put_slot(a.eslot, c, 0, 2, (int32_t)((par << 2) | 1));
+        put_slot(a.eslot, c, 2, 2, -1);
+    } else {
+        const int pat = k - CODE_SPLIT;
+        const int cells = 2 + (pat & 1) + (pat >> 1);
+        for (int j = 0; j < cells; ++j) {
+            const SplitCell sc = c_split[pat][j];
+            const int64_t idx = b0 + j;
+            a.depth_out[idx] = (int8_t)(d + sc.dd);
+            a.plotter_out[idx] = (int8_t)split_type(p, sc.rule);
+            a.eta_out[idx] = a.eta[c];
+            a.hu_out[idx] = a.hu[c];
+            a.hv_out[idx] = a.hv[c];
+            a.b_out[idx] = a.b[c];
+            for (int e = 0; e < 3; ++e) {
+                const int src = sc.e[e];
+                if (src & 0x20) {
+                    a.nbr_out[3 * idx + e] = (int32_t)(((b0 + ((src >> 2) & 3)) << 2) | (src & 3));
+                } else {
+                    put_slot(a.eslot, c, (src >> 2) & 3, src & 3, (int32_t)((idx << 2) | e));
+                }
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(AT) void link_kernel(AdaptArgs a) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (c >= a.n) return;
+    const int pc = a.plotter[c];
+    for (int k = 0; k < 3; ++k) {
+        const int32_t x0 = a.eslot[6 * c + 2 * k], x1 = a.eslot[6 * c + 2 * k + 1];
+        if (x0 < 0) continue;
+        const int32_t q = a.nbr[3 * c + k];
+        for (int s = 0; s < 2; ++s) {
+            const int32_t x = s ? x1 : x0;
+            if (s == 1 && x1 == x0) break;
+            int32_t y = -1;
+            if (q >= 0) {
+                const int64_t o = q >> 2;
+                const int ko = q & 3;
+                const int so = ((pc > 0) == (a.plotter[o] > 0)) ? 1 - s : s;
+                y = a.eslot[6 * o + 2 * ko + so];
+            }
+            a.nbr_out[3 * (int64_t)(x >> 2) + (x & 3)] = y;
+        }
+    }
+}
+
+inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+hipError_t scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *bsum, hipStream_t s) {
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb > 0) {
+        scan_reduce_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(in, n, bsum);
+        scan_top_kernel<<<1, TOP_T, 0, s>>>(bsum, nb);
+        scan_apply_kernel<<<(unsigned)nb, SCAN_T, 0, s>>>(in, out, n, bsum);
+    } else {
+        scan_top_kernel<<<1, TOP_T, 0, s>>>(bsum, 0);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+int64_t adapt_scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+
+hipError_t adapt_conformity(const AdaptArgs &a, hipStream_t s, int max_sweeps, int *sweeps) {
+    const unsigned nb = blocks_for(a.n, AT);
+    // status[1] holds the number of the last sweep that changed a word
+    conf_init_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.flags, a.dmin, a.dmax, a.slots, a.status);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    for (int it = 1; it <= max_sweeps; ++it) {
+        uint32_t last = 0;
+        conf_sweep_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.status + 1,
+                                            (uint32_t)it);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(&last, a.status + 1, sizeof last, hipMemcpyDeviceToHost, s)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        if (last != (uint32_t)it) {
+            *sweeps = it;
+            return hipSuccess;
+        }
+    }
+    *sweeps = -1;
+    return hipSuccess;
+}
+
+hipError_t adapt_count(const AdaptArgs &a, hipStream_t s, int64_t *n_new, int64_t *tree_total,
+                       uint32_t *status) {
+    const unsigned nb = blocks_for(a.n, AT);
+    const int64_t tiles = adapt_scan_tiles(a.n);
+    hipError_t e;
+    tree_size_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.off);
+    if ((e = scan_i64(a.off, a.off, a.n, a.bsum, s)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(tree_total, a.bsum + tiles, sizeof(int64_t), hipMemcpyDeviceToHost, s)) !=
+        hipSuccess)
+        return e;
+    count_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.off, a.code, a.base, a.status);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = scan_i64(a.base, a.base, a.n, a.bsum, s)) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(n_new, a.bsum + tiles, sizeof(int64_t), hipMemcpyDeviceToHost, s)) !=
+        hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(status, a.status, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) !=
+        hipSuccess)
+        return e;
+    return hipStreamSynchronize(s);
+}
+
+hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s) {
+    const unsigned nb = blocks_for(a.n, AT);
+    emit_kernel<<<nb, AT, 0, s>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    link_kernel<<<nb, AT, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace swe

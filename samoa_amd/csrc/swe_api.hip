@@ -125,7 +125,7 @@ int ensure_tables(int device) {
 // buffer it started from, so the graph stays valid across run calls.
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
-    int cur = -1, forward = -1, chunk = 0;
+    int cur = -1, forward = -1, chunk = 0, keep = -1;
     int64_t grid = -1;
 };
 
@@ -211,6 +211,21 @@ struct Batch {
     int cur = 0;
     std::vector<char> uploaded;
     GraphCache graph;
+    // remeshing (single mesh): the state and topology buffers hold `cap`
+    // cells; a spare topology and bathymetry set receives a new mesh that
+    // fits, the spare state buffer eta/hu/hv[1 - cur] its state
+    int64_t cap = 0;
+    double *b2 = nullptr;
+    int32_t *nbr2 = nullptr;
+    int8_t *plotter2 = nullptr, *depth2 = nullptr;
+    int64_t cap2 = 0;
+    uint32_t *ad_slots = nullptr, *ad_status = nullptr;
+    int64_t *ad_off = nullptr, *ad_base = nullptr, *ad_bsum = nullptr;
+    int8_t *ad_code = nullptr;
+    int32_t *ad_eslot = nullptr;
+    int64_t ad_cap = 0;
+    int keep_flags = 0;       // every step writes its refinement flags to `refine`
+    bool flags_valid = false; // `refine` holds the flags of the current state's last step
 };
 
 struct swe_handle {
@@ -241,6 +256,17 @@ void release(Batch &B) {
     (void)hipFree(B.d_dt);
     (void)hipFree(B.d_part_min);
     (void)hipFree(B.d_part_nref);
+    (void)hipFree(B.b2);
+    (void)hipFree(B.nbr2);
+    (void)hipFree(B.plotter2);
+    (void)hipFree(B.depth2);
+    (void)hipFree(B.ad_slots);
+    (void)hipFree(B.ad_status);
+    (void)hipFree(B.ad_off);
+    (void)hipFree(B.ad_base);
+    (void)hipFree(B.ad_bsum);
+    (void)hipFree(B.ad_code);
+    (void)hipFree(B.ad_eslot);
     if (B.own_stream) (void)hipStreamDestroy(B.own_stream);
 }
 
@@ -387,6 +413,7 @@ int batch_init(Batch &B, int32_t nscen, const swe_mesh_desc *descs) {
     HIP_TRY(hipMemcpy(B.d_slots, init.data(), sizeof(swe::Slots) * init.size(),
                       hipMemcpyHostToDevice));
     B.uploaded.assign((size_t)nscen, 0);
+    B.cap = B.ncells;
     return rebuild_chunks(B);
 }
 
@@ -428,6 +455,7 @@ int batch_upload(Batch &B, int32_t s, const double *eta, const double *hu, const
     HIP_TRY(hipMemcpyAsync(B.b + o, b, bytes, hipMemcpyHostToDevice, B.stream));
     HIP_TRY(hipStreamSynchronize(B.stream));
     B.uploaded[s] = 1;
+    B.flags_valid = false;
     return SWE_OK;
 }
 
@@ -453,7 +481,7 @@ int batch_step(Batch &B, const double *dt, int forward, double *dt_cfl, int64_t 
                            B.stream));
     swe::StepArgs a = make_args(B, B.cur, 1 - B.cur, forward);
     a.dt_host = B.d_dt;
-    a.refine = refine_host ? B.refine : nullptr;
+    a.refine = (refine_host || B.keep_flags) ? B.refine : nullptr;
     HIP_TRY(swe::launch_step(a, B.solver, B.chunk, B.grid, B.stream));
     std::vector<swe::Slots> res((size_t)B.nscen);
     HIP_TRY(hipMemcpyAsync(res.data(), B.d_slots, sizeof(swe::Slots) * res.size(),
@@ -467,6 +495,7 @@ int batch_step(Batch &B, const double *dt, int forward, double *dt_cfl, int64_t 
         if (n_refine) n_refine[s] = (int64_t)res[s].res_nref;
     }
     B.cur = 1 - B.cur;
+    B.flags_valid = a.refine != nullptr;
     return SWE_OK;
 }
 
@@ -497,6 +526,7 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
     auto one_step = [&](hipStream_t st, int from, int fwd) -> hipError_t {
         swe::StepArgs a = make_args(B, from, 1 - from, fwd);
         a.device_dt = 1;
+        a.refine = B.keep_flags ? B.refine : nullptr;
         return swe::launch_step(a, B.solver, B.chunk, B.grid, st);
     };
     int fwd = forward ? 1 : 0;
@@ -504,7 +534,7 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
     if (pairs > 0) {
         GraphCache &gc = B.graph;
         if (!gc.exec || gc.cur != B.cur || gc.forward != fwd || gc.chunk != B.chunk ||
-            gc.grid != B.grid) {
+            gc.grid != B.grid || gc.keep != B.keep_flags) {
             if (gc.exec) {
                 (void)hipGraphExecDestroy(gc.exec);
                 gc.exec = nullptr;
@@ -526,6 +556,7 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
             gc.forward = fwd;
             gc.chunk = B.chunk;
             gc.grid = B.grid;
+            gc.keep = B.keep_flags;
         }
         for (int64_t i = 0; i < pairs; ++i) HIP_TRY(hipGraphLaunch(gc.exec, B.stream));
     }
@@ -541,6 +572,198 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
         time[s] = res[s].time;
         if (dt_next) dt_next[s] = res[s].dt;
     }
+    if (n_steps > 0) B.flags_valid = B.keep_flags != 0;
+    return SWE_OK;
+}
+
+template <typename T>
+int dev_alloc(T **p, int64_t count) {
+    *p = nullptr;
+    if (hipMalloc(p, sizeof(T) * (size_t)(count > 0 ? count : 1)) != hipSuccess) {
+        *p = nullptr;
+        return fail(SWE_ERR_NOMEM, "device allocation of %lld elements failed", (long long)count);
+    }
+    return SWE_OK;
+}
+
+// Scratch of the remesher for n cells.
+int ensure_adapt_scratch(Batch &B, int64_t n) {
+    if (n <= B.ad_cap) return SWE_OK;
+    (void)hipFree(B.ad_slots);
+    (void)hipFree(B.ad_status);
+    (void)hipFree(B.ad_off);
+    (void)hipFree(B.ad_base);
+    (void)hipFree(B.ad_bsum);
+    (void)hipFree(B.ad_code);
+    (void)hipFree(B.ad_eslot);
+    B.ad_slots = B.ad_status = nullptr;
+    B.ad_off = B.ad_base = B.ad_bsum = nullptr;
+    B.ad_code = nullptr;
+    B.ad_eslot = nullptr;
+    B.ad_cap = 0;
+    SWE_TRY(dev_alloc(&B.ad_slots, n));
+    SWE_TRY(dev_alloc(&B.ad_status, 2));
+    SWE_TRY(dev_alloc(&B.ad_off, n));
+    SWE_TRY(dev_alloc(&B.ad_base, n));
+    SWE_TRY(dev_alloc(&B.ad_bsum, swe::adapt_scan_tiles(n) + 1));
+    SWE_TRY(dev_alloc(&B.ad_code, n));
+    SWE_TRY(dev_alloc(&B.ad_eslot, 6 * n));
+    B.ad_cap = n;
+    return SWE_OK;
+}
+
+// One remeshing step of a single-mesh handle: conformity of the refinement
+// requests (the last step's flags on the device, or the caller's), split and
+// merge in curve order, state transfer; the handle then steps the new mesh.
+int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps_out) {
+    if (B.nscen != 1) return fail(SWE_ERR_ARG, "adapt: only a single-mesh handle remeshes");
+    if (!all_uploaded(B)) return fail(SWE_ERR_STATE, "adapt: no state uploaded");
+    const int64_t n = B.ncells;
+    if (refine_host) {
+        HIP_TRY(hipMemcpyAsync(B.refine, refine_host, (size_t)n, hipMemcpyHostToDevice, B.stream));
+    } else if (!B.flags_valid) {
+        return fail(SWE_ERR_STATE,
+                    "adapt: no refinement flags on the device (pass them, or keep the steps' "
+                    "flags with swe_keep_refinement_flags)");
+    }
+    SWE_TRY(ensure_adapt_scratch(B, n));
+    const swe::Scen &sc = B.scen[0];
+    swe::AdaptArgs a{};
+    a.n = n;
+    a.depth = B.depth;
+    a.plotter = B.plotter;
+    a.nbr = B.nbr;
+    a.flags = B.refine;
+    a.eta = B.eta[B.cur];
+    a.hu = B.hu[B.cur];
+    a.hv = B.hv[B.cur];
+    a.b = B.b;
+    a.dmin = sc.dmin;
+    a.dmax = sc.dmax;
+    a.slots = B.ad_slots;
+    a.off = B.ad_off;
+    a.base = B.ad_base;
+    a.bsum = B.ad_bsum;
+    a.code = B.ad_code;
+    a.eslot = B.ad_eslot;
+    a.status = B.ad_status;
+    int sweeps = 0;
+    HIP_TRY(swe::adapt_conformity(a, B.stream, 1 << 16, &sweeps));
+    if (sweeps < 0) return fail(SWE_ERR_STATE, "adapt: the conformity sweeps did not converge");
+    int64_t n_new = 0, tree = 0;
+    uint32_t status = 0;
+    HIP_TRY(swe::adapt_count(a, B.stream, &n_new, &tree, &status));
+    if (tree != (int64_t(1) << 61))
+        return fail(SWE_ERR_ARG,
+                    "adapt: the mesh is not a leaf sequence of the two root triangles (its depths "
+                    "cover %.17g of the unit square)", std::ldexp((double)tree, -61));
+    if (status) return fail(SWE_ERR_STATE, "adapt: inconsistent coarsening (status %u)", status);
+    if (n_new <= 0 || n_new > (int64_t(1) << 29))
+        return fail(SWE_ERR_ARG, "adapt: the new mesh would have %lld cells (1..2^29)",
+                    (long long)n_new);
+
+    // outputs: the spare set when the new mesh fits, else fresh buffers with
+    // room to grow (everything is allocated before anything is freed)
+    const bool fresh = n_new > B.cap;
+    const int64_t cap = fresh ? std::min<int64_t>(n_new + n_new / 4, int64_t(1) << 29) : B.cap;
+    double *eo = nullptr, *ho = nullptr, *vo = nullptr, *bo = nullptr;
+    double *e1 = nullptr, *h1 = nullptr, *v1 = nullptr;
+    int32_t *no = nullptr;
+    int8_t *po = nullptr, *dpo = nullptr, *ro = nullptr;
+    auto drop = [&]() {
+        for (void *p : {(void *)eo, (void *)ho, (void *)vo, (void *)bo, (void *)e1, (void *)h1,
+                        (void *)v1, (void *)no, (void *)po, (void *)dpo, (void *)ro})
+            (void)hipFree(p);
+    };
+    if (fresh) {
+        int rc = SWE_OK;
+        if (rc == SWE_OK) rc = dev_alloc(&eo, cap);
+        if (rc == SWE_OK) rc = dev_alloc(&ho, cap);
+        if (rc == SWE_OK) rc = dev_alloc(&vo, cap);
+        if (rc == SWE_OK) rc = dev_alloc(&bo, cap);
+        if (rc == SWE_OK) rc = dev_alloc(&e1, cap);
+        if (rc == SWE_OK) rc = dev_alloc(&h1, cap);
+        if (rc == SWE_OK) rc = dev_alloc(&v1, cap);
+        if (rc == SWE_OK) rc = dev_alloc(&no, 3 * cap);
+        if (rc == SWE_OK) rc = dev_alloc(&po, cap);
+        if (rc == SWE_OK) rc = dev_alloc(&dpo, cap);
+        if (rc == SWE_OK) rc = dev_alloc(&ro, cap);
+        if (rc != SWE_OK) {
+            drop();
+            return rc;
+        }
+    } else {
+        if (B.cap2 < B.cap) {
+            (void)hipFree(B.b2);
+            (void)hipFree(B.nbr2);
+            (void)hipFree(B.plotter2);
+            (void)hipFree(B.depth2);
+            B.b2 = nullptr;
+            B.nbr2 = nullptr;
+            B.plotter2 = B.depth2 = nullptr;
+            B.cap2 = 0;
+            SWE_TRY(dev_alloc(&B.b2, B.cap));
+            SWE_TRY(dev_alloc(&B.nbr2, 3 * B.cap));
+            SWE_TRY(dev_alloc(&B.plotter2, B.cap));
+            SWE_TRY(dev_alloc(&B.depth2, B.cap));
+            B.cap2 = B.cap;
+        }
+    }
+    a.eta_out = fresh ? eo : B.eta[1 - B.cur];
+    a.hu_out = fresh ? ho : B.hu[1 - B.cur];
+    a.hv_out = fresh ? vo : B.hv[1 - B.cur];
+    a.b_out = fresh ? bo : B.b2;
+    a.nbr_out = fresh ? no : B.nbr2;
+    a.plotter_out = fresh ? po : B.plotter2;
+    a.depth_out = fresh ? dpo : B.depth2;
+    hipError_t e = swe::adapt_emit(a, B.stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(B.stream);
+    if (e != hipSuccess) {
+        if (fresh) drop();
+        return fail(SWE_ERR_HIP, "adapt: %s", hipGetErrorString(e));
+    }
+    if (fresh) {
+        for (int i = 0; i < 2; ++i) {
+            (void)hipFree(B.eta[i]);
+            (void)hipFree(B.hu[i]);
+            (void)hipFree(B.hv[i]);
+        }
+        for (void *p : {(void *)B.b, (void *)B.nbr, (void *)B.plotter, (void *)B.depth,
+                        (void *)B.refine, (void *)B.b2, (void *)B.nbr2, (void *)B.plotter2,
+                        (void *)B.depth2})
+            (void)hipFree(p);
+        B.b2 = nullptr;
+        B.nbr2 = nullptr;
+        B.plotter2 = B.depth2 = nullptr;
+        B.cap2 = 0;
+        B.eta[0] = eo;
+        B.hu[0] = ho;
+        B.hv[0] = vo;
+        B.eta[1] = e1;
+        B.hu[1] = h1;
+        B.hv[1] = v1;
+        B.b = bo;
+        B.nbr = no;
+        B.plotter = po;
+        B.depth = dpo;
+        B.refine = ro;
+        B.cur = 0;
+        B.cap = cap;
+    } else {
+        std::swap(B.b, B.b2);
+        std::swap(B.nbr, B.nbr2);
+        std::swap(B.plotter, B.plotter2);
+        std::swap(B.depth, B.depth2);
+        B.cur = 1 - B.cur;
+    }
+    B.ncells = B.ntopo = n_new;
+    B.scen[0].n = n_new;
+    B.scen[0].depth = -1;
+    B.uniform_depth[0] = -1;
+    B.flags_valid = false;
+    SWE_TRY(rebuild_chunks(B));
+    if (n_out) *n_out = n_new;
+    if (sweeps_out) *sweeps_out = sweeps;
     return SWE_OK;
 }
 
@@ -690,6 +913,36 @@ int swe_run(swe_handle *h, int64_t n_steps, double dt0, int forward, double dt_c
             double *time, double *dt_next) {
     if (!h) return fail(SWE_ERR_ARG, "swe_run: null handle");
     return batch_run(h->B, n_steps, &dt0, forward, dt_clamp, time, dt_next);
+}
+
+int64_t swe_cells(swe_handle *h) { return h ? h->B.ncells : -1; }
+
+int swe_keep_refinement_flags(swe_handle *h, int on) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_keep_refinement_flags: null handle");
+    h->B.keep_flags = on ? 1 : 0;
+    return SWE_OK;
+}
+
+int swe_adapt(swe_handle *h, const int8_t *refine, int64_t *n_cells, int *sweeps) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_adapt: null handle");
+    return batch_adapt(h->B, refine, n_cells, sweeps);
+}
+
+int swe_download_mesh(swe_handle *h, int8_t *depth, int8_t *plotter, int32_t *nbr) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_download_mesh: null handle");
+    Batch &B = h->B;
+    if (B.nscen != 1) return fail(SWE_ERR_ARG, "swe_download_mesh: a batch has no single mesh");
+    const int64_t n = B.ncells;
+    if (depth) HIP_TRY(hipMemcpyAsync(depth, B.depth, (size_t)n, hipMemcpyDeviceToHost, B.stream));
+    if (plotter)
+        HIP_TRY(hipMemcpyAsync(plotter, B.plotter, (size_t)n, hipMemcpyDeviceToHost, B.stream));
+    if (nbr)
+        HIP_TRY(hipMemcpyAsync(nbr, B.nbr, sizeof(int32_t) * 3 * (size_t)n, hipMemcpyDeviceToHost,
+                               B.stream));
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    if (nbr)  // (o << 2) | back edge -> o
+        for (int64_t i = 0; i < 3 * n; ++i) nbr[i] = nbr[i] < 0 ? -1 : (nbr[i] >> 2);
+    return SWE_OK;
 }
 
 int swe_set_stream(swe_handle *h, void *stream) {

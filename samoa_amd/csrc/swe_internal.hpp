@@ -81,6 +81,39 @@ struct AdjArgs {
     int bc, forward;
 };
 
+// Remeshing of one mesh (swe_adapt.hip).  Topology in the step kernels'
+// format (nbr packed (o << 2) | back edge); scratch sized for n cells.
+struct AdaptArgs {
+    int64_t n;
+    const int8_t *depth, *plotter;
+    const int32_t *nbr;
+    const int8_t *flags;                   // requests -1 / 0 / 1 per cell
+    const double *eta, *hu, *hv, *b;
+    int dmin, dmax;
+    uint32_t *slots;                       // [n] conformity words
+    int64_t *off;                          // [n] tree offsets
+    int64_t *base;                         // [n] new-cell counts, then their prefix
+    int64_t *bsum;                         // [adapt_scan_tiles(n) + 1]
+    int8_t *code;                          // [n]
+    int32_t *eslot;                        // [6n] new cell-edge on each half of each old edge
+    uint32_t *status;                      // [2]
+    int8_t *depth_out, *plotter_out;       // [n'] new mesh
+    int32_t *nbr_out;                      // [3n']
+    double *eta_out, *hu_out, *hv_out, *b_out;
+};
+
+int64_t adapt_scan_tiles(int64_t n);
+// Marks the edges and sweeps until no word changes (at most max_sweeps; then
+// *sweeps = -1).  Synchronises the stream after every sweep.
+hipError_t adapt_conformity(const AdaptArgs &a, hipStream_t s, int max_sweeps, int *sweeps);
+// New-cell counts and their prefix; returns the new cell count, the covered
+// part of the two root trees (2^61 when the mesh is whole) and the status
+// (bit 0: an inconsistent merge).  Synchronises the stream.
+hipError_t adapt_count(const AdaptArgs &a, hipStream_t s, int64_t *n_new, int64_t *tree_total,
+                       uint32_t *status);
+// Writes the new mesh and state (outputs sized for n_new).
+hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s);
+
 hipError_t upload_tables(const double *normal, const double *leg);
 hipError_t upload_tables_adjoint(const double *normal, const double *leg);
 hipError_t launch_adjoint(const AdjArgs &a, int solver, int chunk, int64_t grid_blocks,

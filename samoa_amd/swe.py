@@ -106,6 +106,37 @@ class EulerTimestep:
             _ptr(rf) if flags else None))
         return StepResult(dtc.value, nref.value, rf)
 
+    # -- remeshing --------------------------------------------------------
+    def keep_refinement_flags(self, on: bool = True) -> None:
+        """Every step leaves its refinement flags on the device for adapt()."""
+        _native.check("swe_keep_refinement_flags",
+                      self._lib.swe_keep_refinement_flags(self._h, int(bool(on))))
+
+    def adapt(self, refine=None):
+        """One remeshing step (swe_adapt): conformity of the requests (the
+        last step's flags on the device when `refine` is None), split and
+        merge, state transfer.  Returns (new cell count, conformity sweeps).
+        The reference's traversal direction flips once per adaption."""
+        n_new, sweeps = C.c_int64(), C.c_int()
+        rf = None
+        if refine is not None:
+            rf = np.ascontiguousarray(refine, dtype=np.int8)
+            if rf.shape != (self.n,):
+                raise ValueError(f"refine must have shape ({self.n},), got {rf.shape}")
+        _native.check("swe_adapt", self._lib.swe_adapt(
+            self._h, None if rf is None else _ptr(rf), C.byref(n_new), C.byref(sweeps)))
+        self.n = n_new.value
+        self.mesh = None
+        return self.n, sweeps.value
+
+    def download_mesh(self):
+        """(depth[n], plotter[n], nbr[n, 3]) of the current mesh."""
+        depth, plotter = np.empty(self.n, np.int8), np.empty(self.n, np.int8)
+        nbr = np.empty(3 * self.n, np.int32)
+        _native.check("swe_download_mesh", self._lib.swe_download_mesh(
+            self._h, _ptr(depth), _ptr(plotter), _ptr(nbr)))
+        return depth, plotter, nbr.reshape(self.n, 3)
+
     def run(self, n_steps: int, dt0: float, forward: bool, dt_clamp: float = -1.0,
             time: float = 0.0):
         """n_steps traversals with the dt chain on the device, starting at

@@ -1,0 +1,141 @@
+"""The device remesher (swe_adapt) against the reference's runs that remesh
+before every step and against the oracle's remeshing at larger sizes: the
+new mesh (depths, plotter types, neighbour table) and the transferred state
+bit for bit, through the flags the steps leave on the device and through
+flags passed from the host, across buffer growth and reuse."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from samoa_amd import _native, host, swe
+from test_remesh import RUNS, random_case
+
+pytestmark = pytest.mark.gpu
+
+
+def params(meta):
+    return dict(scaling=meta["scaling"], dry_tolerance=meta["drytol"], courant=meta["courant"],
+                min_depth=meta["dmin"], max_depth=meta["dmax"], boundary=meta["bc"])
+
+
+def check_mesh(op, depth):
+    want = host.mesh_from_depths(depth)
+    d, p, nb = op.download_mesh()
+    assert np.array_equal(d, want.depth)
+    assert np.array_equal(p, want.plotter)
+    assert np.array_equal(nb, want.nbr)
+
+
+@pytest.mark.parametrize("name,outs,meta", RUNS, ids=[r[0] for r in RUNS])
+@pytest.mark.parametrize("flags_from", ["device", "host"])
+def test_follows_reference_remeshing(name, outs, meta, flags_from):
+    print("case:", name, flags_from, flush=True)
+    mesh = host.mesh_from_depths(outs[0]["depth"])
+    op = swe.EulerTimestep(mesh, **params(meta))
+    op.keep_refinement_flags(flags_from == "device")
+    op.upload(*outs[0]["state"])
+    dt = po.initial_dt(mesh.depth, outs[0]["state"], meta["courant"])
+    t = 0.0
+    for k in range(1, len(outs)):
+        a, b = outs[k - 1], outs[k]
+        if k == 1 or flags_from == "host":
+            flags = a["refine"] if k > 1 else np.zeros(len(a["depth"]), np.int8)
+            n, sweeps = op.adapt(flags)
+        else:
+            n, sweeps = op.adapt()
+        assert n == len(b["depth"]) and sweeps >= 1
+        check_mesh(op, b["depth"])
+        res = op.traverse(dt, not b["forward"], flags=True)
+        t += dt
+        assert t == b["time"]
+        eta, hu, hv, bb = op.download()
+        assert np.array_equal(np.stack([eta, hu, hv, bb]), b["state"]), f"step {k}"
+        assert np.array_equal(res.refine, b["refine"]), f"step {k}: flags"
+        dt = meta["courant"] * res.dt_cfl
+    op.close()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_requests_match_oracle(seed):
+    """Rounds of random requests on meshes of 10^4..10^5 cells: mesh and state
+    equal the oracle's after every round; a step on the result equals the
+    oracle's step."""
+    mesh, rng = random_case(seed)
+    target = int(rng.integers(10_000, 100_000))
+    while mesh.n < target:  # grow it by random refinement
+        flags = rng.choice(np.array([0, 1], np.int8), size=mesh.n, p=[0.4, 0.6])
+        mesh = host.mesh_from_depths(po.adapt(mesh.depth, mesh.plotter, mesh.nbr, flags,
+                                              np.zeros((4, mesh.n)), dmin=1, dmax=16)[0])
+    print("case: seed", seed, "cells", mesh.n, flush=True)
+    st = np.stack([5.0 + rng.random(mesh.n), rng.normal(size=mesh.n), rng.normal(size=mesh.n),
+                   -rng.random(mesh.n)])
+    op = swe.EulerTimestep(mesh, scaling=1.0, dry_tolerance=1e-3, min_depth=1, max_depth=16)
+    op.upload(*st)
+    cur = mesh
+    for r in range(4):
+        flags = rng.choice(np.array([-1, 0, 1], np.int8), size=cur.n, p=[0.45, 0.35, 0.2])
+        d2, p2, s2, nb2, _, sweeps = po.adapt(cur.depth, cur.plotter, cur.nbr, flags, st, dmin=1,
+                                              dmax=16, coords=cur.coords)
+        n, sw = op.adapt(flags)
+        assert n == len(d2)
+        d, p, nb = op.download_mesh()
+        assert np.array_equal(d, d2) and np.array_equal(p, p2) and np.array_equal(nb, nb2), r
+        assert np.array_equal(np.stack(op.download()), s2), r
+        cur = host.mesh_from_depths(d2)
+        st = s2
+    eo, ho, vo, rf, cand, nref = po.step(cur.depth, cur.plotter, cur.nbr, st, 1e-4, scaling=1.0,
+                                         drytol=1e-3, dmin=1, dmax=16, forward=False)
+    res = op.traverse(1e-4, False, flags=True)
+    assert np.array_equal(np.stack(op.download()[:3]), np.stack([eo, ho, vo]))
+    assert np.array_equal(res.refine, rf) and res.dt_cfl == cand
+    op.close()
+
+
+def test_growth_and_reuse_of_buffers():
+    """Refine everything from depth 4 to 12 (buffers grow), coarsen back
+    (spare buffers reused), against the uniform meshes and the oracle."""
+    mesh = host.mesh_uniform(4)
+    rng = np.random.default_rng(7)
+    st = rng.normal(size=(4, mesh.n))
+    op = swe.EulerTimestep(mesh, min_depth=2, max_depth=12)
+    op.upload(*st)
+    for d in range(5, 13):
+        n, _ = op.adapt(np.ones(op.n, np.int8))
+        assert n == 2 ** (d + 1)
+        st = np.repeat(st, 2, axis=1)
+    check_mesh(op, np.full(op.n, 12, np.int8))
+    assert np.array_equal(np.stack(op.download()), st)
+    for d in range(11, 1, -1):
+        n, _ = op.adapt(np.full(op.n, -1, np.int8))
+        assert n == 2 ** (d + 1)
+        st = 0.5 * (st[:, 0::2] + st[:, 1::2])
+    check_mesh(op, np.full(op.n, 2, np.int8))
+    assert np.array_equal(np.stack(op.download()), st)
+    # a request beyond min_depth is dropped
+    n, _ = op.adapt(np.full(op.n, -1, np.int8))
+    assert n == op.n == 8
+    op.close()
+
+
+def test_adapt_errors():
+    mesh = host.mesh_uniform(3)
+    op = swe.EulerTimestep(mesh)
+    with pytest.raises(_native.SweError) as e:
+        op.adapt()
+    assert e.value.code == _native.SWE_ERR_STATE  # no state yet
+    op.upload(*np.zeros((4, mesh.n)))
+    with pytest.raises(_native.SweError) as e:
+        op.adapt()  # the flags of a step were not kept
+    assert e.value.code == _native.SWE_ERR_STATE
+    op.close()
+    # half of the unit square: not a whole mesh
+    half = host.mesh_uniform(3)
+    n = half.n // 2
+    nbr = np.where(half.nbr[:n] >= n, -1, half.nbr[:n])
+    part = swe.Mesh(half.depth[:n].copy(), half.plotter[:n].copy(), nbr, half.coords[:n].copy())
+    op = swe.EulerTimestep(part)
+    op.upload(*np.zeros((4, n)))
+    with pytest.raises(_native.SweError) as e:
+        op.adapt(np.zeros(n, np.int8))
+    assert e.value.code == _native.SWE_ERR_ARG and "leaf sequence" in str(e.value)
+    op.close()
