@@ -47,6 +47,33 @@ module swe_mi355x
             type(c_ptr), value :: eta, hu, hv, b
         end function
 
+        !> swe%adaption%traverse (SWE.f90:387-390); refine = c_null_ptr takes
+        !> the last step's flags left on the device (swe_keep_refinement_flags)
+        integer(c_int) function swe_adapt(handle, refine, n_cells, sweeps) bind(c, name='swe_adapt')
+            import :: c_int, c_ptr, c_int64_t
+            type(c_ptr), value        :: handle
+            type(c_ptr), value        :: refine
+            integer(c_int64_t)        :: n_cells
+            integer(c_int)            :: sweeps
+        end function
+
+        integer(c_int) function swe_keep_refinement_flags(handle, on) bind(c, name='swe_keep_refinement_flags')
+            import :: c_int, c_ptr
+            type(c_ptr), value      :: handle
+            integer(c_int), value   :: on
+        end function
+
+        integer(c_int64_t) function swe_cells(handle) bind(c, name='swe_cells')
+            import :: c_int64_t, c_ptr
+            type(c_ptr), value :: handle
+        end function
+
+        integer(c_int) function swe_download_mesh(handle, depth, plotter, nbr) bind(c, name='swe_download_mesh')
+            import :: c_int, c_ptr
+            type(c_ptr), value :: handle
+            type(c_ptr), value :: depth, plotter, nbr
+        end function
+
         integer(c_int) function swe_step(handle, dt, forward, dt_cfl, n_refine, refine) bind(c, name='swe_step')
             import :: c_int, c_ptr, c_double, c_int64_t
             type(c_ptr), value          :: handle

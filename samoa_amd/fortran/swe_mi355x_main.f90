@@ -1,12 +1,24 @@
 ! swe_mi355x_main.f90 -- command-line SWE run on the MI355X, the counterpart
 ! of the reference's `samoa` binary built with scenario=swe (SFC_main.f90,
-! SWE.f90:205-444) for frozen uniform meshes.
+! SWE.f90:205-444).
 !
-!   swe_mi355x -dmax 14 -nmax 100 [-dmin d] [-tmax t] [-tout t] [-nout k]
+!   swe_mi355x -dmax 14 -nmax 100 [-dmin d] [-dstart d] [-nadapt k]
+!              [-tmax t] [-tout t] [-nout k]
 !              [-courant c] [-drytolerance d] [-scenario dambreak|radial|tsunami]
 !              [-seed s] [-boundary outflow|wall] [-solver aug_riemann|fwave]
 !              [-dump prefix] [-hoststeps] [-xmloutput .true.] [-output_dir dir]
 !              [-stestpoints "x y,x y,..."] [-pointoutput_full .true.]
+!              [-dam_left hL] [-dam_right hR]
+!
+! The mesh starts uniform at depth -dstart (default: -dmax).  Below -dmax the
+! initial phase of SWE.f90:230-262 refines it: a cell shallower than -dmin,
+! or one whose corners see different initial water heights
+! (SWE_initialize.f90:292-320), is refined, closed by the conformity rules,
+! until no cell asks (dambreak and radial scenarios).  With -nadapt k the
+! mesh is remeshed on the device before every k-th step from the refinement
+! flags of the step before (SWE.f90:387-390); each remeshing flips the
+! traversal direction, as the reference's adaption traversal does.  The dam
+! break's heights default to the reference's 1 m | dry (SWE_initialize.f90:354).
 !
 ! Like the reference CLI (Config.f90:213-296, M_kracken rget), -courant,
 ! -drytolerance, -tmax and -tout are read as default REAL before they are
@@ -30,10 +42,13 @@ program swe_mi355x_main
     use swe_gauges
     implicit none
 
-    integer :: dmax, dmin, nout, i, narg, ierr, depth_i
+    integer :: dmax, dmin, dstart, nadapt, nout, i, narg, ierr, depth_i
     integer(c_int64_t) :: nmax, n, seed, steps, k, nsteps_done, chunk
     real :: r_courant, r_drytol, r_tmax, r_tout
     real(c_double) :: courant, drytol, tmax, tout, scaling, offset, dt, time, amp, ratio, x0, y0
+    real(c_double) :: dam_left, dam_right
+    integer(c_int64_t) :: n_new
+    integer(c_int) :: sweeps
     character(len=256) :: arg, val, scenario, dump, bcname, solvername, outdir
     character(len=4096) :: probes
     logical :: host_steps, forward, xml, probe_full
@@ -48,7 +63,8 @@ program swe_mi355x_main
     integer(c_int64_t) :: t0, t1, rate
     integer :: iout
 
-    dmax = 14; dmin = -1; nmax = -1; nout = -1
+    dmax = 14; dmin = -1; dstart = -1; nadapt = -1; nmax = -1; nout = -1
+    dam_left = 1.0_c_double; dam_right = 0.0_c_double
     r_courant = 0.45; r_drytol = 0.01; r_tmax = 3600.0; r_tout = -1.0
     scenario = 'dambreak'; dump = ''; bcname = 'outflow'; solvername = 'aug_riemann'
     seed = 0; host_steps = .false.; xml = .false.; outdir = 'output'; probes = ''
@@ -60,8 +76,12 @@ program swe_mi355x_main
         val = ''
         if (i < narg) call get_command_argument(i + 1, val)
         select case (trim(arg))
-        case ('-dmax', '-dstart'); read(val, *) dmax; i = i + 1
+        case ('-dmax'); read(val, *) dmax; i = i + 1
+        case ('-dstart'); read(val, *) dstart; i = i + 1
         case ('-dmin'); read(val, *) dmin; i = i + 1
+        case ('-nadapt'); read(val, *) nadapt; i = i + 1
+        case ('-dam_left'); read(val, *) dam_left; i = i + 1
+        case ('-dam_right'); read(val, *) dam_right; i = i + 1
         case ('-nmax'); read(val, *) nmax; i = i + 1
         case ('-nout'); read(val, *) nout; i = i + 1
         case ('-tmax'); read(val, *) r_tmax; i = i + 1
@@ -85,37 +105,34 @@ program swe_mi355x_main
         i = i + 1
     end do
     if (dmin < 0) dmin = dmax
+    if (dstart < 0) dstart = dmax
     courant = r_courant; drytol = r_drytol; tmax = r_tmax; tout = r_tout
     call parse_probes(probes)
-
-    n = ishft(1_c_int64_t, dmax + 1)
-    allocate(plotter(n), depth(n), nbr(3, n), coords(2, 3, n), eta(n), hu(n), hv(n), b(n), refine(n))
-    refine = 0
-    call sfc_uniform_mesh(dmax, plotter, nbr, coords, ierr)
-    if (ierr /= 0) then
-        print '(A, I0)', 'swe_mi355x: mesh generation failed: ', ierr
-        stop 1
-    end if
-    depth = int(dmax, c_int8_t)
-
     select case (trim(scenario))
     case ('dambreak')
         scaling = 10.0_c_double; offset = -5.0_c_double
-        call swe_host_reference_dam_break_f(n, coords, eta, hu, hv, b)
     case ('radial')
         scaling = 1.0_c_double; offset = 0.0_c_double
-        do k = 1, n
-            eta(k) = merge(10.0_c_double, 5.0_c_double, norm2(centroid(coords(:, :, k)) - 0.5_c_double) < 0.1_c_double)
-        end do
-        hu = 0; hv = 0; b = 0
     case ('tsunami')
         scaling = 1.0e6_c_double; offset = 0.0_c_double
-        call tsunami_draw(seed, amp, ratio, x0, y0, depth_i, dmax, dmax)
-        call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+        if (dstart /= dmax) then
+            print '(A)', 'swe_mi355x: the tsunami scenario starts at -dmax (no initial refinement)'
+            stop 2
+        end if
     case default
         print '(A, A)', 'swe_mi355x: unknown scenario ', trim(scenario)
         stop 2
     end select
+
+    n = ishft(1_c_int64_t, dstart + 1)
+    allocate(plotter(n), depth(n), nbr(3, n), coords(2, 3, n), eta(n), hu(n), hv(n), b(n), refine(n))
+    refine = 0
+    call sfc_uniform_mesh(dstart, plotter, nbr, coords, ierr)
+    if (ierr /= 0) then
+        print '(A, I0)', 'swe_mi355x: mesh generation failed: ', ierr
+        stop 1
+    end if
+    depth = int(dstart, c_int8_t)
 
     desc%n_cells = n
     desc%depth = c_loc(depth)
@@ -131,8 +148,46 @@ program swe_mi355x_main
     desc%boundary = merge(SWE_BC_WALL, SWE_BC_OUTFLOW, trim(bcname) == 'wall')
     rc = swe_create(desc, h)
     call check(rc, 'swe_create')
+
+    ! initial phase (SWE.f90:230-262): refine until no cell asks
+    if (dstart < dmax .or. dstart < dmin) then
+        eta = 0; hu = 0; hv = 0; b = 0
+        rc = swe_upload(h, eta, hu, hv, b)
+        call check(rc, 'swe_upload')
+        do
+            do k = 1, n
+                refine(k) = 0
+                if (depth(k) < dmin) then
+                    refine(k) = 1
+                else if (depth(k) < dmax) then
+                    if (corners_differ(coords(:, :, k))) refine(k) = 1
+                end if
+            end do
+            if (count(refine(1:n) == 1) == 0) exit
+            rc = swe_adapt(h, c_loc(refine), n_new, sweeps)
+            call check(rc, 'swe_adapt')
+            call sync_mesh()
+        end do
+        refine = 0
+    end if
+    select case (trim(scenario))
+    case ('dambreak')
+        call swe_host_reference_dam_break_f(n, coords, eta, hu, hv, b)
+    case ('radial')
+        do k = 1, n
+            eta(k) = radial_eta(centroid(coords(:, :, k)))
+        end do
+        hu = 0; hv = 0; b = 0
+    case ('tsunami')
+        call tsunami_draw(seed, amp, ratio, x0, y0, depth_i, dmax, dmax)
+        call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+    end select
     rc = swe_upload(h, eta, hu, hv, b)
     call check(rc, 'swe_upload')
+    if (nadapt > 0) then
+        rc = swe_keep_refinement_flags(h, 1_c_int)
+        call check(rc, 'swe_keep_refinement_flags')
+    end if
 
     ! init_grid leaves the grid reversed; init_b runs backward, init_dofs
     ! forward (SWE.f90:230-234); the initial output traversal flips once more.
@@ -158,7 +213,26 @@ program swe_mi355x_main
                 chunk = min(chunk, nmax - nsteps_done)
             end if
         end if
-        if (xml .and. dmin < dmax) then
+        if (nadapt > 0) then
+            ! one step at a time, remeshing before every nadapt-th (SWE.f90:385-390);
+            ! before the first step the flags are the initial phase's, all 0
+            if (mod(nsteps_done + 1, int(nadapt, c_int64_t)) == 0) then
+                if (nsteps_done == 0) then
+                    refine = 0
+                    rc = swe_adapt(h, c_loc(refine), n_new, sweeps)
+                else
+                    rc = swe_adapt(h, c_null_ptr, n_new, sweeps)
+                end if
+                call check(rc, 'swe_adapt')
+                forward = .not. forward
+                if (n_new /= size(refine, kind=c_int64_t)) then
+                    deallocate(refine)
+                    allocate(refine(n_new))
+                end if
+            end if
+            rc = swe_time_loop(h, courant, tmax, tout, 1_c_int64_t, .true., dt, time, forward, steps, &
+                               c_loc(refine))
+        else if (xml .and. dmin < dmax) then
             ! the output carries the refinement flags of the last step
             rc = swe_time_loop(h, courant, tmax, tout, chunk, .true., dt, time, forward, steps, &
                                c_loc(refine))
@@ -169,6 +243,7 @@ program swe_mi355x_main
         call check(rc, 'swe_time_loop')
         nsteps_done = nsteps_done + steps
         if (nout > 0 .and. mod(nsteps_done, int(nout, c_int64_t)) == 0) then
+            if (nadapt > 0 .and. (len_trim(dump) > 0 .or. xml .or. npt > 0)) call sync_mesh()
             if (len_trim(dump) > 0 .or. xml .or. npt > 0) then
                 rc = swe_download(h, c_loc(eta), c_loc(hu), c_loc(hv), c_loc(b))
                 call check(rc, 'swe_download')
@@ -179,6 +254,7 @@ program swe_mi355x_main
         if (steps == 0) exit
     end do
     call system_clock(t1)
+    if (nadapt > 0) call sync_mesh()
     rc = swe_download(h, c_loc(eta), c_loc(hu), c_loc(hv), c_loc(b))
     call check(rc, 'swe_download')
     if (len_trim(dump) > 0 .and. nout <= 0) call write_dump(0)
@@ -197,6 +273,59 @@ contains
             stop 1
         end if
     end subroutine
+
+    ! The handle's current mesh on the host: depths, plotter types and
+    ! neighbours from the device, corners from the depth sequence; the state
+    ! arrays resized to match (their contents are downloaded when needed).
+    ! The refinement flags of the last step are kept.
+    subroutine sync_mesh()
+        integer(c_int8_t), allocatable :: keep(:)
+        n = swe_cells(h)
+        if (size(depth, kind=c_int64_t) /= n) then
+            allocate(keep(min(n, size(refine, kind=c_int64_t))))
+            keep = refine(1:size(keep))
+            deallocate(plotter, depth, nbr, coords, eta, hu, hv, b, refine)
+            allocate(plotter(n), depth(n), nbr(3, n), coords(2, 3, n), eta(n), hu(n), hv(n), b(n), &
+                     refine(n))
+            refine = 0
+            refine(1:size(keep)) = keep
+        end if
+        rc = swe_download_mesh(h, c_loc(depth), c_loc(plotter), c_loc(nbr))
+        call check(rc, 'swe_download_mesh')
+        call sfc_mesh_from_depths(n, depth, plotter, coords, ierr)
+        if (ierr /= 0) then
+            print '(A)', 'swe_mi355x: the device mesh is not a leaf sequence'
+            stop 1
+        end if
+    end subroutine
+
+    ! The initial water height of the scenario at a point of the unit square.
+    real(c_double) function initial_h(x)
+        real(c_double), intent(in) :: x(2)
+        real(c_double) :: xs(2)
+        if (trim(scenario) == 'radial') then
+            initial_h = radial_eta(x)
+        else
+            xs = scaling * x + offset
+            initial_h = merge(dam_left, dam_right, xs(1) < 0.0_c_double)
+        end if
+    end function
+
+    ! SWE_initialize.f90:308-319: refine where the corners see different
+    ! initial water heights.
+    logical function corners_differ(x)
+        real(c_double), intent(in) :: x(2, 3)
+        real(c_double) :: h1, h2, h3
+        h1 = initial_h(x(:, 1))
+        h2 = initial_h(x(:, 2))
+        h3 = initial_h(x(:, 3))
+        corners_differ = max(h1, h2, h3) > min(h1, h2, h3)
+    end function
+
+    real(c_double) function radial_eta(x)
+        real(c_double), intent(in) :: x(2)
+        radial_eta = merge(10.0_c_double, 5.0_c_double, norm2(x - 0.5_c_double) < 0.1_c_double)
+    end function
 
     subroutine write_dump(idx)
         integer, intent(in) :: idx
@@ -280,7 +409,7 @@ contains
         real(c_double) :: p(2)
         do j = 1, nc
             p = 10.0_c_double * centroid(x(:, :, j)) - 5.0_c_double
-            e(j) = merge(1.0_c_double, 0.0_c_double, p(1) < 0.0_c_double)
+            e(j) = merge(dam_left, dam_right, p(1) < 0.0_c_double)
         end do
         u = 0; v = 0; bb = 0
     end subroutine

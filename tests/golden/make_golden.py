@@ -363,6 +363,40 @@ def write_remesh_run(name: str, binary: str, dstart: int, dmin: int, dmax: int, 
 def write_remesh_fixtures() -> None:
     write_remesh_run("dam_d3_12", "samoa_swe_dam", 3, 3, 12, 30, 0)
     write_remesh_run("dam_d2_9", "samoa_swe_dam", 2, 2, 9, 30, 0)
+    write_remesh_vtu_digests()
+
+
+# runs of the whole reference program with the initial refinement (-dstart <
+# -dmax) and remeshing before every step, VTK output after every step
+REMESH_CLI_RUNS = {
+    "plain_d2_9": ("samoa_swe", ["-dstart", "2", "-dmin", "2", "-dmax", "9", "-nmax", "12"]),
+    "dam_d3_12": ("samoa_swe_dam", ["-dstart", "3", "-dmin", "3", "-dmax", "12", "-nmax", "20"]),
+}
+
+
+def write_remesh_vtu_digests() -> None:
+    """SHA-256 of every VTK file the reference writes in REMESH_CLI_RUNS
+    (-nadapt 1 -nout 1), so that the CLI's files can be compared byte for
+    byte without storing them (vtu/remesh_digests.json)."""
+    import hashlib
+
+    out = {}
+    for name, (binary, args) in REMESH_CLI_RUNS.items():
+        od = tempfile.mkdtemp(prefix="swe_vtu_")
+        try:
+            cmd = [os.path.join(REF, binary), "-threads", "1", "-sections", "1", *args, "-nadapt",
+                   "1", "-tmax", "-1", "-xmloutput", ".true.", "-nout", "1", "-output_dir", od]
+            subprocess.run(cmd, check=True, capture_output=True)
+            files = {}
+            for f in glob.glob(os.path.join(od, "*_r0_s1.vtu")):
+                i = int(f.split("_")[-3])
+                files[str(i)] = hashlib.sha256(open(f, "rb").read()).hexdigest()
+        finally:
+            shutil.rmtree(od, ignore_errors=True)
+        out[name] = dict(binary=binary, args=args, files=files)
+        print(f"{name}: {len(files)} VTK files")
+    with open(os.path.join(HERE, "vtu", "remesh_digests.json"), "w") as fh:
+        json.dump(out, fh, indent=1, sort_keys=True)
 
 
 def write_adaptive_fixtures() -> None:

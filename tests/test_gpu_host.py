@@ -132,3 +132,30 @@ def test_tmax_stops_and_clamps():
     dt, t, fwd, steps = host.time_loop(op, dt=dt0, forward=False, tmax=0.05, host_steps=True)
     assert t >= 0.05 and steps > 1
     op.close()
+
+
+@pytest.mark.parametrize("name", ["plain_d2_9", "dam_d3_12"])
+def test_cli_remeshing_run_identical_to_reference_files(name):
+    """The whole program with its initial refinement (-dstart below -dmax)
+    and remeshing before every step (-nadapt 1) on the device: every VTK file
+    equals the reference's byte for byte (tests/golden/vtu/remesh_digests.json;
+    dam_d3_12 is the reference's dam break raised to 10 m | 5 m)."""
+    import hashlib
+    import json
+
+    vdir = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "vtu")
+    ref = json.load(open(os.path.join(vdir, "remesh_digests.json")))[name]
+    extra = ["-dam_left", "10", "-dam_right", "5"] if ref["binary"] == "samoa_swe_dam" else []
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [host.CLI, "-scenario", "dambreak", *ref["args"], *extra, "-nadapt", "1", "-tmax",
+               "-1", "-xmloutput", ".true.", "-nout", "1", "-output_dir", td]
+        print("case:", " ".join(cmd[1:]), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for i in sorted(ref["files"], key=int):
+            path = os.path.join(td, f"swe_{i}_r0_s1.vtu")
+            mine = hashlib.sha256(open(path, "rb").read()).hexdigest()
+            if mine != ref["files"][i]:
+                v = po.read_vtu(path)
+                pytest.fail(f"output {i} differs: {len(v['depth'])} cells, time {v['time'][0]!r}, "
+                            f"depths {np.bincount(v['depth'])}")
