@@ -102,7 +102,10 @@ int swe_step(swe_handle *h, double dt, int forward, double *dt_cfl, int64_t *n_r
  * every step.  *time is the simulated time before the steps on entry and is
  * advanced by each step's dt on the device (t = t + dt, in the reference's
  * order); *dt_next receives the dt of the next step.  The steps are captured
- * once into a HIP graph and replayed. */
+ * once into a HIP graph and replayed.
+ * dt0 <= 0 continues the device chain (dt and time) where the last swe_run
+ * left it; *time is then not read.  With time and dt_next both NULL the call
+ * returns without waiting for the steps (time is required when dt0 > 0). */
 int swe_run(swe_handle *h, int64_t n_steps, double dt0, int forward, double dt_clamp,
             double *time, double *dt_next);
 
@@ -164,6 +167,27 @@ int swe_adjoint_step_device(swe_handle *h, const double *eta, const double *hu, 
  * handles only.  Device memory grows as the mesh does; it is not returned
  * when the mesh shrinks. */
 int swe_adapt(swe_handle *h, const int8_t *refine, int64_t *n_cells, int *sweeps);
+
+/* Bathymetry of the cells that swe_adapt splits, in place of the parent's
+ * value: bilinear interpolation in a raster of nx x ny values over the world
+ * points (x0 + i dx, y0 + j dy), values[j * nx + i], at the child's centroid
+ * in world coordinates, scaling * (x1 + x2 + x3) / 3 + offset (unit-square
+ * corners), clamped to the raster's last cell.  This stands for
+ * get_bathymetry_at_element (SWE_initialize.f90:120-147) with gridded data
+ * (the ASAGI path, SWE_initialize.f90:149-187).  The library copies the
+ * values; NULL removes the raster. */
+int swe_set_bathymetry_raster(swe_handle *h, int64_t nx, int64_t ny, double x0, double y0,
+                              double dx, double dy, double offset, const double *values);
+
+/* Refinement flags from the gradient of the surface elevation, for runs that
+ * refine on |grad eta| rather than on volume_op's |dQ_h| criterion
+ * (SWE_euler_timestep.f90:336-344); no counterpart in the reference.  For a
+ * wet cell (eta - b > dry_tolerance), j = the largest |eta - eta_n| over its
+ * wet neighbours n; a dry cell has j = 0.  The flag is 1 if j > refine_tol
+ * and depth < max_depth, -1 if j < coarsen_tol and depth > min_depth, else
+ * 0.  Replaces the device flags that swe_adapt(h, NULL, ...) reads.
+ * Asynchronous on the handle's stream. */
+int swe_flag_eta_jumps(swe_handle *h, double refine_tol, double coarsen_tol);
 
 /* on != 0: every step of swe_step and swe_run leaves its refinement flags on
  * the device for swe_adapt(h, NULL, ...). */

@@ -46,7 +46,7 @@ def build_hip(verbose: bool = False) -> str:
 
 FORTRAN_SOURCES = ["swe_mi355x.f90", "swe_sfc_mesh.f90", "swe_scenarios.f90", "swe_vtu.f90",
                    "swe_gauges.f90", "swe_driver.f90"]
-FFLAGS = ["-O2", "-fPIC"]
+FFLAGS = ["-O2", "-fPIC", "-ffp-contract=off"]
 
 
 def build_fortran() -> None:

@@ -14,7 +14,9 @@
 //   3. counts of new cells (split 2..4, merge 1 for the first child of a
 //      pair and 0 for the second, else 1), an exclusive scan of them.
 //   4. emit: depth, plotter type and state of every new cell (a split copies
-//      eta, hu, hv and b, a merge averages all four, SWE_adapt.f90:95-169),
+//      eta, hu, hv and b, or samples b from a raster at the child's centroid
+//      as get_bathymetry_at_element does from its data; a merge averages all
+//      four, SWE_adapt.f90:95-169),
 //      the neighbours inside each old cell, and for every old cell-edge the
 //      new cell-edge that lies on each half of it.
 //   5. link: a new edge on half s of old edge k of cell A meets the new edge on
@@ -104,6 +106,7 @@ __global__ __launch_bounds__(AT) void conf_init_kernel(int64_t n, const int8_t *
     if (c == 0) {
         status[0] = 0u;
         status[1] = 0u;
+        status[2] = 0u;
     }
     if (c >= n) return;
     int f = flags[c];
@@ -115,12 +118,16 @@ __global__ __launch_bounds__(AT) void conf_init_kernel(int64_t n, const int8_t *
     slots[c] = legs | (hyp << 2) | (legs << 4);
 }
 
-// mark_edges_integrity (Conformity.f90:1271-1317), in place.
+// mark_edges_integrity (Conformity.f90:1271-1317), in place.  *changed gets
+// the sweep's number if a word changed, *acting if a cell's word asks for a
+// split (hypotenuse refined) or a merge (legs coarsened): in a sweep that
+// changes nothing, those are the cells the remeshing will split or merge.
 __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t *depth,
                                                         const int32_t *nbr, uint32_t *slots,
-                                                        uint32_t *changed, uint32_t sweep) {
+                                                        uint32_t *changed, uint32_t *acting,
+                                                        uint32_t sweep) {
     const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
-    bool diff = false;
+    bool diff = false, act = false;
     if (c < n) {
         const uint32_t own = ld_relaxed(&slots[c]);
         const uint32_t v0 = edge_value(slots, nbr, c, own, 0);
@@ -133,8 +140,15 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
             st_relaxed(&slots[c], nw);
             diff = true;
         }
+        act = rh || cc;
     }
-    if (__any(diff) && (threadIdx.x & 63) == 0) st_relaxed(changed, sweep);
+    // one status write per workgroup, and only if no other has written it:
+    // stores from every wave to one word contend across the whole device
+    const int any_diff = __syncthreads_or(diff), any_act = __syncthreads_or(act);
+    if (threadIdx.x == 0) {
+        if (any_diff && ld_relaxed(changed) != sweep) st_relaxed(changed, sweep);
+        if (any_act && ld_relaxed(acting) != sweep) st_relaxed(acting, sweep);
+    }
 }
 
 __device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
@@ -261,7 +275,8 @@ __global__ __launch_bounds__(AT) void count_kernel(int64_t n, const int8_t *dept
         m = first ? 1 : 0;
     }
     code[c] = k;
-    count[c] = m;
+    // the high word counts the cells that change, through the same scan
+    count[c] = m | ((int64_t)(k != CODE_KEEP) << 32);
 }
 
 __device__ __forceinline__ int split_type(int p, int rule) {
@@ -274,6 +289,45 @@ __device__ __forceinline__ int split_type(int p, int rule) {
         case 4: return c_child[c2 + 8];
         default: return -c_child[-c2 + 8];
     }
+}
+
+// Centroid (unit square) of the cell at tree offset `off` and depth d: the
+// corners of the root, then d halvings along the path the offset spells
+// (bit 60 - l chooses the second child at level l), with the arithmetic of
+// the host's mesh builders: m = 0.5 (x1 + x3), centroid (x1 + x2 + x3) / 3.
+__device__ void tree_centroid(int64_t off, int d, double *cx, double *cy) {
+    double ax, ay, bx, by, ex, ey;  // corners 1, 2, 3
+    if ((off >> 60) == 0) {
+        ax = 0.0; ay = 0.0; bx = 1.0; by = 0.0; ex = 1.0; ey = 1.0;
+    } else {
+        ax = 1.0; ay = 1.0; bx = 0.0; by = 1.0; ex = 0.0; ey = 0.0;
+    }
+    for (int l = 1; l <= d; ++l) {
+        const double mx = 0.5 * (ax + ex), my = 0.5 * (ay + ey);
+        if ((off >> (60 - l)) & 1) {  // (x2, m, x3)
+            ax = bx;
+            ay = by;
+        } else {  // (x1, m, x2)
+            ex = bx;
+            ey = by;
+        }
+        bx = mx;
+        by = my;
+    }
+    *cx = (ax + bx + ex) / 3.0;
+    *cy = (ay + by + ey) / 3.0;
+}
+
+// Bilinear sample of the raster at the world point (X, Y), clamped to its
+// last cell (the host's scenario code evaluates the same expression).
+__device__ double raster_sample(const AdaptArgs &a, double X, double Y) {
+    const double fx = (X - a.rx0) / a.rdx, fy = (Y - a.ry0) / a.rdy;
+    const int64_t ix = min(max((int64_t)fx, (int64_t)0), a.rnx - 2);
+    const int64_t iy = min(max((int64_t)fy, (int64_t)0), a.rny - 2);
+    const double wx = fx - (double)ix, wy = fy - (double)iy;
+    const double *r0 = a.raster + iy * a.rnx + ix, *r1 = r0 + a.rnx;
+    return (1.0 - wx) * (1.0 - wy) * r0[0] + wx * (1.0 - wy) * r0[1] + (1.0 - wx) * wy * r1[0] +
+           wx * wy * r1[1];
 }
 
 __device__ __forceinline__ void put_slot(int32_t *eslot, int64_t c, int k, int half, int32_t v) {
@@ -289,7 +343,7 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
     const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
     if (c >= a.n) return;
     const int8_t k = a.code[c];
-    const int64_t b0 = a.base[c];
+    const int64_t b0 = a.base[c] & 0xffffffffLL;  // the high word counts changing cells
     const int d = a.depth[c];
     const int p = a.plotter[c];
     if (k == CODE_KEEP) {
@@ -320,15 +374,24 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
     } else {
         const int pat = k - CODE_SPLIT;
         const int cells = 2 + (pat & 1) + (pat >> 1);
+        int64_t coff = a.off[c];  // tree offset of the next child
         for (int j = 0; j < cells; ++j) {
             const SplitCell sc = c_split[pat][j];
             const int64_t idx = b0 + j;
-            a.depth_out[idx] = (int8_t)(d + sc.dd);
+            const int dc = d + sc.dd;
+            a.depth_out[idx] = (int8_t)dc;
             a.plotter_out[idx] = (int8_t)split_type(p, sc.rule);
             a.eta_out[idx] = a.eta[c];
             a.hu_out[idx] = a.hu[c];
             a.hv_out[idx] = a.hv[c];
-            a.b_out[idx] = a.b[c];
+            if (a.raster) {
+                double cx, cy;
+                tree_centroid(coff, dc, &cx, &cy);
+                a.b_out[idx] = raster_sample(a, a.scaling * cx + a.offset, a.scaling * cy + a.offset);
+            } else {
+                a.b_out[idx] = a.b[c];
+            }
+            coff += (int64_t)1 << (60 - dc);
             for (int e = 0; e < 3; ++e) {
                 const int src = sc.e[e];
                 if (src & 0x20) {
@@ -378,54 +441,118 @@ hipError_t scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *bsum, h
     return hipGetLastError();
 }
 
+// Refinement flags from the largest jump of the surface elevation across an
+// edge to a wet neighbour (a wet cell only; h = eta - b > drytol).
+__global__ __launch_bounds__(AT) void eta_jump_kernel(int64_t n, const double *eta,
+                                                      const double *b, const int32_t *nbr,
+                                                      const int8_t *depth, double drytol,
+                                                      double refine_tol, double coarsen_tol,
+                                                      int dmin, int dmax, int8_t *flags) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (c >= n) return;
+    const double e = eta[c];
+    double j = 0.0;
+    if (e - b[c] > drytol) {
+        for (int k = 0; k < 3; ++k) {
+            const int32_t q = nbr[3 * c + k];
+            if (q < 0) continue;
+            const int64_t o = q >> 2;
+            const double eo = eta[o];
+            if (eo - b[o] > drytol) j = fmax(j, fabs(e - eo));
+        }
+    }
+    const int d = depth[c];
+    int8_t f = 0;
+    if (d < dmax && j > refine_tol) f = 1;
+    else if (d > dmin && j < coarsen_tol) f = -1;
+    flags[c] = f;
+}
+
+// res[0] = the covered part of the two root trees, res[1] = the new cell
+// count, res[2] = status bits, res[3] = the last sweep that changed a word.
+__global__ void copy_total_kernel(int64_t *dst, const int64_t *bsum, int64_t tiles) {
+    if (threadIdx.x == 0) *dst = bsum[tiles];
+}
+__global__ void results_kernel(int64_t *res, const int64_t *bsum, int64_t tiles,
+                               const uint32_t *status) {
+    if (threadIdx.x == 0) {
+        res[1] = bsum[tiles] & 0xffffffffLL;
+        res[2] = status[0] | ((bsum[tiles] >> 32) != 0 ? 2u : 0u);
+    }
+}
+// res[3] = the last sweep that changed a word, res[4] = the last sweep that
+// saw a cell ask for a split or merge
+__global__ void sweep_results_kernel(int64_t *res, const uint32_t *status) {
+    if (threadIdx.x == 0) {
+        res[3] = status[1];
+        res[4] = status[2];
+    }
+}
+
 }  // namespace
 
 int64_t adapt_scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
-hipError_t adapt_conformity(const AdaptArgs &a, hipStream_t s, int max_sweeps, int *sweeps) {
+hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
+                      int max_sweeps, int64_t *host, AdaptPlan *plan) {
     const unsigned nb = blocks_for(a.n, AT);
-    // status[1] holds the number of the last sweep that changed a word
+    const int64_t tiles = adapt_scan_tiles(a.n);
+    int64_t *res = a.bsum + tiles + 1;
+    hipError_t e;
+    auto fetch = [&](int first, int count) -> hipError_t {
+        hipError_t r = hipMemcpyAsync(host + first, res + first, count * sizeof(int64_t),
+                                      hipMemcpyDeviceToHost, s);
+        return r != hipSuccess ? r : hipStreamSynchronize(s);
+    };
     conf_init_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.flags, a.dmin, a.dmax, a.slots, a.status);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    for (int it = 1; it <= max_sweeps; ++it) {
-        uint32_t last = 0;
-        conf_sweep_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.status + 1,
-                                            (uint32_t)it);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (!offsets_valid) {  // else the offsets of the unchanged mesh are in place
+        tree_size_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.off);
+        if ((e = scan_i64(a.off, a.off, a.n, a.bsum, s)) != hipSuccess) return e;
+        copy_total_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles);
+    }
+    // sweeps go in batches between host checks: a batch whose last sweep
+    // changed nothing has reached the fixpoint (further sweeps are no-ops)
+    plan->sweeps = -1;
+    int batch = first_batch < 2 ? 2 : first_batch, done = 0;
+    for (; done < max_sweeps; done += batch, batch = 4) {
+        for (int j = 1; j <= batch; ++j)
+            conf_sweep_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.status + 1,
+                                                a.status + 2, (uint32_t)(done + j));
+        sweep_results_kernel<<<1, 64, 0, s>>>(res, a.status);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(&last, a.status + 1, sizeof last, hipMemcpyDeviceToHost, s)) !=
-            hipSuccess)
-            return e;
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
-        if (last != (uint32_t)it) {
-            *sweeps = it;
-            return hipSuccess;
+        if ((e = fetch(0, 5)) != hipSuccess) return e;
+        if (host[3] < done + batch) {
+            plan->sweeps = (int)host[3] + 1;
+            done += batch;
+            break;
         }
     }
-    *sweeps = -1;
+    if (plan->sweeps < 0) return hipSuccess;
+    plan->tree_total = offsets_valid ? (int64_t(1) << 61) : host[0];
+    if (host[4] < done) {  // no cell splits or merges: the mesh stands
+        plan->n_new = a.n;
+        plan->status = 0;
+        return hipSuccess;
+    }
+    count_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.off, a.code, a.base,
+                                   a.status);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = scan_i64(a.base, a.base, a.n, a.bsum, s)) != hipSuccess) return e;
+    results_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles, a.status);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = fetch(1, 2)) != hipSuccess) return e;
+    plan->n_new = host[1];
+    plan->status = (uint32_t)host[2];
     return hipSuccess;
 }
 
-hipError_t adapt_count(const AdaptArgs &a, hipStream_t s, int64_t *n_new, int64_t *tree_total,
-                       uint32_t *status) {
-    const unsigned nb = blocks_for(a.n, AT);
-    const int64_t tiles = adapt_scan_tiles(a.n);
-    hipError_t e;
-    tree_size_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.off);
-    if ((e = scan_i64(a.off, a.off, a.n, a.bsum, s)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(tree_total, a.bsum + tiles, sizeof(int64_t), hipMemcpyDeviceToHost, s)) !=
-        hipSuccess)
-        return e;
-    count_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.off, a.code, a.base, a.status);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = scan_i64(a.base, a.base, a.n, a.bsum, s)) != hipSuccess) return e;
-    if ((e = hipMemcpyAsync(n_new, a.bsum + tiles, sizeof(int64_t), hipMemcpyDeviceToHost, s)) !=
-        hipSuccess)
-        return e;
-    if ((e = hipMemcpyAsync(status, a.status, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) !=
-        hipSuccess)
-        return e;
-    return hipStreamSynchronize(s);
+hipError_t flag_eta_jumps(int64_t n, const double *eta, const double *b, const int32_t *nbr,
+                          const int8_t *depth, double drytol, double refine_tol,
+                          double coarsen_tol, int dmin, int dmax, int8_t *flags, hipStream_t s) {
+    eta_jump_kernel<<<blocks_for(n, AT), AT, 0, s>>>(n, eta, b, nbr, depth, drytol, refine_tol,
+                                                    coarsen_tol, dmin, dmax, flags);
+    return hipGetLastError();
 }
 
 hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s) {

@@ -224,6 +224,12 @@ struct Batch {
     int8_t *ad_code = nullptr;
     int32_t *ad_eslot = nullptr;
     int64_t ad_cap = 0;
+    bool ad_off_valid = false;  // ad_off holds the current mesh's tree offsets
+    int64_t *ad_host = nullptr; // [8] pinned results of the remesher
+    int ad_last_sweeps = 3;     // sweeps the last remeshing needed
+    double *raster = nullptr;  // bathymetry raster for split cells (swe_set_bathymetry_raster)
+    int64_t rnx = 0, rny = 0;
+    double rx0 = 0, ry0 = 0, rdx = 1, rdy = 1, roffset = 0;
     int keep_flags = 0;       // every step writes its refinement flags to `refine`
     bool flags_valid = false; // `refine` holds the flags of the current state's last step
 };
@@ -267,6 +273,8 @@ void release(Batch &B) {
     (void)hipFree(B.ad_bsum);
     (void)hipFree(B.ad_code);
     (void)hipFree(B.ad_eslot);
+    (void)hipFree(B.raster);
+    if (B.ad_host) (void)hipHostFree(B.ad_host);
     if (B.own_stream) (void)hipStreamDestroy(B.own_stream);
 }
 
@@ -286,16 +294,18 @@ int rebuild_chunks(Batch &B) {
     B.nchunks = off;
     if (B.nchunks >= (int64_t(1) << 31))
         return fail(SWE_ERR_ARG, "too many chunks (%lld)", (long long)B.nchunks);
-    if (B.nchunks > B.part_cap) {
+    if (B.nchunks > B.part_cap) {  // with room to grow: remeshing changes the count
+        const int64_t cap = B.nchunks + B.nchunks / 4;
+        if (B.stream) HIP_TRY(hipStreamSynchronize(B.stream));
         (void)hipFree(B.d_part_min);
         (void)hipFree(B.d_part_nref);
         B.d_part_min = nullptr;
         B.d_part_nref = nullptr;
         B.part_cap = 0;
-        if (hipMalloc(&B.d_part_min, sizeof(double) * (size_t)B.nchunks) != hipSuccess ||
-            hipMalloc(&B.d_part_nref, sizeof(int32_t) * (size_t)B.nchunks) != hipSuccess)
+        if (hipMalloc(&B.d_part_min, sizeof(double) * (size_t)cap) != hipSuccess ||
+            hipMalloc(&B.d_part_nref, sizeof(int32_t) * (size_t)cap) != hipSuccess)
             return fail(SWE_ERR_NOMEM, "chunk partials allocation failed");
-        B.part_cap = B.nchunks;
+        B.part_cap = cap;
     }
     (void)hipFree(B.d_chunk_scen);
     B.d_chunk_scen = nullptr;
@@ -309,8 +319,8 @@ int rebuild_chunks(Batch &B) {
         HIP_TRY(hipMemcpy(B.d_chunk_scen, map.data(), sizeof(int32_t) * map.size(),
                           hipMemcpyHostToDevice));
     }
-    HIP_TRY(hipMemcpy(B.d_scen, B.scen.data(), sizeof(swe::Scen) * B.scen.size(),
-                      hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(B.d_scen, B.scen.data(), sizeof(swe::Scen) * B.scen.size(),
+                           hipMemcpyHostToDevice, B.stream));
     if (B.graph.exec) {
         (void)hipGraphExecDestroy(B.graph.exec);
         B.graph = GraphCache{};
@@ -502,7 +512,9 @@ int batch_step(Batch &B, const double *dt, int forward, double *dt_cfl, int64_t 
 // n_steps traversals with the dt chain on the device, per scenario.
 int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double dt_clamp,
               double *time, double *dt_next) {
-    if (n_steps < 0 || !dt0 || !time) return fail(SWE_ERR_ARG, "run: invalid argument");
+    // dt0 == NULL: continue the device dt chain where the last run left it;
+    // time == dt_next == NULL: do not wait for the steps
+    if (n_steps < 0 || (dt0 && !time)) return fail(SWE_ERR_ARG, "run: invalid argument");
     if (!all_uploaded(B)) return fail(SWE_ERR_STATE, "run: not every scenario has a state");
     bool table_changed = false;
     for (auto &s : B.scen)
@@ -515,13 +527,15 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
                                hipMemcpyHostToDevice, B.stream));
     // the chain starts from dt0 (clamped as pre_traversal_grid_op does) and the
     // caller's time
-    std::vector<swe::Slots> init((size_t)B.nscen);
-    for (int32_t s = 0; s < B.nscen; ++s) {
-        init[s].dt = (dt_clamp > 0.0 && dt_clamp < dt0[s]) ? dt_clamp : dt0[s];
-        init[s].time = time[s];
+    if (dt0) {
+        std::vector<swe::Slots> init((size_t)B.nscen);
+        for (int32_t s = 0; s < B.nscen; ++s) {
+            init[s].dt = (dt_clamp > 0.0 && dt_clamp < dt0[s]) ? dt_clamp : dt0[s];
+            init[s].time = time[s];
+        }
+        HIP_TRY(hipMemcpyAsync(B.d_slots, init.data(), sizeof(swe::Slots) * init.size(),
+                               hipMemcpyHostToDevice, B.stream));
     }
-    HIP_TRY(hipMemcpyAsync(B.d_slots, init.data(), sizeof(swe::Slots) * init.size(),
-                           hipMemcpyHostToDevice, B.stream));
 
     auto one_step = [&](hipStream_t st, int from, int fwd) -> hipError_t {
         swe::StepArgs a = make_args(B, from, 1 - from, fwd);
@@ -564,15 +578,16 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
         HIP_TRY(one_step(B.stream, B.cur, fwd));
         B.cur = 1 - B.cur;
     }
+    if (n_steps > 0) B.flags_valid = B.keep_flags != 0;
+    if (!time && !dt_next) return SWE_OK;
     std::vector<swe::Slots> res((size_t)B.nscen);
     HIP_TRY(hipMemcpyAsync(res.data(), B.d_slots, sizeof(swe::Slots) * res.size(),
                            hipMemcpyDeviceToHost, B.stream));
     HIP_TRY(hipStreamSynchronize(B.stream));
     for (int32_t s = 0; s < B.nscen; ++s) {
-        time[s] = res[s].time;
+        if (time) time[s] = res[s].time;
         if (dt_next) dt_next[s] = res[s].dt;
     }
-    if (n_steps > 0) B.flags_valid = B.keep_flags != 0;
     return SWE_OK;
 }
 
@@ -586,9 +601,11 @@ int dev_alloc(T **p, int64_t count) {
     return SWE_OK;
 }
 
-// Scratch of the remesher for n cells.
+// Scratch of the remesher for n cells, with room to grow.
 int ensure_adapt_scratch(Batch &B, int64_t n) {
     if (n <= B.ad_cap) return SWE_OK;
+    const int64_t cap = n + n / 4;
+    HIP_TRY(hipStreamSynchronize(B.stream));  // queued work may still use the old scratch
     (void)hipFree(B.ad_slots);
     (void)hipFree(B.ad_status);
     (void)hipFree(B.ad_off);
@@ -601,14 +618,15 @@ int ensure_adapt_scratch(Batch &B, int64_t n) {
     B.ad_code = nullptr;
     B.ad_eslot = nullptr;
     B.ad_cap = 0;
-    SWE_TRY(dev_alloc(&B.ad_slots, n));
-    SWE_TRY(dev_alloc(&B.ad_status, 2));
-    SWE_TRY(dev_alloc(&B.ad_off, n));
-    SWE_TRY(dev_alloc(&B.ad_base, n));
-    SWE_TRY(dev_alloc(&B.ad_bsum, swe::adapt_scan_tiles(n) + 1));
-    SWE_TRY(dev_alloc(&B.ad_code, n));
-    SWE_TRY(dev_alloc(&B.ad_eslot, 6 * n));
-    B.ad_cap = n;
+    B.ad_off_valid = false;
+    SWE_TRY(dev_alloc(&B.ad_slots, cap));
+    SWE_TRY(dev_alloc(&B.ad_status, 3));
+    SWE_TRY(dev_alloc(&B.ad_off, cap));
+    SWE_TRY(dev_alloc(&B.ad_base, cap));
+    SWE_TRY(dev_alloc(&B.ad_bsum, swe::adapt_scan_tiles(cap) + 1 + 8));
+    SWE_TRY(dev_alloc(&B.ad_code, cap));
+    SWE_TRY(dev_alloc(&B.ad_eslot, 6 * cap));
+    B.ad_cap = cap;
     return SWE_OK;
 }
 
@@ -627,6 +645,10 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
                     "flags with swe_keep_refinement_flags)");
     }
     SWE_TRY(ensure_adapt_scratch(B, n));
+    if (!B.ad_host && hipHostMalloc(&B.ad_host, 8 * sizeof(int64_t)) != hipSuccess) {
+        B.ad_host = nullptr;
+        return fail(SWE_ERR_NOMEM, "adapt: pinned host allocation failed");
+    }
     const swe::Scen &sc = B.scen[0];
     swe::AdaptArgs a{};
     a.n = n;
@@ -647,20 +669,39 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.code = B.ad_code;
     a.eslot = B.ad_eslot;
     a.status = B.ad_status;
-    int sweeps = 0;
-    HIP_TRY(swe::adapt_conformity(a, B.stream, 1 << 16, &sweeps));
+    a.raster = B.raster;
+    a.rnx = B.rnx;
+    a.rny = B.rny;
+    a.rx0 = B.rx0;
+    a.ry0 = B.ry0;
+    a.rdx = B.rdx;
+    a.rdy = B.rdy;
+    a.scaling = sc.scaling;
+    a.offset = B.roffset;
+    swe::AdaptPlan plan{};
+    HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, 1 << 16, B.ad_host,
+                            &plan));
+    const int sweeps = plan.sweeps;
     if (sweeps < 0) return fail(SWE_ERR_STATE, "adapt: the conformity sweeps did not converge");
-    int64_t n_new = 0, tree = 0;
-    uint32_t status = 0;
-    HIP_TRY(swe::adapt_count(a, B.stream, &n_new, &tree, &status));
+    B.ad_last_sweeps = sweeps;
+    const int64_t n_new = plan.n_new, tree = plan.tree_total;
+    const uint32_t status = plan.status & 1u;
     if (tree != (int64_t(1) << 61))
         return fail(SWE_ERR_ARG,
                     "adapt: the mesh is not a leaf sequence of the two root triangles (its depths "
                     "cover %.17g of the unit square)", std::ldexp((double)tree, -61));
     if (status) return fail(SWE_ERR_STATE, "adapt: inconsistent coarsening (status %u)", status);
+    B.ad_off_valid = true;
     if (n_new <= 0 || n_new > (int64_t(1) << 29))
         return fail(SWE_ERR_ARG, "adapt: the new mesh would have %lld cells (1..2^29)",
                     (long long)n_new);
+    if (!(plan.status & 2u)) {  // every cell keeps its place: the mesh and state stand
+        B.flags_valid = false;
+        if (n_out) *n_out = n;
+        if (sweeps_out) *sweeps_out = sweeps;
+        return SWE_OK;
+    }
+    B.ad_off_valid = false;
 
     // outputs: the spare set when the new mesh fits, else fresh buffers with
     // room to grow (everything is allocated before anything is freed)
@@ -717,12 +758,13 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.plotter_out = fresh ? po : B.plotter2;
     a.depth_out = fresh ? dpo : B.depth2;
     hipError_t e = swe::adapt_emit(a, B.stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(B.stream);
     if (e != hipSuccess) {
         if (fresh) drop();
         return fail(SWE_ERR_HIP, "adapt: %s", hipGetErrorString(e));
     }
     if (fresh) {
+        // the emit kernels read the old buffers
+        HIP_TRY(hipStreamSynchronize(B.stream));
         for (int i = 0; i < 2; ++i) {
             (void)hipFree(B.eta[i]);
             (void)hipFree(B.hu[i]);
@@ -912,7 +954,8 @@ int swe_step(swe_handle *h, double dt, int forward, double *dt_cfl, int64_t *n_r
 int swe_run(swe_handle *h, int64_t n_steps, double dt0, int forward, double dt_clamp,
             double *time, double *dt_next) {
     if (!h) return fail(SWE_ERR_ARG, "swe_run: null handle");
-    return batch_run(h->B, n_steps, &dt0, forward, dt_clamp, time, dt_next);
+    if (dt0 > 0.0 && !time) return fail(SWE_ERR_ARG, "swe_run: time is required with dt0 > 0");
+    return batch_run(h->B, n_steps, dt0 > 0.0 ? &dt0 : nullptr, forward, dt_clamp, time, dt_next);
 }
 
 int64_t swe_cells(swe_handle *h) { return h ? h->B.ncells : -1; }
@@ -926,6 +969,53 @@ int swe_keep_refinement_flags(swe_handle *h, int on) {
 int swe_adapt(swe_handle *h, const int8_t *refine, int64_t *n_cells, int *sweeps) {
     if (!h) return fail(SWE_ERR_ARG, "swe_adapt: null handle");
     return batch_adapt(h->B, refine, n_cells, sweeps);
+}
+
+int swe_set_bathymetry_raster(swe_handle *h, int64_t nx, int64_t ny, double x0, double y0,
+                              double dx, double dy, double offset, const double *values) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_set_bathymetry_raster: null handle");
+    Batch &B = h->B;
+    if (!values) {
+        (void)hipFree(B.raster);
+        B.raster = nullptr;
+        return SWE_OK;
+    }
+    if (nx < 2 || ny < 2 || nx > (int64_t(1) << 20) || ny > (int64_t(1) << 20) ||
+        !(dx > 0.0) || !(dy > 0.0) || !std::isfinite(x0) || !std::isfinite(y0) ||
+        !std::isfinite(offset))
+        return fail(SWE_ERR_ARG,
+                    "swe_set_bathymetry_raster: need 2 <= nx, ny <= 2^20 and dx, dy > 0");
+    double *d = nullptr;
+    SWE_TRY(dev_alloc(&d, nx * ny));
+    hipError_t e = hipMemcpy(d, values, sizeof(double) * (size_t)(nx * ny), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(SWE_ERR_HIP, "swe_set_bathymetry_raster: %s", hipGetErrorString(e));
+    }
+    (void)hipFree(B.raster);
+    B.raster = d;
+    B.rnx = nx;
+    B.rny = ny;
+    B.rx0 = x0;
+    B.ry0 = y0;
+    B.rdx = dx;
+    B.rdy = dy;
+    B.roffset = offset;
+    return SWE_OK;
+}
+
+int swe_flag_eta_jumps(swe_handle *h, double refine_tol, double coarsen_tol) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_flag_eta_jumps: null handle");
+    Batch &B = h->B;
+    if (B.nscen != 1) return fail(SWE_ERR_ARG, "swe_flag_eta_jumps: single-mesh handles only");
+    if (!all_uploaded(B)) return fail(SWE_ERR_STATE, "swe_flag_eta_jumps: no state uploaded");
+    if (!(refine_tol >= 0.0) || !(coarsen_tol >= 0.0))
+        return fail(SWE_ERR_ARG, "swe_flag_eta_jumps: tolerances must be >= 0");
+    const swe::Scen &sc = B.scen[0];
+    HIP_TRY(swe::flag_eta_jumps(B.ncells, B.eta[B.cur], B.b, B.nbr, B.depth, sc.drytol,
+                                refine_tol, coarsen_tol, sc.dmin, sc.dmax, B.refine, B.stream));
+    B.flags_valid = true;
+    return SWE_OK;
 }
 
 int swe_download_mesh(swe_handle *h, int8_t *depth, int8_t *plotter, int32_t *nbr) {

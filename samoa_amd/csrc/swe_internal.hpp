@@ -93,24 +93,41 @@ struct AdaptArgs {
     uint32_t *slots;                       // [n] conformity words
     int64_t *off;                          // [n] tree offsets
     int64_t *base;                         // [n] new-cell counts, then their prefix
-    int64_t *bsum;                         // [adapt_scan_tiles(n) + 1]
+    int64_t *bsum;                         // [adapt_scan_tiles(n) + 1 + 8]
     int8_t *code;                          // [n]
     int32_t *eslot;                        // [6n] new cell-edge on each half of each old edge
-    uint32_t *status;                      // [2]
+    uint32_t *status;                      // [3]
     int8_t *depth_out, *plotter_out;       // [n'] new mesh
     int32_t *nbr_out;                      // [3n']
     double *eta_out, *hu_out, *hv_out, *b_out;
+    // bathymetry of split cells: bilinear in a raster of nx x ny values at
+    // world points (x0 + i dx, y0 + j dy), values[j nx + i], sampled at the
+    // child's centroid scaling * c + offset; null: the parent's b
+    const double *raster;
+    int64_t rnx, rny;
+    double rx0, ry0, rdx, rdy, scaling, offset;
 };
 
+// bsum holds adapt_scan_tiles(n) + 1 scan words and 8 result words.
 int64_t adapt_scan_tiles(int64_t n);
-// Marks the edges and sweeps until no word changes (at most max_sweeps; then
-// *sweeps = -1).  Synchronises the stream after every sweep.
-hipError_t adapt_conformity(const AdaptArgs &a, hipStream_t s, int max_sweeps, int *sweeps);
-// New-cell counts and their prefix; returns the new cell count, the covered
-// part of the two root trees (2^61 when the mesh is whole) and the status
-// (bit 0: an inconsistent merge).  Synchronises the stream.
-hipError_t adapt_count(const AdaptArgs &a, hipStream_t s, int64_t *n_new, int64_t *tree_total,
-                       uint32_t *status);
+struct AdaptPlan {
+    int sweeps;          // conformity sweeps to the fixpoint, -1 if not reached in max_sweeps
+    int64_t n_new;       // cells of the new mesh
+    int64_t tree_total;  // covered part of the two root trees (2^61 when the mesh is whole)
+    uint32_t status;     // bit 0: an inconsistent merge; bit 1: the mesh changes
+};
+// Marks the edges, sweeps to the conformity fixpoint (first_batch sweeps
+// between host checks, then 4), then counts the new cells and their prefix
+// (base; its high word counts the cells that change) unless no cell splits
+// or merges.  offsets_valid: a.off still holds this mesh's tree offsets.
+// host: 8 words of pinned memory for the results.  Synchronises the stream
+// once, or twice when the mesh changes.
+hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
+                      int max_sweeps, int64_t *host, AdaptPlan *plan);
+// Refinement flags from surface-elevation jumps to wet neighbours.
+hipError_t flag_eta_jumps(int64_t n, const double *eta, const double *b, const int32_t *nbr,
+                          const int8_t *depth, double drytol, double refine_tol,
+                          double coarsen_tol, int dmin, int dmax, int8_t *flags, hipStream_t s);
 // Writes the new mesh and state (outputs sized for n_new).
 hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s);
 

@@ -174,6 +174,18 @@ contains
         depth = int(d, c_int)
     end subroutine
 
+    !> The tsunami scenario's bathymetry raster (tsunami_raster): ng x ng
+    !> values, x fastest, node spacing dx from the origin.  Returns ng.
+    integer(c_int) function swe_host_tsunami_raster(seed, values, dx) &
+            bind(c, name='swe_host_tsunami_raster')
+        integer(c_int64_t), value   :: seed
+        real(c_double), intent(out) :: values(TSU_G, TSU_G)
+        real(c_double), intent(out) :: dx
+        call tsunami_raster(seed, values)
+        dx = TSU_DX
+        swe_host_tsunami_raster = TSU_G
+    end function
+
     subroutine swe_host_tsunami(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b) &
             bind(c, name='swe_host_tsunami')
         integer(c_int64_t), value   :: n, seed

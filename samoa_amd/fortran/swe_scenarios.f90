@@ -22,11 +22,13 @@ module swe_scenarios
     implicit none
     private
     public :: rng_t, rng_seed, rng_uniform, rng_normal, tsunami_draw, tsunami_state, &
-              grf_grid, centroid
+              grf_grid, centroid, tsunami_raster, raster_sample, TSU_G, TSU_DX, TSU_SCALING
 
     integer(c_int64_t), parameter :: M1 = 4294967087_c_int64_t, M2 = 4294944443_c_int64_t
     real(c_double), parameter :: PI = 3.14159265358979323846_c_double
     real(c_double), parameter :: TSU_SCALING = 1.0e6_c_double
+    integer, parameter :: TSU_G = 513
+    real(c_double), parameter :: TSU_DX = TSU_SCALING / real(TSU_G - 1, c_double)
 
     type rng_t
         integer(c_int64_t) :: s1(3), s2(3)
@@ -128,35 +130,62 @@ contains
         depth = dlo + min(dhi - dlo, int(real(dhi - dlo + 1, c_double) * rng_uniform(r)))
     end subroutine
 
-    !> Tsunami state at the centroids of n cells (unit-square coordinates).
+    !> Bathymetry raster of the tsunami scenario: TSU_G x TSU_G nodes over
+    !> [0, TSU_SCALING]^2 (spacing TSU_SCALING / (TSU_G - 1), x fastest):
+    !> -4000 m plus the Gaussian random field, blended over the last 150 km of
+    !> x towards a +200 m shelf.
+    subroutine tsunami_raster(seed, grid)
+        integer(c_int64_t), intent(in) :: seed
+        real(c_double), intent(out)    :: grid(TSU_G, TSU_G)
+        type(rng_t) :: r
+        real(c_double) :: x, w, shelf0
+        integer :: i
+        call rng_seed(r, 7919_c_int64_t * seed + 3_c_int64_t)
+        call grf_grid(r, TSU_G, TSU_SCALING, 800.0_c_double, 5.0e4_c_double, 256, grid)
+        shelf0 = TSU_SCALING - 1.5e5_c_double
+        do i = 1, TSU_G
+            x = real(i - 1, c_double) * TSU_DX
+            w = min(max((x - shelf0) / 1.5e5_c_double, 0.0_c_double), 1.0_c_double)
+            grid(i, :) = (1.0_c_double - w) * (-4000.0_c_double + grid(i, :)) + w * 200.0_c_double
+        end do
+    end subroutine
+
+    !> Bilinear sample of a raster of nx x ny nodes (x0 + (i-1) dx, y0 + (j-1) dy)
+    !> at the world point p, clamped to the last raster cell.  The device's
+    !> remesher evaluates the same expression for the cells it splits.
+    pure real(c_double) function raster_sample(nx, ny, grid, x0, y0, dx, dy, p) result(v)
+        integer, intent(in)        :: nx, ny
+        real(c_double), intent(in) :: grid(nx, ny), x0, y0, dx, dy, p(2)
+        real(c_double) :: fx, fy, wx, wy
+        integer :: ix, iy
+        fx = (p(1) - x0) / dx
+        fy = (p(2) - y0) / dy
+        ix = min(max(int(fx), 0), nx - 2)
+        iy = min(max(int(fy), 0), ny - 2)
+        wx = fx - real(ix, c_double)
+        wy = fy - real(iy, c_double)
+        v = (1.0_c_double - wx) * (1.0_c_double - wy) * grid(ix + 1, iy + 1) &
+            + wx * (1.0_c_double - wy) * grid(ix + 2, iy + 1) &
+            + (1.0_c_double - wx) * wy * grid(ix + 1, iy + 2) + wx * wy * grid(ix + 2, iy + 2)
+    end function
+
+    !> Tsunami state at the centroids of n cells (unit-square coordinates):
+    !> b from the raster, eta the elliptical Gaussian source (dry where it
+    !> lies below b + drytol).
     subroutine tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
         integer(c_int64_t), intent(in) :: n, seed
         real(c_double), intent(in)     :: coords(2, 3, n), amp, ratio, x0, y0, drytol
         real(c_double), intent(out)    :: eta(n), hu(n), hv(n), b(n)
-        integer, parameter :: G = 513, NFEAT = 256
         real(c_double), allocatable :: grid(:, :)
-        type(rng_t) :: r
-        real(c_double) :: x(2), fx, fy, wx, wy, w, shelf0, a, c
+        real(c_double) :: x(2), a, c
         integer(c_int64_t) :: i
-        integer :: ix, iy
-        allocate(grid(G, G))
-        call rng_seed(r, 7919_c_int64_t * seed + 3_c_int64_t)
-        call grf_grid(r, G, TSU_SCALING, 800.0_c_double, 5.0e4_c_double, NFEAT, grid)
+        allocate(grid(TSU_G, TSU_G))
+        call tsunami_raster(seed, grid)
         a = 1.0e5_c_double
         c = a / ratio
-        shelf0 = TSU_SCALING - 1.5e5_c_double
         do i = 1, n
-            x = TSU_SCALING * centroid(coords(:, :, i))
-            fx = x(1) / TSU_SCALING * real(G - 1, c_double)
-            fy = x(2) / TSU_SCALING * real(G - 1, c_double)
-            ix = min(max(int(fx), 0), G - 2)
-            iy = min(max(int(fy), 0), G - 2)
-            wx = fx - real(ix, c_double)
-            wy = fy - real(iy, c_double)
-            b(i) = -4000.0_c_double + (1 - wx) * (1 - wy) * grid(ix + 1, iy + 1) + wx * (1 - wy) * grid(ix + 2, iy + 1) &
-                   + (1 - wx) * wy * grid(ix + 1, iy + 2) + wx * wy * grid(ix + 2, iy + 2)
-            w = min(max((x(1) - shelf0) / 1.5e5_c_double, 0.0_c_double), 1.0_c_double)
-            b(i) = (1.0_c_double - w) * b(i) + w * 200.0_c_double
+            x = TSU_SCALING * centroid(coords(:, :, i)) + 0.0_c_double
+            b(i) = raster_sample(TSU_G, TSU_G, grid, 0.0_c_double, 0.0_c_double, TSU_DX, TSU_DX, x)
             eta(i) = amp * exp(-(((x(1) - x0) / a) ** 2 + ((x(2) - y0) / c) ** 2))
             hu(i) = 0.0_c_double
             hv(i) = 0.0_c_double

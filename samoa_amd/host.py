@@ -58,6 +58,8 @@ def load():
                                              C.c_double, C.c_double, C.c_double, _dp]
         L.swe_host_gauges_write.restype = C.c_int
         L.swe_host_gauges_write.argtypes = [C.c_char_p, C.c_int, _dp, _dp, _dp, C.c_int]
+        L.swe_host_tsunami_raster.restype = C.c_int
+        L.swe_host_tsunami_raster.argtypes = [C.c_int64, _dp, C.POINTER(C.c_double)]
         L.swe_host_mesh_from_depths.restype = C.c_int
         L.swe_host_mesh_from_depths.argtypes = [C.c_int64, _i8p, _i8p, _dp]
         _lib = L
@@ -92,6 +94,18 @@ def neighbours_from_corners(coords: np.ndarray) -> np.ndarray:
     nbr[s0] = s1 // 3
     nbr[s1] = s0 // 3
     return nbr.reshape(n, 3)
+
+
+def coords_from_depths(depth) -> tuple:
+    """(plotter[n], corners[n, 3, 2]) of the mesh with these depths in
+    forward curve order (fortran/swe_sfc_mesh.f90 sfc_mesh_from_depths)."""
+    depth = np.ascontiguousarray(depth, np.int8)
+    n = depth.shape[0]
+    plotter = np.empty(n, np.int8)
+    coords = np.empty(6 * n)
+    if load().swe_host_mesh_from_depths(n, depth, plotter, coords) != 0:
+        raise ValueError("depth sequence is not the leaf sequence of the two root triangles")
+    return plotter, coords.reshape(n, 3, 2)
 
 
 def mesh_from_depths(depth) -> Mesh:
@@ -144,6 +158,22 @@ def tsunami(mesh: Mesh, seed: int, drytol: float = 1e-3, amplitude=None, ratio=N
     load().swe_host_tsunami(mesh.n, np.ascontiguousarray(mesh.coords.reshape(-1)), seed, amp, rat,
                             xx, yy, drytol, *out)
     return tuple(out)
+
+
+TSUNAMI_RASTER_NODES = 513
+
+
+def tsunami_raster(seed: int) -> dict:
+    """The tsunami scenario's bathymetry raster (fortran/swe_scenarios.f90
+    tsunami_raster): values[j, i] at world (i dx, j dx), for
+    EulerTimestep.set_bathymetry_raster, which samples it exactly as
+    tsunami() does for the cells that remeshing splits."""
+    g = TSUNAMI_RASTER_NODES
+    values = np.empty(g * g)
+    dx = C.c_double()
+    got = load().swe_host_tsunami_raster(int(seed), values, C.byref(dx))
+    assert got == g
+    return dict(values=values.reshape(g, g), x0=0.0, y0=0.0, dx=dx.value, dy=dx.value, offset=0.0)
 
 
 def initial_dt(mesh: Mesh, state, courant: float, g: float = G) -> float:

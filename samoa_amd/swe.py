@@ -112,6 +112,12 @@ class EulerTimestep:
         _native.check("swe_keep_refinement_flags",
                       self._lib.swe_keep_refinement_flags(self._h, int(bool(on))))
 
+    def flag_eta_jumps(self, refine_tol: float, coarsen_tol: float) -> None:
+        """Device refinement flags from surface-elevation jumps to wet
+        neighbours (swe_flag_eta_jumps), for adapt() with refine=None."""
+        _native.check("swe_flag_eta_jumps",
+                      self._lib.swe_flag_eta_jumps(self._h, float(refine_tol), float(coarsen_tol)))
+
     def adapt(self, refine=None):
         """One remeshing step (swe_adapt): conformity of the requests (the
         last step's flags on the device when `refine` is None), split and
@@ -129,6 +135,20 @@ class EulerTimestep:
         self.mesh = None
         return self.n, sweeps.value
 
+    def set_bathymetry_raster(self, values, x0: float, y0: float, dx: float, dy: float,
+                              offset: float) -> None:
+        """Cells that adapt() splits take b from this raster (values[ny, nx]
+        over the world points (x0 + i dx, y0 + j dy)), bilinear at their
+        centroid; None: they copy the parent's b."""
+        if values is None:
+            _native.check("swe_set_bathymetry_raster", self._lib.swe_set_bathymetry_raster(
+                self._h, 0, 0, 0.0, 0.0, 1.0, 1.0, 0.0, None))
+            return
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        ny, nx = v.shape
+        _native.check("swe_set_bathymetry_raster", self._lib.swe_set_bathymetry_raster(
+            self._h, nx, ny, float(x0), float(y0), float(dx), float(dy), float(offset), _ptr(v)))
+
     def download_mesh(self):
         """(depth[n], plotter[n], nbr[n, 3]) of the current mesh."""
         depth, plotter = np.empty(self.n, np.int8), np.empty(self.n, np.int8)
@@ -137,17 +157,18 @@ class EulerTimestep:
             self._h, _ptr(depth), _ptr(plotter), _ptr(nbr)))
         return depth, plotter, nbr.reshape(self.n, 3)
 
-    def run(self, n_steps: int, dt0: float, forward: bool, dt_clamp: float = -1.0,
-            time: float = 0.0):
+    def run(self, n_steps: int, dt0, forward: bool, dt_clamp: float = -1.0,
+            time: float = 0.0, wait: bool = True):
         """n_steps traversals with the dt chain on the device, starting at
         simulated time `time`; returns (time after the steps, dt of the next
-        step)."""
+        step).  dt0=None continues the device chain of the last run;
+        wait=False returns None at once, leaving time and dt on the device."""
         t = C.c_double(time)
         dtn = C.c_double()
-        _native.check("swe_run", self._lib.swe_run(self._h, int(n_steps), float(dt0),
-                                                   int(bool(forward)), float(dt_clamp),
-                                                   C.byref(t), C.byref(dtn)))
-        return t.value, dtn.value
+        _native.check("swe_run", self._lib.swe_run(
+            self._h, int(n_steps), -1.0 if dt0 is None else float(dt0), int(bool(forward)),
+            float(dt_clamp), C.byref(t) if wait else None, C.byref(dtn) if wait else None))
+        return (t.value, dtn.value) if wait else None
 
     def step_device(self, eta, hu, hv, b, eta_out, hu_out, hv_out, dt, forward, dt_cfl,
                     n_refine=0, refine=0, stream=0) -> None:

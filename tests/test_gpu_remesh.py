@@ -139,3 +139,74 @@ def test_adapt_errors():
         op.adapt(np.zeros(n, np.int8))
     assert e.value.code == _native.SWE_ERR_ARG and "leaf sequence" in str(e.value)
     op.close()
+
+
+def test_split_cells_sample_the_bathymetry_raster():
+    """With the tsunami scenario's raster set, the cells a refinement creates
+    take b from it at their centroids, bit-identical to the host's scenario
+    code on the finer mesh (the same bilinear expression), while a merge
+    still averages its pair."""
+    d = 10
+    mesh = host.mesh_uniform(d)
+    st = host.tsunami(mesh, seed=3, drytol=1e-3)
+    op = swe.EulerTimestep(mesh, scaling=1e6, dry_tolerance=1e-3, min_depth=d - 1,
+                           max_depth=d + 1)
+    op.upload(*st)
+    r = host.tsunami_raster(3)
+    op.set_bathymetry_raster(r["values"], r["x0"], r["y0"], r["dx"], r["dy"], r["offset"])
+    op.adapt(np.ones(mesh.n, np.int8))
+    fine = host.tsunami(host.mesh_uniform(d + 1), seed=3, drytol=1e-3)
+    b = op.download()[3]
+    assert np.array_equal(b, fine[3])
+    op.adapt(np.full(op.n, -1, np.int8))
+    assert np.array_equal(op.download()[3], 0.5 * (fine[3][0::2] + fine[3][1::2]))
+    op.close()
+
+
+def test_eta_jump_flags_drive_adapt():
+    """swe_flag_eta_jumps against its definition restated in numpy, through
+    the remeshing it drives (compared with the oracle's remeshing)."""
+    mesh = host.mesh_uniform(9)
+    st = host.tsunami(mesh, seed=1, drytol=1e-3)
+    eta, b = st[0], st[3]
+    wet = eta - b > 1e-3
+    j = np.zeros(mesh.n)
+    for k in range(3):
+        o = mesh.nbr[:, k]
+        ok = (o >= 0) & wet & wet[np.maximum(o, 0)]
+        j = np.where(ok, np.maximum(j, np.abs(eta - eta[np.maximum(o, 0)])), j)
+    tol = np.quantile(j[j > 0], 0.8)
+    flags = np.where((mesh.depth < 12) & (j > tol), 1,
+                     np.where((mesh.depth > 5) & (j < tol / 10), -1, 0)).astype(np.int8)
+    assert (flags == 1).any() and (flags == -1).any()
+    d2, p2, s2, nb2, _, _ = po.adapt(mesh.depth, mesh.plotter, mesh.nbr, flags, st, dmin=5,
+                                     dmax=12, coords=mesh.coords)
+    op = swe.EulerTimestep(mesh, scaling=1e6, dry_tolerance=1e-3, min_depth=5, max_depth=12)
+    op.upload(*st)
+    op.flag_eta_jumps(tol, tol / 10)
+    n, _ = op.adapt()
+    assert n == len(d2)
+    d, p, nb = op.download_mesh()
+    assert np.array_equal(d, d2) and np.array_equal(nb, nb2)
+    assert np.array_equal(np.stack(op.download()), s2)
+    op.close()
+
+
+def test_run_continues_the_device_chain():
+    """swe_run with dt0 <= 0 continues the device dt chain, and without time
+    and dt_next it does not wait: 7 steps in pieces equal 7 steps at once."""
+    mesh = host.mesh_uniform(8)
+    st = host.tsunami(mesh, seed=2, drytol=1e-3)
+    dt0 = host.initial_dt(mesh, st, 0.45)
+    ops = [swe.EulerTimestep(mesh, scaling=1e6, dry_tolerance=1e-3) for _ in range(2)]
+    for op in ops:
+        op.upload(*st)
+    t_all, dt_all = ops[0].run(7, dt0, False)
+    ops[1].run(2, dt0, False)  # backward, forward
+    ops[1].run(1, None, False, wait=False)
+    ops[1].run(4, None, True, wait=False)
+    t, dt = ops[1].run(0, None, True)
+    assert (t, dt) == (t_all, dt_all)
+    assert np.array_equal(np.stack(ops[0].download()), np.stack(ops[1].download()))
+    for op in ops:
+        op.close()
