@@ -3,9 +3,10 @@
  *
  * A drop-in for the per-step edge/cell traversal of sam(oa)^2's SWE scenario:
  * one call replaces `swe%euler%traverse(grid)` (SWE.f90:393), i.e. the
- * operators of SWE_euler_timestep.f90:36-54 run over every cell and edge of a
- * frozen mesh (flux = GeoClaw augmented Riemann solver on every edge, cell
- * update with the wet/dry limiter, CFL reduction).  Plain pointers and sizes
+ * operators of SWE_euler_timestep.f90:36-54 run over every cell and edge of
+ * the mesh (flux = GeoClaw augmented Riemann solver on every edge, cell
+ * update with the wet/dry limiter, CFL reduction).  Between steps, swe_adapt
+ * replaces the adaption traversal (SWE.f90:387-390) on the device.  Plain pointers and sizes
  * only; every entry point returns SWE_OK (0) or a negative status and never
  * aborts.  The text of the last error of the calling thread is available from
  * swe_last_error().
