@@ -13,16 +13,17 @@
 //      parent's size.  off is an exclusive scan of the cell sizes.
 //   3. counts of new cells (split 2..4, merge 1 for the first child of a
 //      pair and 0 for the second, else 1), an exclusive scan of them.
-//   4. emit: depth, plotter type and state of every new cell (a split copies
-//      eta, hu, hv and b, or samples b from a raster at the child's centroid
-//      as get_bathymetry_at_element does from its data; a merge averages all
-//      four, SWE_adapt.f90:95-169),
-//      the neighbours inside each old cell, and for every old cell-edge the
-//      new cell-edge that lies on each half of it.
-//   5. link: a new edge on half s of old edge k of cell A meets the new edge on
-//      the same half of the same old edge in A's neighbour B; the same end of
-//      the shared edge is half s in B when A and B run in opposite senses (the
-//      plotter types differ in sign) and half 1-s when they run in the same.
+//   4. emit: depth, plotter type, tree offset and state of every new cell (a
+//      split copies eta, hu, hv and b, or samples b from a raster at the
+//      child's centroid as get_bathymetry_at_element does from its data; a
+//      merge averages all four, SWE_adapt.f90:95-169); the neighbours inside
+//      each old cell, and between two cells that keep their place; for a
+//      changing cell, the new cell-edge on each half of each old edge.
+//   5. link, over the changing cells only: a new edge on half s of old edge k
+//      of cell A meets the new edge on the same half of the same old edge in
+//      A's neighbour B; the same end of the shared edge is half s in B when A
+//      and B run in opposite senses (the plotter types differ in sign) and
+//      half 1-s when they run in the same.  Both sides are written.
 //
 // Cell corners are (entry, right angle, exit); a cell (e, r, x) splits into
 // (e, m, r) and (r, m, x), m the midpoint of the hypotenuse, with the plotter
@@ -339,6 +340,13 @@ __device__ __forceinline__ void put_slot(int32_t *eslot, int64_t c, int k, int h
     }
 }
 
+// The new cell-edge on half s of old edge k of cell c: computed for a cell
+// that keeps its place, read from the slots emit_kernel wrote otherwise.
+__device__ __forceinline__ int32_t edge_slot(const AdaptArgs &a, int64_t c, int k, int s) {
+    if (a.code[c] == CODE_KEEP) return (int32_t)(((a.base[c] & 0xffffffffLL) << 2) | k);
+    return a.eslot[6 * c + 2 * k + s];
+}
+
 __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
     const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
     if (c >= a.n) return;
@@ -349,14 +357,29 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
     if (k == CODE_KEEP) {
         a.depth_out[b0] = (int8_t)d;
         a.plotter_out[b0] = (int8_t)p;
+        a.off_out[b0] = a.off[c];
         a.eta_out[b0] = a.eta[c];
         a.hu_out[b0] = a.hu[c];
         a.hv_out[b0] = a.hv[c];
         a.b_out[b0] = a.b[c];
-        for (int e = 0; e < 3; ++e) put_slot(a.eslot, c, e, 2, (int32_t)((b0 << 2) | e));
-    } else if (k == CODE_MERGE_FIRST) {
+        // an edge to a cell that keeps its place too keeps its pairing; the
+        // others are written by link_kernel from the changing side
+        for (int e = 0; e < 3; ++e) {
+            const int32_t q = a.nbr[3 * c + e];
+            if (q < 0) {
+                a.nbr_out[3 * b0 + e] = -1;
+            } else if (a.code[q >> 2] == CODE_KEEP) {
+                a.nbr_out[3 * b0 + e] =
+                    (int32_t)(((a.base[q >> 2] & 0xffffffffLL) << 2) | (q & 3));
+            }
+        }
+        return;
+    }
+    a.list[a.base[c] >> 32] = (int32_t)c;  // the changing cells, in order
+    if (k == CODE_MERGE_FIRST) {
         a.depth_out[b0] = (int8_t)(d - 1);
         a.plotter_out[b0] = c_child[p + 8];  // the parent's type
+        a.off_out[b0] = a.off[c];
         a.eta_out[b0] = 0.5 * (a.eta[c] + a.eta[c + 1]);
         a.hu_out[b0] = 0.5 * (a.hu[c] + a.hu[c + 1]);
         a.hv_out[b0] = 0.5 * (a.hv[c] + a.hv[c + 1]);
@@ -381,6 +404,7 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
             const int dc = d + sc.dd;
             a.depth_out[idx] = (int8_t)dc;
             a.plotter_out[idx] = (int8_t)split_type(p, sc.rule);
+            a.off_out[idx] = coff;
             a.eta_out[idx] = a.eta[c];
             a.hu_out[idx] = a.hu[c];
             a.hv_out[idx] = a.hv[c];
@@ -404,9 +428,13 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
     }
 }
 
+// Over the changing cells: each new edge on an old edge meets the new edge on
+// the same half across it, and both sides are written (the other side may
+// keep its place and then leaves the pairing to this kernel).
 __global__ __launch_bounds__(AT) void link_kernel(AdaptArgs a) {
-    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
-    if (c >= a.n) return;
+    const int64_t i = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (i >= a.n_changing) return;
+    const int64_t c = a.list[i];
     const int pc = a.plotter[c];
     for (int k = 0; k < 3; ++k) {
         const int32_t x0 = a.eslot[6 * c + 2 * k], x1 = a.eslot[6 * c + 2 * k + 1];
@@ -420,7 +448,8 @@ __global__ __launch_bounds__(AT) void link_kernel(AdaptArgs a) {
                 const int64_t o = q >> 2;
                 const int ko = q & 3;
                 const int so = ((pc > 0) == (a.plotter[o] > 0)) ? 1 - s : s;
-                y = a.eslot[6 * o + 2 * ko + so];
+                y = edge_slot(a, o, ko, so);
+                if (y >= 0) a.nbr_out[3 * (int64_t)(y >> 2) + (y & 3)] = x;
             }
             a.nbr_out[3 * (int64_t)(x >> 2) + (x & 3)] = y;
         }
@@ -478,6 +507,7 @@ __global__ void results_kernel(int64_t *res, const int64_t *bsum, int64_t tiles,
     if (threadIdx.x == 0) {
         res[1] = bsum[tiles] & 0xffffffffLL;
         res[2] = status[0] | ((bsum[tiles] >> 32) != 0 ? 2u : 0u);
+        res[5] = bsum[tiles] >> 32;
     }
 }
 // res[3] = the last sweep that changed a word, res[4] = the last sweep that
@@ -532,6 +562,7 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
     plan->tree_total = offsets_valid ? (int64_t(1) << 61) : host[0];
     if (host[4] < done) {  // no cell splits or merges: the mesh stands
         plan->n_new = a.n;
+        plan->n_changing = 0;
         plan->status = 0;
         return hipSuccess;
     }
@@ -541,9 +572,10 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
     if ((e = scan_i64(a.base, a.base, a.n, a.bsum, s)) != hipSuccess) return e;
     results_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles, a.status);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = fetch(1, 2)) != hipSuccess) return e;
+    if ((e = fetch(1, 5)) != hipSuccess) return e;
     plan->n_new = host[1];
     plan->status = (uint32_t)host[2];
+    plan->n_changing = host[5];
     return hipSuccess;
 }
 
@@ -556,11 +588,10 @@ hipError_t flag_eta_jumps(int64_t n, const double *eta, const double *b, const i
 }
 
 hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s) {
-    const unsigned nb = blocks_for(a.n, AT);
-    emit_kernel<<<nb, AT, 0, s>>>(a);
+    emit_kernel<<<blocks_for(a.n, AT), AT, 0, s>>>(a);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    link_kernel<<<nb, AT, 0, s>>>(a);
+    if (e != hipSuccess || a.n_changing == 0) return e;
+    link_kernel<<<blocks_for(a.n_changing, AT), AT, 0, s>>>(a);
     return hipGetLastError();
 }
 

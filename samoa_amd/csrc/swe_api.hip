@@ -221,6 +221,8 @@ struct Batch {
     int64_t cap2 = 0;
     uint32_t *ad_slots = nullptr, *ad_status = nullptr;
     int64_t *ad_off = nullptr, *ad_base = nullptr, *ad_bsum = nullptr;
+    int64_t *ad_off2 = nullptr;  // the new mesh's tree offsets, swapped with ad_off
+    int64_t ad_off_cap = 0, ad_off2_cap = 0;
     int8_t *ad_code = nullptr;
     int32_t *ad_eslot = nullptr;
     int64_t ad_cap = 0;
@@ -269,6 +271,7 @@ void release(Batch &B) {
     (void)hipFree(B.ad_slots);
     (void)hipFree(B.ad_status);
     (void)hipFree(B.ad_off);
+    (void)hipFree(B.ad_off2);
     (void)hipFree(B.ad_base);
     (void)hipFree(B.ad_bsum);
     (void)hipFree(B.ad_code);
@@ -609,12 +612,14 @@ int ensure_adapt_scratch(Batch &B, int64_t n) {
     (void)hipFree(B.ad_slots);
     (void)hipFree(B.ad_status);
     (void)hipFree(B.ad_off);
+    (void)hipFree(B.ad_off2);
     (void)hipFree(B.ad_base);
     (void)hipFree(B.ad_bsum);
     (void)hipFree(B.ad_code);
     (void)hipFree(B.ad_eslot);
     B.ad_slots = B.ad_status = nullptr;
-    B.ad_off = B.ad_base = B.ad_bsum = nullptr;
+    B.ad_off = B.ad_off2 = B.ad_base = B.ad_bsum = nullptr;
+    B.ad_off_cap = B.ad_off2_cap = 0;
     B.ad_code = nullptr;
     B.ad_eslot = nullptr;
     B.ad_cap = 0;
@@ -622,6 +627,8 @@ int ensure_adapt_scratch(Batch &B, int64_t n) {
     SWE_TRY(dev_alloc(&B.ad_slots, cap));
     SWE_TRY(dev_alloc(&B.ad_status, 3));
     SWE_TRY(dev_alloc(&B.ad_off, cap));
+    SWE_TRY(dev_alloc(&B.ad_off2, cap));
+    B.ad_off_cap = B.ad_off2_cap = cap;
     SWE_TRY(dev_alloc(&B.ad_base, cap));
     SWE_TRY(dev_alloc(&B.ad_bsum, swe::adapt_scan_tiles(cap) + 1 + 8));
     SWE_TRY(dev_alloc(&B.ad_code, cap));
@@ -750,6 +757,17 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
             B.cap2 = B.cap;
         }
     }
+    if (n_new > B.ad_off2_cap) {  // room for the new mesh's tree offsets
+        HIP_TRY(hipStreamSynchronize(B.stream));
+        (void)hipFree(B.ad_off2);
+        B.ad_off2 = nullptr;
+        B.ad_off2_cap = 0;
+        SWE_TRY(dev_alloc(&B.ad_off2, n_new + n_new / 4));
+        B.ad_off2_cap = n_new + n_new / 4;
+    }
+    a.off_out = B.ad_off2;
+    a.list = reinterpret_cast<int32_t *>(B.ad_slots);  // the conformity words are spent
+    a.n_changing = plan.n_changing;
     a.eta_out = fresh ? eo : B.eta[1 - B.cur];
     a.hu_out = fresh ? ho : B.hu[1 - B.cur];
     a.hv_out = fresh ? vo : B.hv[1 - B.cur];
@@ -798,6 +816,9 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
         std::swap(B.depth, B.depth2);
         B.cur = 1 - B.cur;
     }
+    std::swap(B.ad_off, B.ad_off2);  // the new mesh's offsets stay valid
+    std::swap(B.ad_off_cap, B.ad_off2_cap);
+    B.ad_off_valid = true;
     B.ncells = B.ntopo = n_new;
     B.scen[0].n = n_new;
     B.scen[0].depth = -1;

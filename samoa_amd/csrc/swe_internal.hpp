@@ -96,7 +96,11 @@ struct AdaptArgs {
     int64_t *bsum;                         // [adapt_scan_tiles(n) + 1 + 8]
     int8_t *code;                          // [n]
     int32_t *eslot;                        // [6n] new cell-edge on each half of each old edge
+                                           //      (changing cells only)
     uint32_t *status;                      // [3]
+    int32_t *list;                         // [n] the changing cells (may alias slots)
+    int64_t n_changing;                    // their number (from adapt_plan)
+    int64_t *off_out;                      // [n'] tree offsets of the new mesh
     int8_t *depth_out, *plotter_out;       // [n'] new mesh
     int32_t *nbr_out;                      // [3n']
     double *eta_out, *hu_out, *hv_out, *b_out;
@@ -115,6 +119,7 @@ struct AdaptPlan {
     int64_t n_new;       // cells of the new mesh
     int64_t tree_total;  // covered part of the two root trees (2^61 when the mesh is whole)
     uint32_t status;     // bit 0: an inconsistent merge; bit 1: the mesh changes
+    int64_t n_changing;  // cells that split or merge
 };
 // Marks the edges, sweeps to the conformity fixpoint (first_batch sweeps
 // between host checks, then 4), then counts the new cells and their prefix
