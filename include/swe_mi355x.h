@@ -172,13 +172,15 @@ int swe_adapt(swe_handle *h, const int8_t *refine, int64_t *n_cells, int *sweeps
 /* Bathymetry of the cells that swe_adapt splits, in place of the parent's
  * value: bilinear interpolation in a raster of nx x ny values over the world
  * points (x0 + i dx, y0 + j dy), values[j * nx + i], at the child's centroid
- * in world coordinates, scaling * (x1 + x2 + x3) / 3 + offset (unit-square
- * corners), clamped to the raster's last cell.  This stands for
+ * in world coordinates, scaling * (x1 + x2 + x3) / 3 + (offset_x, offset_y)
+ * (unit-square corners; cfg%scaling and cfg%offset), clamped to the raster's
+ * last cell.  This stands for
  * get_bathymetry_at_element (SWE_initialize.f90:120-147) with gridded data
  * (the ASAGI path, SWE_initialize.f90:149-187).  The library copies the
  * values; NULL removes the raster. */
 int swe_set_bathymetry_raster(swe_handle *h, int64_t nx, int64_t ny, double x0, double y0,
-                              double dx, double dy, double offset, const double *values);
+                              double dx, double dy, double offset_x, double offset_y,
+                              const double *values);
 
 /* Refinement flags from the gradient of the surface elevation, for runs that
  * refine on |grad eta| rather than on volume_op's |dQ_h| criterion

@@ -411,7 +411,7 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
             if (a.raster) {
                 double cx, cy;
                 tree_centroid(coff, dc, &cx, &cy);
-                a.b_out[idx] = raster_sample(a, a.scaling * cx + a.offset, a.scaling * cy + a.offset);
+                a.b_out[idx] = raster_sample(a, a.scaling * cx + a.offset_x, a.scaling * cy + a.offset_y);
             } else {
                 a.b_out[idx] = a.b[c];
             }

@@ -231,7 +231,7 @@ struct Batch {
     int ad_last_sweeps = 3;     // sweeps the last remeshing needed
     double *raster = nullptr;  // bathymetry raster for split cells (swe_set_bathymetry_raster)
     int64_t rnx = 0, rny = 0;
-    double rx0 = 0, ry0 = 0, rdx = 1, rdy = 1, roffset = 0;
+    double rx0 = 0, ry0 = 0, rdx = 1, rdy = 1, roffset_x = 0, roffset_y = 0;
     int keep_flags = 0;       // every step writes its refinement flags to `refine`
     bool flags_valid = false; // `refine` holds the flags of the current state's last step
 };
@@ -684,7 +684,8 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.rdx = B.rdx;
     a.rdy = B.rdy;
     a.scaling = sc.scaling;
-    a.offset = B.roffset;
+    a.offset_x = B.roffset_x;
+    a.offset_y = B.roffset_y;
     swe::AdaptPlan plan{};
     HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, 1 << 16, B.ad_host,
                             &plan));
@@ -993,7 +994,8 @@ int swe_adapt(swe_handle *h, const int8_t *refine, int64_t *n_cells, int *sweeps
 }
 
 int swe_set_bathymetry_raster(swe_handle *h, int64_t nx, int64_t ny, double x0, double y0,
-                              double dx, double dy, double offset, const double *values) {
+                              double dx, double dy, double offset_x, double offset_y,
+                              const double *values) {
     if (!h) return fail(SWE_ERR_ARG, "swe_set_bathymetry_raster: null handle");
     Batch &B = h->B;
     if (!values) {
@@ -1003,7 +1005,7 @@ int swe_set_bathymetry_raster(swe_handle *h, int64_t nx, int64_t ny, double x0, 
     }
     if (nx < 2 || ny < 2 || nx > (int64_t(1) << 20) || ny > (int64_t(1) << 20) ||
         !(dx > 0.0) || !(dy > 0.0) || !std::isfinite(x0) || !std::isfinite(y0) ||
-        !std::isfinite(offset))
+        !std::isfinite(offset_x) || !std::isfinite(offset_y))
         return fail(SWE_ERR_ARG,
                     "swe_set_bathymetry_raster: need 2 <= nx, ny <= 2^20 and dx, dy > 0");
     double *d = nullptr;
@@ -1021,7 +1023,8 @@ int swe_set_bathymetry_raster(swe_handle *h, int64_t nx, int64_t ny, double x0, 
     B.ry0 = y0;
     B.rdx = dx;
     B.rdy = dy;
-    B.roffset = offset;
+    B.roffset_x = offset_x;
+    B.roffset_y = offset_y;
     return SWE_OK;
 }
 

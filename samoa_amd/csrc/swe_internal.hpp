@@ -109,7 +109,7 @@ struct AdaptArgs {
     // child's centroid scaling * c + offset; null: the parent's b
     const double *raster;
     int64_t rnx, rny;
-    double rx0, ry0, rdx, rdy, scaling, offset;
+    double rx0, ry0, rdx, rdy, scaling, offset_x, offset_y;
 };
 
 // bsum holds adapt_scan_tiles(n) + 1 scan words and 8 result words.

@@ -136,18 +136,21 @@ class EulerTimestep:
         return self.n, sweeps.value
 
     def set_bathymetry_raster(self, values, x0: float, y0: float, dx: float, dy: float,
-                              offset: float) -> None:
+                              offset) -> None:
         """Cells that adapt() splits take b from this raster (values[ny, nx]
         over the world points (x0 + i dx, y0 + j dy)), bilinear at their
-        centroid; None: they copy the parent's b."""
+        centroid scaling * c + offset (offset: a number or (x, y)); values
+        None: they copy the parent's b."""
         if values is None:
             _native.check("swe_set_bathymetry_raster", self._lib.swe_set_bathymetry_raster(
-                self._h, 0, 0, 0.0, 0.0, 1.0, 1.0, 0.0, None))
+                self._h, 0, 0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0, None))
             return
         v = np.ascontiguousarray(values, dtype=np.float64)
         ny, nx = v.shape
+        ox, oy = (offset, offset) if np.isscalar(offset) else offset
         _native.check("swe_set_bathymetry_raster", self._lib.swe_set_bathymetry_raster(
-            self._h, nx, ny, float(x0), float(y0), float(dx), float(dy), float(offset), _ptr(v)))
+            self._h, nx, ny, float(x0), float(y0), float(dx), float(dy), float(ox), float(oy),
+            _ptr(v)))
 
     def download_mesh(self):
         """(depth[n], plotter[n], nbr[n, 3]) of the current mesh."""

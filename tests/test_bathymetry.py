@@ -1,0 +1,78 @@
+"""Gridded bathymetry (samoa_amd/bathymetry.py): netCDF classic grids read
+back exactly, the reference's domain rule (SWE.f90:133-137) on a worked
+example, and bilinear sampling that reproduces bilinear functions."""
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from samoa_amd import bathymetry as bt
+from samoa_amd import host
+
+
+def write_grid(path, x, y, z, name="z"):
+    from scipy.io import netcdf_file
+
+    with netcdf_file(path, "w") as nc:
+        nc.createDimension("x", len(x))
+        nc.createDimension("y", len(y))
+        nc.createVariable("x", "f8", ("x",))[:] = x
+        nc.createVariable("y", "f8", ("y",))[:] = y
+        nc.createVariable(name, "f4", ("y", "x"))[:] = z
+
+
+def test_read_netcdf_grid_round_trip():
+    x = -1000.0 + 250.0 * np.arange(9)
+    y = 500.0 + 125.0 * np.arange(5)
+    z = np.random.default_rng(0).normal(size=(5, 9)).astype(np.float32)
+    with tempfile.TemporaryDirectory() as td:
+        f = os.path.join(td, "bath.nc")
+        write_grid(f, x, y, z)
+        r = bt.read_netcdf_grid(f)
+    assert (r.x0, r.y0, r.dx, r.dy) == (-1000.0, 500.0, 250.0, 125.0)
+    assert np.array_equal(r.values, z.astype(np.float64))
+    assert r.extent == ((-1000.0, 1000.0), (500.0, 1000.0))
+
+
+def test_read_netcdf_rejects_uneven_axes():
+    with tempfile.TemporaryDirectory() as td:
+        f = os.path.join(td, "bad.nc")
+        write_grid(f, np.array([0.0, 1.0, 3.0]), np.array([0.0, 1.0]), np.zeros((2, 3)))
+        with pytest.raises(ValueError):
+            bt.read_netcdf_grid(f)
+
+
+def test_asagi_domain_worked_example():
+    # bathymetry over [0, 100] x [0, 50], displacement over [70, 90] x [15, 35]:
+    # scaling 100; the square centred on (80, 25) starts at (30, -25), which
+    # the clamp moves to x = min(max(30, 0), 100 - 100) = 0 and
+    # y = min(max(-25, 0), 50 - 100) = -50
+    bath = bt.Raster(np.zeros((11, 21)), 0.0, 0.0, 5.0, 5.0)
+    displ = bt.Raster(np.zeros((5, 5)), 70.0, 15.0, 5.0, 5.0)
+    assert bt.asagi_domain(bath, displ) == (100.0, (0.0, -50.0))
+    # without a displacement grid the square sits on the bathymetry
+    assert bt.asagi_domain(bath) == (100.0, (0.0, -50.0))
+
+
+def test_sample_reproduces_bilinear_functions():
+    r = bt.Raster(np.zeros((7, 13)), -3.0, 2.0, 0.5, 0.25)
+    ny, nx = r.values.shape
+    X, Y = np.meshgrid(r.x0 + r.dx * np.arange(nx), r.y0 + r.dy * np.arange(ny))
+    f = lambda x, y: 1.5 - 0.25 * x + 2.0 * y + 0.125 * x * y  # noqa: E731
+    r.values = f(X, Y)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-3.0, -3.0 + 6.0, 1000)
+    y = rng.uniform(2.0, 2.0 + 1.5, 1000)
+    assert np.allclose(bt.sample(r, x, y), f(x, y), rtol=0, atol=1e-12)
+    assert np.array_equal(bt.sample(r, X.ravel(), Y.ravel()), r.values.ravel())
+    # outside the grid: the edge cell's bilinear extended
+    assert np.allclose(bt.sample(r, np.array([-4.0]), np.array([2.5])), f(-4.0, 2.5), atol=1e-12)
+
+
+def test_cell_bathymetry_uses_centroids():
+    mesh = host.mesh_uniform(4)
+    r = bt.Raster(np.add.outer(np.arange(5.0), 10.0 * np.arange(6.0)), 0.0, 0.0, 2.5, 2.0)
+    b = bt.cell_bathymetry(r, mesh.coords, 10.0, (0.0, 0.0))
+    c = 10.0 * mesh.coords.mean(axis=1)
+    assert np.allclose(b, 10.0 * c[:, 0] / 2.5 + c[:, 1] / 2.0, atol=1e-12)

@@ -210,3 +210,35 @@ def test_run_continues_the_device_chain():
     assert np.array_equal(np.stack(ops[0].download()), np.stack(ops[1].download()))
     for op in ops:
         op.close()
+
+
+def test_split_cells_sample_a_netcdf_grid():
+    """A netCDF bathymetry grid placed with the reference's domain rule:
+    the cells a refinement creates take b from it, bit-identical to the
+    host's sampling at their centroids on the finer mesh."""
+    import os
+    import tempfile
+
+    from samoa_amd import bathymetry as bt
+    from test_bathymetry import write_grid
+
+    rng = np.random.default_rng(5)
+    x = 1000.0 * np.arange(41)
+    y = 2000.0 + 1000.0 * np.arange(31)
+    z = (-3000.0 + 500.0 * rng.normal(size=(31, 41))).astype(np.float32)
+    with tempfile.TemporaryDirectory() as td:
+        f = os.path.join(td, "bath.nc")
+        write_grid(f, x, y, z)
+        r = bt.read_netcdf_grid(f)
+    S, off = bt.asagi_domain(r)
+    d = 9
+    mesh = host.mesh_uniform(d)
+    b = bt.cell_bathymetry(r, mesh.coords, S, off)
+    eta = np.zeros(mesh.n)
+    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3, min_depth=d, max_depth=d + 1)
+    op.upload(eta, eta, eta, b)
+    op.set_bathymetry_raster(r.values, r.x0, r.y0, r.dx, r.dy, off)
+    op.adapt(np.ones(mesh.n, np.int8))
+    fine = host.mesh_uniform(d + 1)
+    assert np.array_equal(op.download()[3], bt.cell_bathymetry(r, fine.coords, S, off))
+    op.close()
