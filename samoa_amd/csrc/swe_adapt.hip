@@ -511,11 +511,13 @@ __global__ void results_kernel(int64_t *res, const int64_t *bsum, int64_t tiles,
     }
 }
 // res[3] = the last sweep that changed a word, res[4] = the last sweep that
-// saw a cell ask for a split or merge
-__global__ void sweep_results_kernel(int64_t *res, const uint32_t *status) {
+// saw a cell ask for a split or merge; clears the error bits for the counts
+// that follow
+__global__ void sweep_results_kernel(int64_t *res, uint32_t *status) {
     if (threadIdx.x == 0) {
         res[3] = status[1];
         res[4] = status[2];
+        status[0] = 0u;
     }
 }
 
@@ -524,7 +526,7 @@ __global__ void sweep_results_kernel(int64_t *res, const uint32_t *status) {
 int64_t adapt_scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
-                      int max_sweeps, int64_t *host, AdaptPlan *plan) {
+                      bool count_ahead, int max_sweeps, int64_t *host, AdaptPlan *plan) {
     const unsigned nb = blocks_for(a.n, AT);
     const int64_t tiles = adapt_scan_tiles(a.n);
     int64_t *res = a.bsum + tiles + 1;
@@ -543,18 +545,32 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
     }
     // sweeps go in batches between host checks: a batch whose last sweep
     // changed nothing has reached the fixpoint (further sweeps are no-ops)
+    // the counts behind the sweeps: valid when the batch's last sweep changed
+    // nothing, which is when they are read
+    auto counts = [&]() -> hipError_t {
+        count_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.off, a.code, a.base,
+                                       a.status);
+        hipError_t r = hipGetLastError();
+        if (r != hipSuccess) return r;
+        if ((r = scan_i64(a.base, a.base, a.n, a.bsum, s)) != hipSuccess) return r;
+        results_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles, a.status);
+        return hipGetLastError();
+    };
     plan->sweeps = -1;
     int batch = first_batch < 2 ? 2 : first_batch, done = 0;
+    bool counted = false;
     for (; done < max_sweeps; done += batch, batch = 4) {
         for (int j = 1; j <= batch; ++j)
             conf_sweep_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.status + 1,
                                                 a.status + 2, (uint32_t)(done + j));
         sweep_results_kernel<<<1, 64, 0, s>>>(res, a.status);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = fetch(0, 5)) != hipSuccess) return e;
+        if (count_ahead && (e = counts()) != hipSuccess) return e;
+        if ((e = fetch(0, 6)) != hipSuccess) return e;
         if (host[3] < done + batch) {
             plan->sweeps = (int)host[3] + 1;
             done += batch;
+            counted = count_ahead;
             break;
         }
     }
@@ -566,13 +582,10 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
         plan->status = 0;
         return hipSuccess;
     }
-    count_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.off, a.code, a.base,
-                                   a.status);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = scan_i64(a.base, a.base, a.n, a.bsum, s)) != hipSuccess) return e;
-    results_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles, a.status);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = fetch(1, 5)) != hipSuccess) return e;
+    if (!counted) {
+        if ((e = counts()) != hipSuccess) return e;
+        if ((e = fetch(1, 5)) != hipSuccess) return e;
+    }
     plan->n_new = host[1];
     plan->status = (uint32_t)host[2];
     plan->n_changing = host[5];

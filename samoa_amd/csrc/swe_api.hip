@@ -229,6 +229,7 @@ struct Batch {
     bool ad_off_valid = false;  // ad_off holds the current mesh's tree offsets
     int64_t *ad_host = nullptr; // [8] pinned results of the remesher
     int ad_last_sweeps = 3;     // sweeps the last remeshing needed
+    bool ad_last_changed = false;  // the last remeshing changed the mesh
     double *raster = nullptr;  // bathymetry raster for split cells (swe_set_bathymetry_raster)
     int64_t rnx = 0, rny = 0;
     double rx0 = 0, ry0 = 0, rdx = 1, rdy = 1, roffset_x = 0, roffset_y = 0;
@@ -687,8 +688,9 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.offset_x = B.roffset_x;
     a.offset_y = B.roffset_y;
     swe::AdaptPlan plan{};
-    HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, 1 << 16, B.ad_host,
-                            &plan));
+    HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, B.ad_last_changed,
+                            1 << 16, B.ad_host, &plan));
+    B.ad_last_changed = (plan.status & 2u) != 0;
     const int sweeps = plan.sweeps;
     if (sweeps < 0) return fail(SWE_ERR_STATE, "adapt: the conformity sweeps did not converge");
     B.ad_last_sweeps = sweeps;

@@ -125,10 +125,11 @@ struct AdaptPlan {
 // between host checks, then 4), then counts the new cells and their prefix
 // (base; its high word counts the cells that change) unless no cell splits
 // or merges.  offsets_valid: a.off still holds this mesh's tree offsets.
-// host: 8 words of pinned memory for the results.  Synchronises the stream
-// once, or twice when the mesh changes.
+// count_ahead: count behind every batch of sweeps, before the host knows
+// whether the mesh changes (one round trip instead of two when it does).
+// host: 8 words of pinned memory for the results.  Synchronises the stream.
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
-                      int max_sweeps, int64_t *host, AdaptPlan *plan);
+                      bool count_ahead, int max_sweeps, int64_t *host, AdaptPlan *plan);
 // Refinement flags from surface-elevation jumps to wet neighbours.
 hipError_t flag_eta_jumps(int64_t n, const double *eta, const double *b, const int32_t *nbr,
                           const int8_t *depth, double drytol, double refine_tol,
