@@ -126,50 +126,35 @@ def tsunami_adaptive(args, torch):
 
     S, courant, tol = 1.0e6, 0.45, args.adapt_tol
     dmin, dmax = 16, 26
-    src = host.tsunami_draw(0)
-    a_ax, c_ax = 1.0e5, 1.0e5 / src["ratio"]
-
-    def source_eta(p):
-        return src["amplitude"] * np.exp(-(((p[..., 0] - src["x0"]) / a_ax) ** 2 +
-                                           ((p[..., 1] - src["y0"]) / c_ax) ** 2))
-
     mesh0 = host.mesh_uniform(dmin)
     op = swe.EulerTimestep(mesh0, scaling=S, dry_tolerance=1e-3, courant=courant,
                            min_depth=dmin, max_depth=dmax, solver=swe.SOLVER_AUG_RIEMANN,
                            boundary=swe.BC_OUTFLOW)
     z = np.zeros(mesh0.n)
     op.upload(z, z, z, z)
-    depth, rounds = mesh0.depth, 0
+    # the Fortran host's set-up: the initial phase of SWE.f90:230-262 with a
+    # tolerance on the source's variation across a cell, the scenario state,
+    # the bathymetry raster for later splits (swe_driver.f90)
     t0 = time.perf_counter()
-    while True:  # the initial phase of SWE.f90:230-262 with a tolerance on the corner range
-        plotter, corners = host.coords_from_depths(depth)
-        e = source_eta(S * corners)
-        flags = ((depth < dmax) & (e.max(axis=1) - e.min(axis=1) > tol)).astype(np.int8)
-        if not flags.any():
-            break
-        op.adapt(flags)
-        depth = op.download_mesh()[0]
-        rounds += 1
+    n0, dt0 = host.tsunami_adaptive_setup(op, 0, tol, dmax, 1e-3, courant)
     t_init = time.perf_counter() - t0
-    mesh = swe.Mesh(depth, plotter, None, corners)
-    state = host.tsunami(mesh, seed=0, drytol=1e-3)
-    op.upload(*state)
-    r = host.tsunami_raster(0)
-    op.set_bathymetry_raster(r["values"], r["x0"], r["y0"], r["dx"], r["dy"], r["offset"])
-    n0, depths0 = mesh.n, np.bincount(depth, minlength=dmax + 1)[dmin:].tolist()
-    dt = dt0 = host.initial_dt(mesh, state, courant)
+    depth = op.download_mesh()[0]
+    depths0 = np.bincount(depth, minlength=dmax + 1)[dmin:].tolist()
+    dt = dt0
     stream = torch.cuda.Stream(device=torch.device("cuda:0"))
     op.set_stream(stream.cuda_stream)
 
+    state0 = op.download()
     t, fwd = 0.0, False
     step_ev = []
 
     def one_step(k, timed):
         nonlocal t, dt, fwd
-        if k > 0 and k % args.nadapt == 0:
-            # a jump to a neighbour's centroid spans about half a corner range
-            op.flag_eta_jumps(tol / 2.0, tol / 16.0)
-            op.adapt()
+        if (k + 1) % args.nadapt == 0:  # time step k + 1 of SWE.f90:385-390
+            if k > 0:  # before the first step the flags are all 0: nothing changes
+                # a jump to a neighbour's centroid spans about half a corner range
+                op.flag_eta_jumps(tol / 2.0, tol / 16.0)
+                op.adapt()
             fwd = not fwd  # the adaption traversal flips the direction
         n = op.n
         if timed:
@@ -208,7 +193,6 @@ def tsunami_adaptive(args, torch):
         raise RuntimeError("non-finite state after the adaptive steps")
     d_end = op.download_mesh()[0]
     op.close()
-    mesh.nbr = host.neighbours_from_corners(corners)
     updates = float(np.sum(cells))
     elapsed = max(ms * 1e-3, wall)
     return {
@@ -222,9 +206,9 @@ def tsunami_adaptive(args, torch):
         "cells_final": int(len(d_end)),
         "depths_final_16_to_26": np.bincount(d_end, minlength=dmax + 1)[dmin:].tolist(),
         "cells_min": int(min(cells)), "cells_max": int(max(cells)),
-        "initial_refinement_rounds": rounds, "initial_refinement_s": t_init,
+        "initial_refinement_s": t_init,
         "nadapt": args.nadapt, "adapt_tol_m": tol, "simulated_time_s": t,
-        "mesh": mesh, "state0": state, "dt0": dt0,
+        "depth0": depth, "state0": state0, "dt0": dt0,
     }
 
 
@@ -411,7 +395,7 @@ def adjoint(args, torch):
         t_bwd = ev[1].elapsed_time(ev[2])
         finite = bool(torch.isfinite(e0g.grad).all()) and bool(torch.isfinite(bg.grad).all())
         sweeps[mode] = {"K": prob.K, "store_all": prob.store_all, "ms_forward_sweep": t_fwd,
-                        "ms_backward_sweep": t_bwd, "ratio": t_bwd / t_fwd, "loss": float(loss),
+                        "ms_backward_sweep": t_bwd, "ratio": t_bwd / t_fwd, "loss": float(loss.detach()),
                         "gradients_finite": finite}
         if mode == "sqrt":
             g_sqrt = (e0g.grad.clone(), bg.grad.clone())
@@ -601,9 +585,12 @@ def main():
         out["adjoint"] = adjoint(args, torch)
         out["inversion"] = inversion(args, torch)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(ad["mesh"], ad["state0"], ad["dt0"] * 1.0e6)
+        from samoa_amd import host
+
+        mesh0 = host.mesh_from_depths(ad["depth0"])
+        out["cpu_baseline"] = cpu_baseline(mesh0, ad["state0"], ad["dt0"] * 1.0e6)
         out["cpu_baseline"]["sample"] += "; the adaptive run's initial mesh, Euler steps only"
-    ad.pop("mesh"), ad.pop("state0"), ad.pop("dt0")
+    ad.pop("depth0"), ad.pop("state0"), ad.pop("dt0")
     print(json.dumps(out), flush=True)
 
 

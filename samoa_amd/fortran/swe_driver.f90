@@ -19,7 +19,7 @@ module swe_driver
     use swe_scenarios
     implicit none
     private
-    public :: swe_time_loop, initial_dt, leg_size
+    public :: swe_time_loop, initial_dt, leg_size, adaptive_loop, tsunami_adaptive_setup
 
     real(c_double), parameter :: HUGE_SR = huge(1.0_c_double)
 
@@ -79,13 +79,14 @@ contains
         real(c_double), intent(in)         :: courant, tmax, tout
         integer(c_int64_t), intent(in)     :: nmax
         logical, intent(in)                :: host_steps
-        real(c_double), intent(inout)      :: dt, time
+        real(c_double), intent(inout), target :: dt, time
         logical, intent(inout)             :: forward
         integer(c_int64_t), intent(out)    :: steps
         !> int8 [n] or c_null_ptr: refinement flags of the last step (host steps only)
         type(c_ptr), intent(in), optional  :: refine
         type(c_ptr) :: rf
-        real(c_double) :: dt_cfl, clamp
+        real(c_double), target :: dt_cfl
+        real(c_double) :: clamp
         integer(c_int64_t) :: nref
         integer(c_int) :: fw
 
@@ -98,7 +99,7 @@ contains
             if (tout > 0.0_c_double) clamp = tout
             if (tout > 0.0_c_double) dt = min(tout, dt)
             fw = merge(1_c_int, 0_c_int, forward)
-            rc = swe_run(handle, nmax, dt, fw, clamp, time, dt_cfl)
+            rc = swe_run(handle, nmax, dt, fw, clamp, c_loc(time), c_loc(dt_cfl))
             if (rc /= SWE_OK) return
             dt = dt_cfl
             steps = nmax
@@ -119,8 +120,147 @@ contains
         end do
     end function
 
+    !> SWE.f90:380-429 with remeshing before every nadapt-th step
+    !> (:385-390), the flags taken from the surface-elevation jumps between
+    !> wet neighbours (refine above refine_tol, coarsen below coarsen_tol).
+    !> k0 numbers the first step from 0, so that calls continue one another:
+    !> step k remeshes when mod(k + 1, nadapt) = 0, as time step k + 1 of the
+    !> reference does; before step 0 the flags are the initial phase's, all 0,
+    !> so that remeshing only flips the direction.  dt > 0 starts the device dt chain
+    !> at (dt, time); dt <= 0 continues it.  Nothing waits for the steps
+    !> until the end, where time and dt (the next step's) are read back.
+    !> cells: the cells stepped, summed over the steps.  Returns a SWE_* status.
+    integer(c_int) function adaptive_loop(handle, nsteps, nadapt, refine_tol, coarsen_tol, k0, &
+                                          dt, time, forward, cells) result(rc)
+        type(c_ptr), intent(in)         :: handle
+        integer(c_int64_t), intent(in)  :: nsteps, nadapt, k0
+        real(c_double), intent(in)      :: refine_tol, coarsen_tol
+        real(c_double), intent(inout), target :: dt, time
+        logical, intent(inout)          :: forward
+        integer(c_int64_t), intent(out) :: cells
+        integer(c_int64_t) :: k, n_new
+        integer(c_int) :: sweeps
+        real(c_double) :: dt_first
+        dt_first = dt
+        cells = 0
+        rc = SWE_OK
+        do k = k0, k0 + nsteps - 1
+            if (nadapt > 0) then
+                if (mod(k + 1, nadapt) == 0) then
+                    if (k > 0) then
+                        rc = swe_flag_eta_jumps(handle, refine_tol, coarsen_tol)
+                        if (rc /= SWE_OK) return
+                        rc = swe_adapt(handle, c_null_ptr, n_new, sweeps)
+                        if (rc /= SWE_OK) return
+                    end if
+                    forward = .not. forward  ! the adaption traversal flips it too
+                end if
+            end if
+            cells = cells + swe_cells(handle)
+            if (k == k0 .and. dt_first > 0.0_c_double) then
+                rc = swe_run(handle, 1_c_int64_t, dt_first, merge(1_c_int, 0_c_int, forward), &
+                             -1.0_c_double, c_loc(time), c_null_ptr)
+            else
+                rc = swe_run(handle, 1_c_int64_t, -1.0_c_double, merge(1_c_int, 0_c_int, forward), &
+                             -1.0_c_double, c_null_ptr, c_null_ptr)
+            end if
+            if (rc /= SWE_OK) return
+            forward = .not. forward
+        end do
+        rc = swe_run(handle, 0_c_int64_t, -1.0_c_double, 0_c_int, -1.0_c_double, c_loc(time), c_loc(dt))
+    end function
+
+    !> The adaptive tsunami's initial mesh and state on `handle`, which holds
+    !> a uniform mesh and any state: the initial phase of SWE.f90:230-262
+    !> with the tsunami's criterion, refining every cell shallower than dmax
+    !> whose corners see the source's surface elevation differ by more than
+    !> tol, until none does; then the scenario state at the centroids, the
+    !> bathymetry raster for the cells later remeshings split, and dt0
+    !> (initial_dt).  n_out: the cells.  Returns a SWE_* status.
+    integer(c_int) function tsunami_adaptive_setup(handle, seed, tol, dmax, drytol, courant, &
+                                                   n_out, dt0) result(rc)
+        type(c_ptr), intent(in)         :: handle
+        integer(c_int64_t), intent(in)  :: seed
+        real(c_double), intent(in)      :: tol, drytol, courant
+        integer, intent(in)             :: dmax
+        integer(c_int64_t), intent(out) :: n_out
+        real(c_double), intent(out)     :: dt0
+        integer(c_int8_t), allocatable, target :: depth(:), plotter(:), refine(:)
+        real(c_double), allocatable, target :: coords(:, :, :), eta(:), hu(:), hv(:), b(:)
+        real(c_double), allocatable, target :: grid(:, :)
+        real(c_double) :: amp, ratio, x0, y0, a, c, e(3), p(2)
+        integer(c_int64_t) :: n, i, n_new
+        integer :: depth_i, j, ierr
+        integer(c_int) :: sweeps
+        call tsunami_draw(seed, amp, ratio, x0, y0, depth_i, dmax, dmax)
+        a = 1.0e5_c_double
+        c = a / ratio
+        do
+            n = swe_cells(handle)
+            if (allocated(depth)) deallocate(depth, plotter, refine, coords)
+            allocate(depth(n), plotter(n), refine(n), coords(2, 3, n))
+            rc = swe_download_mesh(handle, c_loc(depth), c_loc(plotter), c_null_ptr)
+            if (rc /= SWE_OK) return
+            call sfc_mesh_from_depths(n, depth, plotter, coords, ierr)
+            if (ierr /= 0) then
+                rc = SWE_ERR_STATE
+                return
+            end if
+            do i = 1, n
+                refine(i) = 0
+                if (depth(i) >= dmax) cycle
+                do j = 1, 3
+                    p = TSU_SCALING * coords(:, j, i)
+                    e(j) = amp * exp(-(((p(1) - x0) / a) ** 2 + ((p(2) - y0) / c) ** 2))
+                end do
+                if (maxval(e) - minval(e) > tol) refine(i) = 1
+            end do
+            if (count(refine == 1) == 0) exit
+            rc = swe_adapt(handle, c_loc(refine), n_new, sweeps)
+            if (rc /= SWE_OK) return
+        end do
+        allocate(eta(n), hu(n), hv(n), b(n), grid(TSU_G, TSU_G))
+        call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+        rc = swe_upload(handle, eta, hu, hv, b)
+        if (rc /= SWE_OK) return
+        call tsunami_raster(seed, grid)
+        rc = swe_set_bathymetry_raster(handle, int(TSU_G, c_int64_t), int(TSU_G, c_int64_t), &
+                                       0.0_c_double, 0.0_c_double, TSU_DX, TSU_DX, 0.0_c_double, &
+                                       0.0_c_double, c_loc(grid))
+        if (rc /= SWE_OK) return
+        dt0 = initial_dt(n, depth, eta, hu, hv, b, 9.80665_c_double, courant)
+        n_out = n
+    end function
+
     ! ------------------------------------------------------------------
     ! C entry points for the Python harness
+
+    integer(c_int) function swe_host_adaptive_loop(handle, nsteps, nadapt, refine_tol, coarsen_tol, k0, &
+                                                   dt, time, forward, cells) bind(c, name='swe_host_adaptive_loop')
+        type(c_ptr), value               :: handle
+        integer(c_int64_t), value        :: nsteps, nadapt, k0
+        real(c_double), value            :: refine_tol, coarsen_tol
+        real(c_double), intent(inout)    :: dt, time
+        integer(c_int), intent(inout)    :: forward
+        integer(c_int64_t), intent(out)  :: cells
+        logical :: fw
+        fw = forward /= 0
+        swe_host_adaptive_loop = adaptive_loop(handle, nsteps, nadapt, refine_tol, coarsen_tol, k0, &
+                                               dt, time, fw, cells)
+        forward = merge(1_c_int, 0_c_int, fw)
+    end function
+
+    integer(c_int) function swe_host_tsunami_adaptive_setup(handle, seed, tol, dmax, drytol, courant, &
+                                                            n_out, dt0) bind(c, name='swe_host_tsunami_adaptive_setup')
+        type(c_ptr), value               :: handle
+        integer(c_int64_t), value        :: seed
+        real(c_double), value            :: tol, drytol, courant
+        integer(c_int), value            :: dmax
+        integer(c_int64_t), intent(out)  :: n_out
+        real(c_double), intent(out)      :: dt0
+        swe_host_tsunami_adaptive_setup = tsunami_adaptive_setup(handle, seed, tol, int(dmax), drytol, &
+                                                                 courant, n_out, dt0)
+    end function
 
     real(c_double) function swe_host_initial_dt(n, depth, eta, hu, hv, b, g, courant) &
             bind(c, name='swe_host_initial_dt')

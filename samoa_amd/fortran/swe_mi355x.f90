@@ -84,6 +84,8 @@ module swe_mi355x
             type(c_ptr), value          :: refine
         end function
 
+        !> time, dt_next: c_loc of the values, or c_null_ptr (both: do not wait);
+        !> dt0 <= 0 continues the device dt chain
         integer(c_int) function swe_run(handle, n_steps, dt0, forward, dt_clamp, time, dt_next) bind(c, name='swe_run')
             import :: c_int, c_ptr, c_double, c_int64_t
             type(c_ptr), value          :: handle
@@ -91,8 +93,24 @@ module swe_mi355x
             real(c_double), value       :: dt0
             integer(c_int), value       :: forward
             real(c_double), value       :: dt_clamp
-            real(c_double), intent(inout) :: time
-            real(c_double), intent(out) :: dt_next
+            type(c_ptr), value          :: time, dt_next
+        end function
+
+        !> refinement flags from surface-elevation jumps, for swe_adapt(h, c_null_ptr, ...)
+        integer(c_int) function swe_flag_eta_jumps(handle, refine_tol, coarsen_tol) bind(c, name='swe_flag_eta_jumps')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: handle
+            real(c_double), value   :: refine_tol, coarsen_tol
+        end function
+
+        !> bathymetry of split cells: bilinear in values(nx, ny) at world (x0 + i dx, y0 + j dy)
+        integer(c_int) function swe_set_bathymetry_raster(handle, nx, ny, x0, y0, dx, dy, offset_x, offset_y, &
+                                                          values) bind(c, name='swe_set_bathymetry_raster')
+            import :: c_int, c_ptr, c_double, c_int64_t
+            type(c_ptr), value          :: handle
+            integer(c_int64_t), value   :: nx, ny
+            real(c_double), value       :: x0, y0, dx, dy, offset_x, offset_y
+            type(c_ptr), value          :: values
         end function
 
         integer(c_int) function swe_destroy(handle) bind(c, name='swe_destroy')

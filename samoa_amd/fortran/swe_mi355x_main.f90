@@ -8,16 +8,20 @@
 !              [-seed s] [-boundary outflow|wall] [-solver aug_riemann|fwave]
 !              [-dump prefix] [-hoststeps] [-xmloutput .true.] [-output_dir dir]
 !              [-stestpoints "x y,x y,..."] [-pointoutput_full .true.]
-!              [-dam_left hL] [-dam_right hR]
+!              [-dam_left hL] [-dam_right hR] [-refine_tol m]
 !
 ! The mesh starts uniform at depth -dstart (default: -dmax).  Below -dmax the
 ! initial phase of SWE.f90:230-262 refines it: a cell shallower than -dmin,
 ! or one whose corners see different initial water heights
 ! (SWE_initialize.f90:292-320), is refined, closed by the conformity rules,
-! until no cell asks (dambreak and radial scenarios).  With -nadapt k the
-! mesh is remeshed on the device before every k-th step from the refinement
-! flags of the step before (SWE.f90:387-390); each remeshing flips the
-! traversal direction, as the reference's adaption traversal does.  The dam
+! until no cell asks (dambreak and radial scenarios); the tsunami scenario
+! refines where its source varies by more than -refine_tol metres across a
+! cell instead.  With -nadapt k the mesh is remeshed on the device before
+! every k-th step from the refinement flags of the step before
+! (SWE.f90:387-390), or for the tsunami from the surface-elevation jumps
+! between wet neighbours (refine above -refine_tol / 2, coarsen below
+! -refine_tol / 16); each remeshing flips the traversal direction, as the
+! reference's adaption traversal does.  The dam
 ! break's heights default to the reference's 1 m | dry (SWE_initialize.f90:354).
 !
 ! Like the reference CLI (Config.f90:213-296, M_kracken rget), -courant,
@@ -46,7 +50,7 @@ program swe_mi355x_main
     integer(c_int64_t) :: nmax, n, seed, steps, k, nsteps_done, chunk
     real :: r_courant, r_drytol, r_tmax, r_tout
     real(c_double) :: courant, drytol, tmax, tout, scaling, offset, dt, time, amp, ratio, x0, y0
-    real(c_double) :: dam_left, dam_right
+    real(c_double) :: dam_left, dam_right, refine_tol
     integer(c_int64_t) :: n_new
     integer(c_int) :: sweeps
     character(len=256) :: arg, val, scenario, dump, bcname, solvername, outdir
@@ -64,7 +68,7 @@ program swe_mi355x_main
     integer :: iout
 
     dmax = 14; dmin = -1; dstart = -1; nadapt = -1; nmax = -1; nout = -1
-    dam_left = 1.0_c_double; dam_right = 0.0_c_double
+    dam_left = 1.0_c_double; dam_right = 0.0_c_double; refine_tol = 1.5e-3_c_double
     r_courant = 0.45; r_drytol = 0.01; r_tmax = 3600.0; r_tout = -1.0
     scenario = 'dambreak'; dump = ''; bcname = 'outflow'; solvername = 'aug_riemann'
     seed = 0; host_steps = .false.; xml = .false.; outdir = 'output'; probes = ''
@@ -82,6 +86,7 @@ program swe_mi355x_main
         case ('-nadapt'); read(val, *) nadapt; i = i + 1
         case ('-dam_left'); read(val, *) dam_left; i = i + 1
         case ('-dam_right'); read(val, *) dam_right; i = i + 1
+        case ('-refine_tol'); read(val, *) refine_tol; i = i + 1
         case ('-nmax'); read(val, *) nmax; i = i + 1
         case ('-nout'); read(val, *) nout; i = i + 1
         case ('-tmax'); read(val, *) r_tmax; i = i + 1
@@ -115,10 +120,6 @@ program swe_mi355x_main
         scaling = 1.0_c_double; offset = 0.0_c_double
     case ('tsunami')
         scaling = 1.0e6_c_double; offset = 0.0_c_double
-        if (dstart /= dmax) then
-            print '(A)', 'swe_mi355x: the tsunami scenario starts at -dmax (no initial refinement)'
-            stop 2
-        end if
     case default
         print '(A, A)', 'swe_mi355x: unknown scenario ', trim(scenario)
         stop 2
@@ -150,7 +151,18 @@ program swe_mi355x_main
     call check(rc, 'swe_create')
 
     ! initial phase (SWE.f90:230-262): refine until no cell asks
-    if (dstart < dmax .or. dstart < dmin) then
+    if (trim(scenario) == 'tsunami' .and. dstart < dmax) then
+        ! the tsunami refines where the source varies by more than -refine_tol
+        ! across a cell, and samples b from its raster for later splits
+        eta = 0; hu = 0; hv = 0; b = 0
+        rc = swe_upload(h, eta, hu, hv, b)
+        call check(rc, 'swe_upload')
+        rc = tsunami_adaptive_setup(h, seed, refine_tol, dmax, drytol, courant, n_new, dt)
+        call check(rc, 'tsunami_adaptive_setup')
+        call sync_mesh()
+        rc = swe_download(h, c_loc(eta), c_loc(hu), c_loc(hv), c_loc(b))
+        call check(rc, 'swe_download')
+    else if (dstart < dmax .or. dstart < dmin) then
         eta = 0; hu = 0; hv = 0; b = 0
         rc = swe_upload(h, eta, hu, hv, b)
         call check(rc, 'swe_upload')
@@ -179,11 +191,15 @@ program swe_mi355x_main
         end do
         hu = 0; hv = 0; b = 0
     case ('tsunami')
-        call tsunami_draw(seed, amp, ratio, x0, y0, depth_i, dmax, dmax)
-        call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+        if (dstart == dmax) then
+            call tsunami_draw(seed, amp, ratio, x0, y0, depth_i, dmax, dmax)
+            call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+        end if
     end select
-    rc = swe_upload(h, eta, hu, hv, b)
-    call check(rc, 'swe_upload')
+    if (.not. (trim(scenario) == 'tsunami' .and. dstart < dmax)) then
+        rc = swe_upload(h, eta, hu, hv, b)
+        call check(rc, 'swe_upload')
+    end if
     if (nadapt > 0) then
         rc = swe_keep_refinement_flags(h, 1_c_int)
         call check(rc, 'swe_keep_refinement_flags')
@@ -221,6 +237,11 @@ program swe_mi355x_main
                     refine = 0
                     rc = swe_adapt(h, c_loc(refine), n_new, sweeps)
                 else
+                    if (trim(scenario) == 'tsunami') then
+                        ! |grad eta|: a jump to a neighbour spans about half a corner range
+                        rc = swe_flag_eta_jumps(h, refine_tol / 2.0_c_double, refine_tol / 16.0_c_double)
+                        call check(rc, 'swe_flag_eta_jumps')
+                    end if
                     rc = swe_adapt(h, c_null_ptr, n_new, sweeps)
                 end if
                 call check(rc, 'swe_adapt')

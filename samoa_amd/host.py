@@ -60,6 +60,16 @@ def load():
         L.swe_host_gauges_write.argtypes = [C.c_char_p, C.c_int, _dp, _dp, _dp, C.c_int]
         L.swe_host_tsunami_raster.restype = C.c_int
         L.swe_host_tsunami_raster.argtypes = [C.c_int64, _dp, C.POINTER(C.c_double)]
+        L.swe_host_tsunami_adaptive_setup.restype = C.c_int
+        L.swe_host_tsunami_adaptive_setup.argtypes = [C.c_void_p, C.c_int64, C.c_double, C.c_int,
+                                                      C.c_double, C.c_double,
+                                                      C.POINTER(C.c_int64),
+                                                      C.POINTER(C.c_double)]
+        L.swe_host_adaptive_loop.restype = C.c_int
+        L.swe_host_adaptive_loop.argtypes = [C.c_void_p, C.c_int64, C.c_int64, C.c_double,
+                                             C.c_double, C.c_int64, C.POINTER(C.c_double),
+                                             C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int64)]
         L.swe_host_mesh_from_depths.restype = C.c_int
         L.swe_host_mesh_from_depths.argtypes = [C.c_int64, _i8p, _i8p, _dp]
         _lib = L
@@ -174,6 +184,41 @@ def tsunami_raster(seed: int) -> dict:
     got = load().swe_host_tsunami_raster(int(seed), values, C.byref(dx))
     assert got == g
     return dict(values=values.reshape(g, g), x0=0.0, y0=0.0, dx=dx.value, dy=dx.value, offset=0.0)
+
+
+def tsunami_adaptive_setup(op, seed: int, tol: float, dmax: int, drytol: float,
+                           courant: float):
+    """The Fortran host's adaptive tsunami set-up (swe_driver.f90
+    tsunami_adaptive_setup) on an EulerTimestep holding a uniform mesh:
+    initial refinement where the source varies by more than `tol` across a
+    cell, scenario state, bathymetry raster.  Returns (cells, dt0)."""
+    from . import _native
+
+    n, dt0 = C.c_int64(), C.c_double()
+    rc = load().swe_host_tsunami_adaptive_setup(op._h, int(seed), float(tol), int(dmax),
+                                                float(drytol), float(courant), C.byref(n),
+                                                C.byref(dt0))
+    _native.check("tsunami_adaptive_setup", rc)
+    op.n = n.value
+    op.mesh = None
+    return n.value, dt0.value
+
+
+def adaptive_loop(op, nsteps: int, nadapt: int, refine_tol: float, coarsen_tol: float, k0: int,
+                  dt: float, time: float, forward: bool):
+    """The Fortran host's time loop with remeshing (swe_driver.f90
+    adaptive_loop).  Returns (dt of the next step, time, forward, cells
+    stepped in total)."""
+    from . import _native
+
+    d, t, f, cells = C.c_double(dt), C.c_double(time), C.c_int(int(bool(forward))), C.c_int64()
+    rc = load().swe_host_adaptive_loop(op._h, int(nsteps), int(nadapt), float(refine_tol),
+                                       float(coarsen_tol), int(k0), C.byref(d), C.byref(t),
+                                       C.byref(f), C.byref(cells))
+    _native.check("adaptive_loop", rc)
+    op.n = int(op._lib.swe_cells(op._h))
+    op.mesh = None
+    return d.value, t.value, bool(f.value), cells.value
 
 
 def initial_dt(mesh: Mesh, state, courant: float, g: float = G) -> float:

@@ -159,3 +159,35 @@ def test_cli_remeshing_run_identical_to_reference_files(name):
                 v = po.read_vtu(path)
                 pytest.fail(f"output {i} differs: {len(v['depth'])} cells, time {v['time'][0]!r}, "
                             f"depths {np.bincount(v['depth'])}")
+
+
+def test_cli_adaptive_tsunami_equals_fortran_loop_from_python():
+    """The adaptive tsunami through the CLI (initial refinement by the source's
+    variation, remeshing before every step from surface-elevation jumps)
+    equals, bit for bit, the same run of the Fortran host's set-up and loop
+    driven from Python (swe_driver.f90 tsunami_adaptive_setup, adaptive_loop)."""
+    tol, d0, d1, steps = 2e-2, 10, 14, 12
+    f32 = lambda x: float(np.float32(x))  # noqa: E731  (the CLI reads REAL)
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [host.CLI, "-scenario", "tsunami", "-dstart", str(d0), "-dmin", str(d0), "-dmax",
+               str(d1), "-refine_tol", str(tol), "-nadapt", "1", "-nmax", str(steps), "-nout",
+               str(steps), "-tmax", "-1", "-dump", os.path.join(td, "s")]
+        print("case:", " ".join(cmd[1:]), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        t_cli, s_cli = read_dump(os.path.join(td, "s_1.bin"))
+        _, s0 = read_dump(os.path.join(td, "s_0.bin"))
+    mesh = host.mesh_uniform(d0)
+    op = swe.EulerTimestep(mesh, scaling=1e6, dry_tolerance=f32(0.01), courant=f32(0.45),
+                           min_depth=d0, max_depth=d1)
+    z = np.zeros(mesh.n)
+    op.upload(z, z, z, z)
+    n, dt0 = host.tsunami_adaptive_setup(op, 0, tol, d1, f32(0.01), f32(0.45))
+    assert n == s0.shape[1] > mesh.n
+    assert np.array_equal(np.stack(op.download()), s0)
+    # the CLI's initial output flipped the direction once
+    dt, t, fwd, cells = host.adaptive_loop(op, steps, 1, tol / 2, tol / 16, 0, dt0, 0.0, True)
+    assert t == t_cli
+    assert op.n == s_cli.shape[1]
+    assert np.array_equal(np.stack(op.download()), s_cli)
+    op.close()
