@@ -196,6 +196,28 @@ int swe_flag_eta_jumps(swe_handle *h, double refine_tol, double coarsen_tol);
  * the device for swe_adapt(h, NULL, ...). */
 int swe_keep_refinement_flags(swe_handle *h, int on);
 
+/* The record of the last swe_adapt, for the adjoint of a run with
+ * remeshing (no counterpart in the reference): code[n_old] (0 keep, 1 first
+ * and 2 second of a merging pair, 4 + pattern for a split into 2-4 cells)
+ * and base[n_old] (the first new cell of each old cell in its low 32 bits),
+ * device addresses of the caller's; *n_old the old cell count, *b_sampled
+ * whether split cells sampled the bathymetry raster.  NULL code and base
+ * return the sizes only.  Asynchronous on `stream` (NULL: the handle's). */
+int swe_adapt_record_device(swe_handle *h, int8_t *code, int64_t *base, int64_t *n_old,
+                            int *b_sampled, void *stream);
+
+/* Transpose of a remeshing's state transfer (no counterpart in the
+ * reference): from the adjoint of the new state (leta, lhu, lhv, gb over
+ * the new cells) to that of the old state (over the n_old old cells): a
+ * split's children sum into the parent, each cell of a merged pair takes
+ * half of the parent's.  gb_out accumulates (+=); split cells that sampled
+ * the raster (b_sampled) pass nothing to the parent's b.  Device pointers,
+ * asynchronous on `stream` (a hipStream_t). */
+int swe_transfer_adjoint_device(int64_t n_old, const int8_t *code, const int64_t *base,
+                                int b_sampled, const double *leta, const double *lhu,
+                                const double *lhv, const double *gb, double *leta_out,
+                                double *lhu_out, double *lhv_out, double *gb_out, void *stream);
+
 /* Cells of the handle's current mesh (changes with swe_adapt); -1 for NULL. */
 int64_t swe_cells(swe_handle *h);
 

@@ -61,6 +61,8 @@ SIGNATURES = {
     "swe_flag_eta_jumps": (_I, [_P, _D, _D]),
     "swe_set_bathymetry_raster": (_I, [_P, _I64, _I64, _D, _D, _D, _D, _D, _D, _P]),
     "swe_cells": (_I64, [_P]),
+    "swe_adapt_record_device": (_I, [_P, _P, _P, C.POINTER(_I64), C.POINTER(_I), _P]),
+    "swe_transfer_adjoint_device": (_I, [_I64, _P, _P, _I] + [_P] * 8 + [_P]),
     "swe_download_mesh": (_I, [_P, _P, _P, _P]),
     "swe_destroy": (_I, [_P]),
     "swe_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),

@@ -456,6 +456,56 @@ __global__ __launch_bounds__(AT) void link_kernel(AdaptArgs a) {
     }
 }
 
+// Transpose of the state transfer of a remeshing (a linear map): a split
+// copied the parent to its children, so the parent's adjoint is the sum of
+// the children's; a merge averaged a pair, so each of the pair gets half of
+// the parent's.  b follows the same map, except that split cells which
+// sampled the raster do not depend on the parent's b.  gb_old accumulates.
+__global__ __launch_bounds__(AT) void transfer_adjoint_kernel(
+    int64_t n_old, const int8_t *code, const int64_t *base, int b_sampled, const double *le,
+    const double *lu, const double *lv, const double *gb, double *le_o, double *lu_o,
+    double *lv_o, double *gb_o) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (c >= n_old) return;
+    const int8_t k = code[c];
+    const int64_t b0 = base[c] & 0xffffffffLL;
+    if (k == CODE_KEEP) {
+        le_o[c] = le[b0];
+        lu_o[c] = lu[b0];
+        lv_o[c] = lv[b0];
+        gb_o[c] += gb[b0];
+    } else if (k == CODE_MERGE_FIRST || k == CODE_MERGE_SECOND) {
+        const int64_t par = k == CODE_MERGE_FIRST ? b0 : b0 - 1;
+        le_o[c] = 0.5 * le[par];
+        lu_o[c] = 0.5 * lu[par];
+        lv_o[c] = 0.5 * lv[par];
+        gb_o[c] += 0.5 * gb[par];
+    } else {
+        const int pat = k - CODE_SPLIT;
+        const int cells = 2 + (pat & 1) + (pat >> 1);
+        double se = 0.0, su = 0.0, sv = 0.0, sb = 0.0;
+        for (int j = 0; j < cells; ++j) {
+            se += le[b0 + j];
+            su += lu[b0 + j];
+            sv += lv[b0 + j];
+            sb += gb[b0 + j];
+        }
+        le_o[c] = se;
+        lu_o[c] = su;
+        lv_o[c] = sv;
+        if (!b_sampled) gb_o[c] += sb;
+    }
+}
+
+// code 0 and base = c: the record of a remeshing that changed nothing
+__global__ __launch_bounds__(AT) void identity_record_kernel(int64_t n, int8_t *code,
+                                                             int64_t *base) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (c >= n) return;
+    code[c] = CODE_KEEP;
+    base[c] = c;
+}
+
 inline unsigned blocks_for(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
 hipError_t scan_i64(const int64_t *in, int64_t *out, int64_t n, int64_t *bsum, hipStream_t s) {
@@ -597,6 +647,20 @@ hipError_t flag_eta_jumps(int64_t n, const double *eta, const double *b, const i
                           double coarsen_tol, int dmin, int dmax, int8_t *flags, hipStream_t s) {
     eta_jump_kernel<<<blocks_for(n, AT), AT, 0, s>>>(n, eta, b, nbr, depth, drytol, refine_tol,
                                                     coarsen_tol, dmin, dmax, flags);
+    return hipGetLastError();
+}
+
+hipError_t transfer_adjoint(int64_t n_old, const int8_t *code, const int64_t *base, int b_sampled,
+                            const double *le, const double *lu, const double *lv, const double *gb,
+                            double *le_o, double *lu_o, double *lv_o, double *gb_o,
+                            hipStream_t s) {
+    transfer_adjoint_kernel<<<blocks_for(n_old, AT), AT, 0, s>>>(n_old, code, base, b_sampled, le,
+                                                                lu, lv, gb, le_o, lu_o, lv_o, gb_o);
+    return hipGetLastError();
+}
+
+hipError_t identity_record(int64_t n, int8_t *code, int64_t *base, hipStream_t s) {
+    identity_record_kernel<<<blocks_for(n, AT), AT, 0, s>>>(n, code, base);
     return hipGetLastError();
 }
 

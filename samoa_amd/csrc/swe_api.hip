@@ -230,6 +230,8 @@ struct Batch {
     int64_t *ad_host = nullptr; // [8] pinned results of the remesher
     int ad_last_sweeps = 3;     // sweeps the last remeshing needed
     bool ad_last_changed = false;  // the last remeshing changed the mesh
+    int64_t ad_record_n = -1;      // old cells of the last remeshing (its record in ad_code/ad_base)
+    bool ad_record_sampled = false;  // its split cells sampled the raster
     double *raster = nullptr;  // bathymetry raster for split cells (swe_set_bathymetry_raster)
     int64_t rnx = 0, rny = 0;
     double rx0 = 0, ry0 = 0, rdx = 1, rdy = 1, roffset_x = 0, roffset_y = 0;
@@ -705,7 +707,10 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     if (n_new <= 0 || n_new > (int64_t(1) << 29))
         return fail(SWE_ERR_ARG, "adapt: the new mesh would have %lld cells (1..2^29)",
                     (long long)n_new);
+    B.ad_record_n = n;
+    B.ad_record_sampled = B.raster != nullptr;
     if (!(plan.status & 2u)) {  // every cell keeps its place: the mesh and state stand
+        B.ad_record_n = -n;     // an identity record
         B.flags_valid = false;
         if (n_out) *n_out = n;
         if (sweeps_out) *sweeps_out = sweeps;
@@ -1041,6 +1046,40 @@ int swe_flag_eta_jumps(swe_handle *h, double refine_tol, double coarsen_tol) {
     HIP_TRY(swe::flag_eta_jumps(B.ncells, B.eta[B.cur], B.b, B.nbr, B.depth, sc.drytol,
                                 refine_tol, coarsen_tol, sc.dmin, sc.dmax, B.refine, B.stream));
     B.flags_valid = true;
+    return SWE_OK;
+}
+
+int swe_adapt_record_device(swe_handle *h, int8_t *code, int64_t *base, int64_t *n_old,
+                            int *b_sampled, void *stream) {
+    if (!h || !n_old) return fail(SWE_ERR_ARG, "swe_adapt_record_device: null argument");
+    Batch &B = h->B;
+    if (B.ad_record_n == -1)
+        return fail(SWE_ERR_STATE, "swe_adapt_record_device: no remeshing yet");
+    const int64_t n = B.ad_record_n < 0 ? -B.ad_record_n : B.ad_record_n;
+    *n_old = n;
+    if (b_sampled) *b_sampled = B.ad_record_sampled ? 1 : 0;
+    if (!code && !base) return SWE_OK;
+    if (!code || !base) return fail(SWE_ERR_ARG, "swe_adapt_record_device: code and base go together");
+    hipStream_t st = stream ? (hipStream_t)stream : B.stream;
+    if (B.ad_record_n < 0) {
+        HIP_TRY(swe::identity_record(n, code, base, st));
+    } else {
+        HIP_TRY(hipMemcpyAsync(code, B.ad_code, (size_t)n, hipMemcpyDeviceToDevice, st));
+        HIP_TRY(hipMemcpyAsync(base, B.ad_base, sizeof(int64_t) * (size_t)n,
+                               hipMemcpyDeviceToDevice, st));
+    }
+    return SWE_OK;
+}
+
+int swe_transfer_adjoint_device(int64_t n_old, const int8_t *code, const int64_t *base,
+                                int b_sampled, const double *leta, const double *lhu,
+                                const double *lhv, const double *gb, double *leta_out,
+                                double *lhu_out, double *lhv_out, double *gb_out, void *stream) {
+    if (n_old <= 0 || !code || !base || !leta || !lhu || !lhv || !gb || !leta_out || !lhu_out ||
+        !lhv_out || !gb_out)
+        return fail(SWE_ERR_ARG, "swe_transfer_adjoint_device: invalid argument");
+    HIP_TRY(swe::transfer_adjoint(n_old, code, base, b_sampled, leta, lhu, lhv, gb, leta_out,
+                                  lhu_out, lhv_out, gb_out, (hipStream_t)stream));
     return SWE_OK;
 }
 

@@ -134,6 +134,13 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
 hipError_t flag_eta_jumps(int64_t n, const double *eta, const double *b, const int32_t *nbr,
                           const int8_t *depth, double drytol, double refine_tol,
                           double coarsen_tol, int dmin, int dmax, int8_t *flags, hipStream_t s);
+// Transpose of a remeshing's state transfer (code, base: its record).
+hipError_t transfer_adjoint(int64_t n_old, const int8_t *code, const int64_t *base, int b_sampled,
+                            const double *le, const double *lu, const double *lv, const double *gb,
+                            double *le_o, double *lu_o, double *lv_o, double *gb_o,
+                            hipStream_t s);
+// The record of a remeshing that changed nothing.
+hipError_t identity_record(int64_t n, int8_t *code, int64_t *base, hipStream_t s);
 // Writes the new mesh and state (outputs sized for n_new).
 hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s);
 
