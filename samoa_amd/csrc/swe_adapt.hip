@@ -119,37 +119,74 @@ __global__ __launch_bounds__(AT) void conf_init_kernel(int64_t n, const int8_t *
     slots[c] = legs | (hyp << 2) | (legs << 4);
 }
 
-// mark_edges_integrity (Conformity.f90:1271-1317), in place.  *changed gets
-// the sweep's number if a word changed, *acting if a cell's word asks for a
-// split (hypotenuse refined) or a merge (legs coarsened): in a sweep that
-// changes nothing, those are the cells the remeshing will split or merge.
+// The word the integrity rule gives a cell from the values (bit 0 refine,
+// bit 1 coarsen) of its three edges: a refined edge refines the hypotenuse,
+// and the legs stay coarsened only if both are and nothing is refined.
+__device__ __forceinline__ uint32_t integrity_word(uint32_t v0, uint32_t v1, uint32_t v2, int d) {
+    const uint32_t rh = (v0 | v1 | v2) & 1u;
+    const uint32_t cc = (!rh && (v0 & v2 & 2u) && d > 0) ? 2u : 0u;
+    return cc | (rh << 2) | (cc << 4);
+}
+
+// mark_edges_integrity (Conformity.f90:1271-1317), in place.  A workgroup
+// iterates its AT cells to a fixpoint of their own in LDS (a neighbour in
+// the tile is read there, any other from memory every round) before it
+// writes them back, so chains of forced refinements inside a tile settle in
+// one sweep.  Every update is monotone, which keeps the global fixpoint the
+// same.  *changed gets the sweep's number if a word changed.
 __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t *depth,
                                                         const int32_t *nbr, uint32_t *slots,
-                                                        uint32_t *changed, uint32_t *acting,
-                                                        uint32_t sweep) {
-    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
-    bool diff = false, act = false;
-    if (c < n) {
-        const uint32_t own = ld_relaxed(&slots[c]);
-        const uint32_t v0 = edge_value(slots, nbr, c, own, 0);
-        const uint32_t v1 = edge_value(slots, nbr, c, own, 1);
-        const uint32_t v2 = edge_value(slots, nbr, c, own, 2);
-        const uint32_t rh = (v0 | v1 | v2) & 1u;
-        const uint32_t cc = (!rh && (v0 & v2 & 2u) && depth[c] > 0) ? 2u : 0u;
-        const uint32_t nw = cc | (rh << 2) | (cc << 4);
-        if (nw != own) {
-            st_relaxed(&slots[c], nw);
-            diff = true;
-        }
-        act = rh || cc;
+                                                        uint32_t *changed, uint32_t sweep) {
+    __shared__ uint32_t w[AT];
+    const int64_t t0 = blockIdx.x * (int64_t)AT;
+    const int64_t c = t0 + threadIdx.x;
+    const bool live = c < n;
+    int32_t q[3] = {-1, -1, -1};
+    int d = 0;
+    uint32_t own = 0u;
+    if (live) {
+        own = ld_relaxed(&slots[c]);
+        q[0] = nbr[3 * c];
+        q[1] = nbr[3 * c + 1];
+        q[2] = nbr[3 * c + 2];
+        d = depth[c];
     }
+    const uint32_t start = own;
+    w[threadIdx.x] = own;
+    __syncthreads();
+    // each round settles at least one more cell of a chain; a tile that has
+    // not settled after 32 rounds leaves the rest to the next sweep
+    for (int round = 0; round < 32; ++round) {
+        uint32_t nw = own;
+        if (live) {
+            uint32_t v[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const uint32_t sk = (own >> (2 * k)) & 3u;
+                v[k] = sk;
+                if (q[k] >= 0) {
+                    const int64_t o = q[k] >> 2;
+                    const uint32_t ow = (o >= t0 && o < t0 + AT) ? w[o - t0] : ld_relaxed(&slots[o]);
+                    const uint32_t tk = (ow >> (2 * (q[k] & 3))) & 3u;
+                    v[k] = ((sk | tk) & 1u) | (sk & tk & 2u);
+                }
+            }
+            nw = integrity_word(v[0], v[1], v[2], d);
+        }
+        const bool moved = nw != own;
+        __syncthreads();  // every read of this round is done
+        if (moved) {
+            w[threadIdx.x] = nw;
+            own = nw;
+        }
+        if (!__syncthreads_or(moved)) break;
+    }
+    const bool diff = own != start;
+    if (diff) st_relaxed(&slots[c], own);
     // one status write per workgroup, and only if no other has written it:
     // stores from every wave to one word contend across the whole device
-    const int any_diff = __syncthreads_or(diff), any_act = __syncthreads_or(act);
-    if (threadIdx.x == 0) {
-        if (any_diff && ld_relaxed(changed) != sweep) st_relaxed(changed, sweep);
-        if (any_act && ld_relaxed(acting) != sweep) st_relaxed(acting, sweep);
-    }
+    if (__syncthreads_or(diff) && threadIdx.x == 0 && ld_relaxed(changed) != sweep)
+        st_relaxed(changed, sweep);
 }
 
 __device__ __forceinline__ int64_t wave_incl_scan(int64_t v) {
@@ -242,42 +279,97 @@ __global__ __launch_bounds__(AT) void tree_size_kernel(int64_t n, const int8_t *
     if (c < n) size[c] = (int64_t)1 << (60 - depth[c]);
 }
 
-// Decision and new-cell count of every cell.  status bit 0: a coarsening cell
-// whose sibling is not a coarsening leaf (cannot happen after conformity).
-__global__ __launch_bounds__(AT) void count_kernel(int64_t n, const int8_t *depth,
-                                                   const int32_t *nbr, const uint32_t *slots,
-                                                   const int64_t *off, int8_t *code,
-                                                   int64_t *count, uint32_t *status) {
-    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
-    if (c >= n) return;
-    const uint32_t own = slots[c];
-    const uint32_t v0 = edge_value(slots, nbr, c, own, 0);
-    const uint32_t v1 = edge_value(slots, nbr, c, own, 1);
-    const uint32_t v2 = edge_value(slots, nbr, c, own, 2);
-    const int d = depth[c];
-    int8_t k = CODE_KEEP;
-    int64_t m = 1;
-    if (v1 & 1u) {
-        const int pat = (int)(v2 & 1u) | ((int)(v0 & 1u) << 1);
-        k = (int8_t)(CODE_SPLIT + pat);
-        m = 2 + (v2 & 1u) + (v0 & 1u);
-    } else if (v0 & 2u) {
-        // d > 0 here: the sweep clears the coarsen bits of root cells
-        const int64_t parent = (int64_t)1 << (61 - d);
-        const bool first = off[c] % parent == 0;
-        const int64_t sib = first ? c + 1 : c - 1;
-        bool ok = sib >= 0 && sib < n && depth[sib] == d;
-        if (ok) {
-            const uint32_t so = slots[sib];
-            ok = (edge_value(slots, nbr, sib, so, 0) & 2u) && !(edge_value(slots, nbr, sib, so, 1) & 1u);
+// New cells of an old cell from its decision code, with the changing-cell
+// count in the high word (a split makes 2..4 cells, the first of a merging
+// pair 1, the second 0).
+__device__ __forceinline__ int64_t code_count(int k) {
+    const int64_t m = k >= CODE_SPLIT ? 2 + (k & 1) + ((k >> 1) & 1) : (k == CODE_MERGE_SECOND ? 0 : 1);
+    return m | ((int64_t)(k != CODE_KEEP) << 32);
+}
+
+// Decision of every cell, and the sum of the new-cell counts of each scan
+// tile.  It also checks the fixpoint the sweeps reached: status bit 2 if the
+// integrity rule would still change a word (more sweeps are needed, and
+// nothing else here is valid), bit 0 if a coarsening cell's sibling is not a
+// coarsening leaf (cannot happen at the fixpoint).
+__global__ __launch_bounds__(SCAN_T) void count_reduce_kernel(int64_t n, const int8_t *depth,
+                                                              const int32_t *nbr,
+                                                              const uint32_t *slots,
+                                                              const int64_t *off, int8_t *code,
+                                                              int64_t *bsum, uint32_t *status) {
+    __shared__ int64_t lds[SCAN_T / 64 + 1];
+    const int64_t tile = blockIdx.x * (int64_t)SCAN_TILE;
+    int64_t sum = 0;
+    bool moved = false, bad = false;
+#pragma unroll 2
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const int64_t c = tile + j * SCAN_T + threadIdx.x;
+        if (c >= n) break;
+        const uint32_t own = slots[c];
+        const uint32_t v0 = edge_value(slots, nbr, c, own, 0);
+        const uint32_t v1 = edge_value(slots, nbr, c, own, 1);
+        const uint32_t v2 = edge_value(slots, nbr, c, own, 2);
+        const int d = depth[c];
+        moved |= integrity_word(v0, v1, v2, d) != own;
+        int k = CODE_KEEP;
+        if (v1 & 1u) {
+            k = CODE_SPLIT + ((int)(v2 & 1u) | ((int)(v0 & 1u) << 1));
+        } else if (v0 & 2u) {
+            // d > 0 here: the integrity rule clears the coarsen bits of root cells
+            const int64_t parent = (int64_t)1 << (61 - d);
+            const bool first = off[c] % parent == 0;
+            const int64_t sib = first ? c + 1 : c - 1;
+            bool ok = sib >= 0 && sib < n && depth[sib] == d;
+            if (ok) {
+                const uint32_t so = slots[sib];
+                ok = (edge_value(slots, nbr, sib, so, 0) & 2u) &&
+                     !(edge_value(slots, nbr, sib, so, 1) & 1u);
+            }
+            bad |= !ok;
+            k = first ? CODE_MERGE_FIRST : CODE_MERGE_SECOND;
         }
-        if (!ok) atomicOr(status, 1u);
-        k = first ? CODE_MERGE_FIRST : CODE_MERGE_SECOND;
-        m = first ? 1 : 0;
+        code[c] = (int8_t)k;
+        sum += code_count(k);
     }
-    code[c] = k;
-    // the high word counts the cells that change, through the same scan
-    count[c] = m | ((int64_t)(k != CODE_KEEP) << 32);
+    int64_t total;
+    (void)block_excl_scan<SCAN_T>(sum, lds, &total);
+    const uint32_t err = (__syncthreads_or(moved) ? 4u : 0u) | (__syncthreads_or(bad) ? 1u : 0u);
+    if (threadIdx.x == 0) {
+        bsum[blockIdx.x] = total;
+        if (err) atomicOr(status, err);
+    }
+}
+
+// base[c] = exclusive prefix of the counts, rebuilt from the decision codes
+// (8 consecutive cells per thread, bsum holding the scanned tile sums).
+__global__ __launch_bounds__(SCAN_T) void scan_codes_kernel(int64_t n, const int8_t *code,
+                                                            const int64_t *bsum, int64_t *base) {
+    __shared__ int64_t lds[SCAN_T / 64 + 1];
+    const int64_t first = blockIdx.x * (int64_t)SCAN_TILE + threadIdx.x * (int64_t)SCAN_ITEMS;
+    int64_t v[SCAN_ITEMS];
+    int64_t sum = 0;
+    if (first + SCAN_ITEMS <= n) {
+        const uint64_t packed = *reinterpret_cast<const uint64_t *>(code + first);
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            v[j] = code_count((int)(int8_t)(packed >> (8 * j)));
+            sum += v[j];
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < SCAN_ITEMS; ++j) {
+            v[j] = first + j < n ? code_count(code[first + j]) : 0;
+            sum += v[j];
+        }
+    }
+    int64_t total;
+    int64_t run = bsum[blockIdx.x] + block_excl_scan<SCAN_T>(sum, lds, &total);
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j)
+        if (first + j < n) {
+            base[first + j] = run;
+            run += v[j];
+        }
 }
 
 __device__ __forceinline__ int split_type(int p, int rule) {
@@ -560,13 +652,11 @@ __global__ void results_kernel(int64_t *res, const int64_t *bsum, int64_t tiles,
         res[5] = bsum[tiles] >> 32;
     }
 }
-// res[3] = the last sweep that changed a word, res[4] = the last sweep that
-// saw a cell ask for a split or merge; clears the error bits for the counts
-// that follow
+// res[3] = the last sweep that changed a word; clears the error bits for the
+// count pass that follows
 __global__ void sweep_results_kernel(int64_t *res, uint32_t *status) {
     if (threadIdx.x == 0) {
         res[3] = status[1];
-        res[4] = status[2];
         status[0] = 0u;
     }
 }
@@ -576,16 +666,11 @@ __global__ void sweep_results_kernel(int64_t *res, uint32_t *status) {
 int64_t adapt_scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
-                      bool count_ahead, int max_sweeps, int64_t *host, AdaptPlan *plan) {
+                      int max_sweeps, int64_t *host, AdaptPlan *plan) {
     const unsigned nb = blocks_for(a.n, AT);
     const int64_t tiles = adapt_scan_tiles(a.n);
     int64_t *res = a.bsum + tiles + 1;
     hipError_t e;
-    auto fetch = [&](int first, int count) -> hipError_t {
-        hipError_t r = hipMemcpyAsync(host + first, res + first, count * sizeof(int64_t),
-                                      hipMemcpyDeviceToHost, s);
-        return r != hipSuccess ? r : hipStreamSynchronize(s);
-    };
     conf_init_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.flags, a.dmin, a.dmax, a.slots, a.status);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (!offsets_valid) {  // else the offsets of the unchanged mesh are in place
@@ -593,51 +678,38 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
         if ((e = scan_i64(a.off, a.off, a.n, a.bsum, s)) != hipSuccess) return e;
         copy_total_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles);
     }
-    // sweeps go in batches between host checks: a batch whose last sweep
-    // changed nothing has reached the fixpoint (further sweeps are no-ops)
-    // the counts behind the sweeps: valid when the batch's last sweep changed
-    // nothing, which is when they are read
-    auto counts = [&]() -> hipError_t {
-        count_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.off, a.code, a.base,
-                                       a.status);
-        hipError_t r = hipGetLastError();
-        if (r != hipSuccess) return r;
-        if ((r = scan_i64(a.base, a.base, a.n, a.bsum, s)) != hipSuccess) return r;
-        results_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles, a.status);
-        return hipGetLastError();
-    };
+    // Sweeps go in batches, each followed by the count pass and one host
+    // read: the count pass also tells whether the words are at the fixpoint,
+    // so no sweep is spent on confirming it.
     plan->sweeps = -1;
-    int batch = first_batch < 2 ? 2 : first_batch, done = 0;
-    bool counted = false;
-    for (; done < max_sweeps; done += batch, batch = 4) {
+    int batch = first_batch < 1 ? 1 : first_batch, done = 0;
+    while (done < max_sweeps) {
         for (int j = 1; j <= batch; ++j)
             conf_sweep_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.status + 1,
-                                                a.status + 2, (uint32_t)(done + j));
+                                                (uint32_t)(done + j));
+        done += batch;
         sweep_results_kernel<<<1, 64, 0, s>>>(res, a.status);
+        if (tiles > 0)
+            count_reduce_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(
+                a.n, a.depth, a.nbr, a.slots, a.off, a.code, a.bsum, a.status);
+        scan_top_kernel<<<1, TOP_T, 0, s>>>(a.bsum, tiles);
+        results_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles, a.status);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if (count_ahead && (e = counts()) != hipSuccess) return e;
-        if ((e = fetch(0, 6)) != hipSuccess) return e;
-        if (host[3] < done + batch) {
-            plan->sweeps = (int)host[3] + 1;
-            done += batch;
-            counted = count_ahead;
+        if ((e = hipMemcpyAsync(host, res, 6 * sizeof(int64_t), hipMemcpyDeviceToHost, s)) !=
+            hipSuccess)
+            return e;
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        if (!(host[2] & 4)) {
+            plan->sweeps = done;
             break;
         }
+        batch = 2;
     }
     if (plan->sweeps < 0) return hipSuccess;
+    plan->last_change = (int)host[3];
     plan->tree_total = offsets_valid ? (int64_t(1) << 61) : host[0];
-    if (host[4] < done) {  // no cell splits or merges: the mesh stands
-        plan->n_new = a.n;
-        plan->n_changing = 0;
-        plan->status = 0;
-        return hipSuccess;
-    }
-    if (!counted) {
-        if ((e = counts()) != hipSuccess) return e;
-        if ((e = fetch(1, 5)) != hipSuccess) return e;
-    }
     plan->n_new = host[1];
-    plan->status = (uint32_t)host[2];
+    plan->status = (uint32_t)host[2] & 3u;
     plan->n_changing = host[5];
     return hipSuccess;
 }
@@ -665,6 +737,8 @@ hipError_t identity_record(int64_t n, int8_t *code, int64_t *base, hipStream_t s
 }
 
 hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s) {
+    const int64_t tiles = adapt_scan_tiles(a.n);
+    if (tiles > 0) scan_codes_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(a.n, a.code, a.bsum, a.base);
     emit_kernel<<<blocks_for(a.n, AT), AT, 0, s>>>(a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.n_changing == 0) return e;

@@ -228,8 +228,7 @@ struct Batch {
     int64_t ad_cap = 0;
     bool ad_off_valid = false;  // ad_off holds the current mesh's tree offsets
     int64_t *ad_host = nullptr; // [8] pinned results of the remesher
-    int ad_last_sweeps = 3;     // sweeps the last remeshing needed
-    bool ad_last_changed = false;  // the last remeshing changed the mesh
+    int ad_last_sweeps = 2;     // sweeps that changed a word in the last remeshing
     int64_t ad_record_n = -1;      // old cells of the last remeshing (its record in ad_code/ad_base)
     bool ad_record_sampled = false;  // its split cells sampled the raster
     double *raster = nullptr;  // bathymetry raster for split cells (swe_set_bathymetry_raster)
@@ -690,12 +689,12 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.offset_x = B.roffset_x;
     a.offset_y = B.roffset_y;
     swe::AdaptPlan plan{};
-    HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, B.ad_last_changed,
-                            1 << 16, B.ad_host, &plan));
-    B.ad_last_changed = (plan.status & 2u) != 0;
+    HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, 1 << 16, B.ad_host,
+                            &plan));
     const int sweeps = plan.sweeps;
     if (sweeps < 0) return fail(SWE_ERR_STATE, "adapt: the conformity sweeps did not converge");
-    B.ad_last_sweeps = sweeps;
+    // the next call starts with as many sweeps as changed a word this time
+    B.ad_last_sweeps = plan.last_change > 1 ? plan.last_change : 1;
     const int64_t n_new = plan.n_new, tree = plan.tree_total;
     const uint32_t status = plan.status & 1u;
     if (tree != (int64_t(1) << 61))

@@ -92,7 +92,7 @@ struct AdaptArgs {
     int dmin, dmax;
     uint32_t *slots;                       // [n] conformity words
     int64_t *off;                          // [n] tree offsets
-    int64_t *base;                         // [n] new-cell counts, then their prefix
+    int64_t *base;                         // [n] prefix of the new-cell counts
     int64_t *bsum;                         // [adapt_scan_tiles(n) + 1 + 8]
     int8_t *code;                          // [n]
     int32_t *eslot;                        // [6n] new cell-edge on each half of each old edge
@@ -115,21 +115,22 @@ struct AdaptArgs {
 // bsum holds adapt_scan_tiles(n) + 1 scan words and 8 result words.
 int64_t adapt_scan_tiles(int64_t n);
 struct AdaptPlan {
-    int sweeps;          // conformity sweeps to the fixpoint, -1 if not reached in max_sweeps
+    int sweeps;          // conformity sweeps run, -1 if the fixpoint was not reached in max_sweeps
+    int last_change;     // the last of them that changed a word (0: none did)
     int64_t n_new;       // cells of the new mesh
     int64_t tree_total;  // covered part of the two root trees (2^61 when the mesh is whole)
     uint32_t status;     // bit 0: an inconsistent merge; bit 1: the mesh changes
     int64_t n_changing;  // cells that split or merge
 };
-// Marks the edges, sweeps to the conformity fixpoint (first_batch sweeps
-// between host checks, then 4), then counts the new cells and their prefix
-// (base; its high word counts the cells that change) unless no cell splits
-// or merges.  offsets_valid: a.off still holds this mesh's tree offsets.
-// count_ahead: count behind every batch of sweeps, before the host knows
-// whether the mesh changes (one round trip instead of two when it does).
-// host: 8 words of pinned memory for the results.  Synchronises the stream.
+// Marks the edges, sweeps to the conformity fixpoint (first_batch sweeps,
+// then 2 at a time), decides every cell (code) and sums the new-cell counts
+// per scan tile; the count pass confirms the fixpoint.  offsets_valid: a.off
+// still holds this mesh's tree offsets.  host: 8 words of pinned memory for
+// the results.  Synchronises the stream once per batch.  adapt_emit then
+// writes the prefix of the counts (base; its high word counts the cells that
+// change) and the new mesh.
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
-                      bool count_ahead, int max_sweeps, int64_t *host, AdaptPlan *plan);
+                      int max_sweeps, int64_t *host, AdaptPlan *plan);
 // Refinement flags from surface-elevation jumps to wet neighbours.
 hipError_t flag_eta_jumps(int64_t n, const double *eta, const double *b, const int32_t *nbr,
                           const int8_t *depth, double drytol, double refine_tol,
