@@ -32,6 +32,8 @@
 // at corner (k+1) mod 3, half 1 the half at corner (k+2) mod 3.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+
 #include "swe_internal.hpp"
 
 namespace swe {
@@ -102,13 +104,15 @@ __device__ __forceinline__ uint32_t edge_value(const uint32_t *slots, const int3
 // volume_op (SWE_euler_timestep.f90:339-342) never issues them.
 __global__ __launch_bounds__(AT) void conf_init_kernel(int64_t n, const int8_t *depth,
                                                        const int8_t *flags, int dmin, int dmax,
-                                                       uint32_t *slots, uint32_t *status) {
+                                                       uint32_t *slots, uint32_t *status,
+                                                       uint32_t *stamp) {
     const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
     if (c == 0) {
         status[0] = 0u;
         status[1] = 0u;
         status[2] = 0u;
     }
+    if (threadIdx.x == 0) stamp[blockIdx.x] = 0u;  // the sweeps' tiles are these workgroups
     if (c >= n) return;
     int f = flags[c];
     const int d = depth[c];
@@ -128,16 +132,29 @@ __device__ __forceinline__ uint32_t integrity_word(uint32_t v0, uint32_t v1, uin
     return cc | (rh << 2) | (cc << 4);
 }
 
-// mark_edges_integrity (Conformity.f90:1271-1317), in place.  A workgroup
-// iterates its AT cells to a fixpoint of their own in LDS (a neighbour in
-// the tile is read there, any other from memory every round) before it
-// writes them back, so chains of forced refinements inside a tile settle in
-// one sweep.  Every update is monotone, which keeps the global fixpoint the
-// same.  *changed gets the sweep's number if a word changed.
+// mark_edges_integrity (Conformity.f90:1271-1317), in place, over tiles of
+// AT cells.  A workgroup iterates its tile to a fixpoint of its own in LDS (a
+// neighbour in the tile is read there, any other from memory every round)
+// before it writes the tile back, so chains of forced refinements inside a
+// tile settle in one sweep.  Every update is monotone, which keeps the global
+// fixpoint the same.
+//
+// stamp[T] is the last sweep in which a cell next to tile T but outside it
+// changed (or T itself did not settle).  From the second sweep on, a tile
+// whose stamp is older than the previous sweep has read every neighbour word
+// since it last changed and is at its fixpoint: it is skipped.  *changed gets
+// the sweep's number if a word changed.
 __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t *depth,
                                                         const int32_t *nbr, uint32_t *slots,
-                                                        uint32_t *changed, uint32_t sweep) {
+                                                        uint32_t *stamp, uint32_t *changed,
+                                                        uint32_t sweep) {
     __shared__ uint32_t w[AT];
+    __shared__ uint32_t s_skip;
+    if (sweep > 1) {
+        if (threadIdx.x == 0) s_skip = ld_relaxed(&stamp[blockIdx.x]) + 1u < sweep ? 1u : 0u;
+        __syncthreads();
+        if (s_skip) return;  // uniform across the workgroup
+    }
     const int64_t t0 = blockIdx.x * (int64_t)AT;
     const int64_t c = t0 + threadIdx.x;
     const bool live = c < n;
@@ -155,7 +172,8 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
     w[threadIdx.x] = own;
     __syncthreads();
     // each round settles at least one more cell of a chain; a tile that has
-    // not settled after 32 rounds leaves the rest to the next sweep
+    // not settled after 32 rounds stamps itself for the next sweep
+    bool settled = false;
     for (int round = 0; round < 32; ++round) {
         uint32_t nw = own;
         if (live) {
@@ -179,10 +197,25 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
             w[threadIdx.x] = nw;
             own = nw;
         }
-        if (!__syncthreads_or(moved)) break;
+        if (!__syncthreads_or(moved)) {
+            settled = true;
+            break;
+        }
     }
     const bool diff = own != start;
-    if (diff) st_relaxed(&slots[c], own);
+    if (diff) {
+        st_relaxed(&slots[c], own);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (q[k] < 0) continue;
+            const int64_t o = q[k] >> 2;
+            if (o < t0 || o >= t0 + AT) {
+                uint32_t *st = &stamp[o / AT];
+                if (ld_relaxed(st) != sweep) st_relaxed(st, sweep);
+            }
+        }
+    }
+    if (!settled && threadIdx.x == 0) st_relaxed(&stamp[blockIdx.x], sweep);
     // one status write per workgroup, and only if no other has written it:
     // stores from every wave to one word contend across the whole device
     if (__syncthreads_or(diff) && threadIdx.x == 0 && ld_relaxed(changed) != sweep)
@@ -235,8 +268,9 @@ __global__ __launch_bounds__(SCAN_T) void scan_reduce_kernel(const int64_t *in, 
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// Exclusive scan of the nb tile sums in place; bsum[nb] = the grand total.
-__global__ __launch_bounds__(TOP_T) void scan_top_kernel(int64_t *bsum, int64_t nb) {
+// Exclusive scan of the nb tile sums in place, by one workgroup of TOP_T
+// threads; returns the grand total.
+__device__ int64_t scan_tile_sums(int64_t *bsum, int64_t nb) {
     __shared__ int64_t lds[TOP_T / 64 + 1];
     int64_t carry = 0;
     for (int64_t start = 0; start < nb; start += TOP_T) {
@@ -247,7 +281,37 @@ __global__ __launch_bounds__(TOP_T) void scan_top_kernel(int64_t *bsum, int64_t 
         if (i < nb) bsum[i] = carry + ex;
         carry += total;
     }
-    if (threadIdx.x == 0) bsum[nb] = carry;
+    return carry;
+}
+
+// bsum[0..nb) scanned in place; bsum[nb] = the grand total.
+__global__ __launch_bounds__(TOP_T) void scan_top_kernel(int64_t *bsum, int64_t nb) {
+    const int64_t total = scan_tile_sums(bsum, nb);
+    if (threadIdx.x == 0) bsum[nb] = total;
+}
+
+// The scan of the count pass's tile sums, then its results straight into
+// host memory (coherent, mapped): out[0] = res[0] (the covered part of the
+// root trees, when this call scanned the offsets), out[1] = the new cell
+// count, out[2] = status bits (0: an inconsistent merge, 1: the mesh
+// changes, 2: not at the fixpoint), out[3] = the last sweep that changed a
+// word, out[5] = the cells that change.  out[7] = seq goes last, a release at
+// system scope: a host that reads seq there reads the rest too.  Clears the
+// error bits for the next count pass.
+__global__ __launch_bounds__(TOP_T) void plan_top_kernel(int64_t *bsum, int64_t nb,
+                                                         uint32_t *status, const int64_t *res,
+                                                         int64_t *out, int64_t seq) {
+    const int64_t total = scan_tile_sums(bsum, nb);
+    if (threadIdx.x == 0) {
+        bsum[nb] = total;
+        out[0] = res[0];
+        out[1] = total & 0xffffffffLL;
+        out[2] = (int64_t)(status[0] | ((total >> 32) != 0 ? 2u : 0u));
+        out[3] = (int64_t)status[1];
+        out[5] = total >> 32;
+        status[0] = 0u;
+        __hip_atomic_store(&out[7], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 // out[i] = exclusive prefix of in (in place allowed: each thread reads its
@@ -301,10 +365,10 @@ __global__ __launch_bounds__(SCAN_T) void count_reduce_kernel(int64_t n, const i
     const int64_t tile = blockIdx.x * (int64_t)SCAN_TILE;
     int64_t sum = 0;
     bool moved = false, bad = false;
-#pragma unroll 2
+#pragma unroll
     for (int j = 0; j < SCAN_ITEMS; ++j) {
         const int64_t c = tile + j * SCAN_T + threadIdx.x;
-        if (c >= n) break;
+        if (c >= n) continue;
         const uint32_t own = slots[c];
         const uint32_t v0 = edge_value(slots, nbr, c, own, 0);
         const uint32_t v1 = edge_value(slots, nbr, c, own, 1);
@@ -341,11 +405,17 @@ __global__ __launch_bounds__(SCAN_T) void count_reduce_kernel(int64_t n, const i
 }
 
 // base[c] = exclusive prefix of the counts, rebuilt from the decision codes
-// (8 consecutive cells per thread, bsum holding the scanned tile sums).
+// (8 consecutive cells per thread, bsum holding the scanned tile sums), and
+// kidx[c] = the new index of a cell that keeps its place, -1 for one that
+// changes.  The prefixes go through LDS so that the stores are coalesced.
 __global__ __launch_bounds__(SCAN_T) void scan_codes_kernel(int64_t n, const int8_t *code,
-                                                            const int64_t *bsum, int64_t *base) {
+                                                            const int64_t *bsum, int64_t *base,
+                                                            int32_t *kidx) {
     __shared__ int64_t lds[SCAN_T / 64 + 1];
-    const int64_t first = blockIdx.x * (int64_t)SCAN_TILE + threadIdx.x * (int64_t)SCAN_ITEMS;
+    __shared__ int64_t stage[SCAN_TILE + SCAN_TILE / SCAN_ITEMS];  // item i at i + i / 8
+    __shared__ uint8_t kstage[SCAN_TILE + SCAN_TILE / SCAN_ITEMS];  // 1: the cell is kept
+    const int64_t tile = blockIdx.x * (int64_t)SCAN_TILE;
+    const int64_t first = tile + threadIdx.x * (int64_t)SCAN_ITEMS;
     int64_t v[SCAN_ITEMS];
     int64_t sum = 0;
     if (first + SCAN_ITEMS <= n) {
@@ -365,11 +435,23 @@ __global__ __launch_bounds__(SCAN_T) void scan_codes_kernel(int64_t n, const int
     int64_t total;
     int64_t run = bsum[blockIdx.x] + block_excl_scan<SCAN_T>(sum, lds, &total);
 #pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; ++j)
-        if (first + j < n) {
-            base[first + j] = run;
-            run += v[j];
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        stage[threadIdx.x * (SCAN_ITEMS + 1) + j] = run;
+        kstage[threadIdx.x * (SCAN_ITEMS + 1) + j] = v[j] == 1 ? 1 : 0;
+        run += v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const int i = j * SCAN_T + threadIdx.x;
+        if (tile + i < n) {
+            const int64_t b = stage[i + i / SCAN_ITEMS];
+            base[tile + i] = b;
+            // a kept cell counts exactly 1 (one new cell, no change); its
+            // new index is below 2^29, and -1 marks the others
+            kidx[tile + i] = kstage[i + i / SCAN_ITEMS] ? (int32_t)(b & 0xffffffffLL) : -1;
         }
+    }
 }
 
 __device__ __forceinline__ int split_type(int p, int rule) {
@@ -435,18 +517,19 @@ __device__ __forceinline__ void put_slot(int32_t *eslot, int64_t c, int k, int h
 // The new cell-edge on half s of old edge k of cell c: computed for a cell
 // that keeps its place, read from the slots emit_kernel wrote otherwise.
 __device__ __forceinline__ int32_t edge_slot(const AdaptArgs &a, int64_t c, int k, int s) {
-    if (a.code[c] == CODE_KEEP) return (int32_t)(((a.base[c] & 0xffffffffLL) << 2) | k);
+    const int32_t kc = a.kidx[c];
+    if (kc >= 0) return (kc << 2) | k;
     return a.eslot[6 * c + 2 * k + s];
 }
 
 __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
     const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
     if (c >= a.n) return;
-    const int8_t k = a.code[c];
-    const int64_t b0 = a.base[c] & 0xffffffffLL;  // the high word counts changing cells
+    const int32_t kc = a.kidx[c];
     const int d = a.depth[c];
     const int p = a.plotter[c];
-    if (k == CODE_KEEP) {
+    if (kc >= 0) {
+        const int64_t b0 = kc;
         a.depth_out[b0] = (int8_t)d;
         a.plotter_out[b0] = (int8_t)p;
         a.off_out[b0] = a.off[c];
@@ -460,13 +543,15 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
             const int32_t q = a.nbr[3 * c + e];
             if (q < 0) {
                 a.nbr_out[3 * b0 + e] = -1;
-            } else if (a.code[q >> 2] == CODE_KEEP) {
-                a.nbr_out[3 * b0 + e] =
-                    (int32_t)(((a.base[q >> 2] & 0xffffffffLL) << 2) | (q & 3));
+            } else {
+                const int32_t ko = a.kidx[q >> 2];
+                if (ko >= 0) a.nbr_out[3 * b0 + e] = (ko << 2) | (q & 3);
             }
         }
         return;
     }
+    const int8_t k = a.code[c];
+    const int64_t b0 = a.base[c] & 0xffffffffLL;  // the high word counts changing cells
     a.list[a.base[c] >> 32] = (int32_t)c;  // the changing cells, in order
     if (k == CODE_MERGE_FIRST) {
         a.depth_out[b0] = (int8_t)(d - 1);
@@ -639,39 +724,40 @@ __global__ __launch_bounds__(AT) void eta_jump_kernel(int64_t n, const double *e
     flags[c] = f;
 }
 
-// res[0] = the covered part of the two root trees, res[1] = the new cell
-// count, res[2] = status bits, res[3] = the last sweep that changed a word.
+// res[0] = the covered part of the two root trees.
 __global__ void copy_total_kernel(int64_t *dst, const int64_t *bsum, int64_t tiles) {
     if (threadIdx.x == 0) *dst = bsum[tiles];
 }
-__global__ void results_kernel(int64_t *res, const int64_t *bsum, int64_t tiles,
-                               const uint32_t *status) {
-    if (threadIdx.x == 0) {
-        res[1] = bsum[tiles] & 0xffffffffLL;
-        res[2] = status[0] | ((bsum[tiles] >> 32) != 0 ? 2u : 0u);
-        res[5] = bsum[tiles] >> 32;
-    }
-}
-// res[3] = the last sweep that changed a word; clears the error bits for the
-// count pass that follows
-__global__ void sweep_results_kernel(int64_t *res, uint32_t *status) {
-    if (threadIdx.x == 0) {
-        res[3] = status[1];
-        status[0] = 0u;
-    }
-}
-
 }  // namespace
 
 int64_t adapt_scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
 
+// Waits until the device has stored seq at host[7]: polls the word for a
+// while (a remeshing's first phase takes well under a millisecond), then
+// blocks on the stream, which also reports a failed kernel.
+static hipError_t wait_results(const int64_t *host, int64_t seq, hipStream_t s) {
+    const auto t0 = std::chrono::steady_clock::now();
+    for (unsigned i = 1;; ++i) {
+        if (__atomic_load_n(&host[7], __ATOMIC_ACQUIRE) == seq) return hipSuccess;
+        if ((i & 255u) == 0u &&
+            std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20))
+            break;
+        __builtin_ia32_pause();
+    }
+    const hipError_t e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    return __atomic_load_n(&host[7], __ATOMIC_ACQUIRE) == seq ? hipSuccess : hipErrorUnknown;
+}
+
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
-                      int max_sweeps, int64_t *host, AdaptPlan *plan) {
+                      int max_sweeps, int64_t *host, int64_t *host_dev, int64_t *seq,
+                      AdaptPlan *plan) {
     const unsigned nb = blocks_for(a.n, AT);
     const int64_t tiles = adapt_scan_tiles(a.n);
     int64_t *res = a.bsum + tiles + 1;
     hipError_t e;
-    conf_init_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.flags, a.dmin, a.dmax, a.slots, a.status);
+    conf_init_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.flags, a.dmin, a.dmax, a.slots, a.status,
+                                       a.stamp);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (!offsets_valid) {  // else the offsets of the unchanged mesh are in place
         tree_size_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.off);
@@ -685,20 +771,15 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
     int batch = first_batch < 1 ? 1 : first_batch, done = 0;
     while (done < max_sweeps) {
         for (int j = 1; j <= batch; ++j)
-            conf_sweep_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.status + 1,
-                                                (uint32_t)(done + j));
+            conf_sweep_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.stamp,
+                                                a.status + 1, (uint32_t)(done + j));
         done += batch;
-        sweep_results_kernel<<<1, 64, 0, s>>>(res, a.status);
         if (tiles > 0)
             count_reduce_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(
                 a.n, a.depth, a.nbr, a.slots, a.off, a.code, a.bsum, a.status);
-        scan_top_kernel<<<1, TOP_T, 0, s>>>(a.bsum, tiles);
-        results_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles, a.status);
+        plan_top_kernel<<<1, TOP_T, 0, s>>>(a.bsum, tiles, a.status, res, host_dev, ++*seq);
         if ((e = hipGetLastError()) != hipSuccess) return e;
-        if ((e = hipMemcpyAsync(host, res, 6 * sizeof(int64_t), hipMemcpyDeviceToHost, s)) !=
-            hipSuccess)
-            return e;
-        if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+        if ((e = wait_results(host, *seq, s)) != hipSuccess) return e;
         if (!(host[2] & 4)) {
             plan->sweeps = done;
             break;
@@ -738,7 +819,8 @@ hipError_t identity_record(int64_t n, int8_t *code, int64_t *base, hipStream_t s
 
 hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s) {
     const int64_t tiles = adapt_scan_tiles(a.n);
-    if (tiles > 0) scan_codes_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(a.n, a.code, a.bsum, a.base);
+    if (tiles > 0)
+        scan_codes_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(a.n, a.code, a.bsum, a.base, a.kidx);
     emit_kernel<<<blocks_for(a.n, AT), AT, 0, s>>>(a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.n_changing == 0) return e;

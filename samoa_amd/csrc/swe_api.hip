@@ -224,10 +224,13 @@ struct Batch {
     int64_t *ad_off2 = nullptr;  // the new mesh's tree offsets, swapped with ad_off
     int64_t ad_off_cap = 0, ad_off2_cap = 0;
     int8_t *ad_code = nullptr;
+    int32_t *ad_kidx = nullptr;  // new index of each kept cell, -1 for a changing one
     int32_t *ad_eslot = nullptr;
     int64_t ad_cap = 0;
     bool ad_off_valid = false;  // ad_off holds the current mesh's tree offsets
     int64_t *ad_host = nullptr; // [8] pinned results of the remesher
+    int64_t *ad_host_dev = nullptr;  // its device address
+    int64_t ad_seq = 0;         // number of the last results
     int ad_last_sweeps = 2;     // sweeps that changed a word in the last remeshing
     int64_t ad_record_n = -1;      // old cells of the last remeshing (its record in ad_code/ad_base)
     bool ad_record_sampled = false;  // its split cells sampled the raster
@@ -277,6 +280,7 @@ void release(Batch &B) {
     (void)hipFree(B.ad_base);
     (void)hipFree(B.ad_bsum);
     (void)hipFree(B.ad_code);
+    (void)hipFree(B.ad_kidx);
     (void)hipFree(B.ad_eslot);
     (void)hipFree(B.raster);
     if (B.ad_host) (void)hipHostFree(B.ad_host);
@@ -618,7 +622,9 @@ int ensure_adapt_scratch(Batch &B, int64_t n) {
     (void)hipFree(B.ad_base);
     (void)hipFree(B.ad_bsum);
     (void)hipFree(B.ad_code);
+    (void)hipFree(B.ad_kidx);
     (void)hipFree(B.ad_eslot);
+    B.ad_kidx = nullptr;
     B.ad_slots = B.ad_status = nullptr;
     B.ad_off = B.ad_off2 = B.ad_base = B.ad_bsum = nullptr;
     B.ad_off_cap = B.ad_off2_cap = 0;
@@ -627,13 +633,14 @@ int ensure_adapt_scratch(Batch &B, int64_t n) {
     B.ad_cap = 0;
     B.ad_off_valid = false;
     SWE_TRY(dev_alloc(&B.ad_slots, cap));
-    SWE_TRY(dev_alloc(&B.ad_status, 3));
+    SWE_TRY(dev_alloc(&B.ad_status, 4 + cap / 256 + 1));  // status words, then sweep stamps
     SWE_TRY(dev_alloc(&B.ad_off, cap));
     SWE_TRY(dev_alloc(&B.ad_off2, cap));
     B.ad_off_cap = B.ad_off2_cap = cap;
     SWE_TRY(dev_alloc(&B.ad_base, cap));
     SWE_TRY(dev_alloc(&B.ad_bsum, swe::adapt_scan_tiles(cap) + 1 + 8));
     SWE_TRY(dev_alloc(&B.ad_code, cap));
+    SWE_TRY(dev_alloc(&B.ad_kidx, cap));
     SWE_TRY(dev_alloc(&B.ad_eslot, 6 * cap));
     B.ad_cap = cap;
     return SWE_OK;
@@ -654,9 +661,18 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
                     "flags with swe_keep_refinement_flags)");
     }
     SWE_TRY(ensure_adapt_scratch(B, n));
-    if (!B.ad_host && hipHostMalloc(&B.ad_host, 8 * sizeof(int64_t)) != hipSuccess) {
-        B.ad_host = nullptr;
-        return fail(SWE_ERR_NOMEM, "adapt: pinned host allocation failed");
+    if (!B.ad_host) {
+        // coherent: the device stores the results there while the host polls
+        if (hipHostMalloc(&B.ad_host, 8 * sizeof(int64_t), hipHostMallocCoherent) != hipSuccess) {
+            B.ad_host = nullptr;
+            return fail(SWE_ERR_NOMEM, "adapt: pinned host allocation failed");
+        }
+        std::memset(B.ad_host, 0, 8 * sizeof(int64_t));
+        if (hipHostGetDevicePointer((void **)&B.ad_host_dev, B.ad_host, 0) != hipSuccess) {
+            (void)hipHostFree(B.ad_host);
+            B.ad_host = nullptr;
+            return fail(SWE_ERR_HIP, "adapt: no device address for the pinned results");
+        }
     }
     const swe::Scen &sc = B.scen[0];
     swe::AdaptArgs a{};
@@ -676,8 +692,10 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.base = B.ad_base;
     a.bsum = B.ad_bsum;
     a.code = B.ad_code;
+    a.kidx = B.ad_kidx;
     a.eslot = B.ad_eslot;
     a.status = B.ad_status;
+    a.stamp = B.ad_status + 4;
     a.raster = B.raster;
     a.rnx = B.rnx;
     a.rny = B.rny;
@@ -690,7 +708,7 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.offset_y = B.roffset_y;
     swe::AdaptPlan plan{};
     HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, 1 << 16, B.ad_host,
-                            &plan));
+                            B.ad_host_dev, &B.ad_seq, &plan));
     const int sweeps = plan.sweeps;
     if (sweeps < 0) return fail(SWE_ERR_STATE, "adapt: the conformity sweeps did not converge");
     // the next call starts with as many sweeps as changed a word this time
