@@ -117,7 +117,9 @@ def tsunami_adaptive(args, torch):
     between a cell's corners (|grad eta| h), then remeshed before every
     --nadapt-th step (the reference's default 1) from the elevation jumps to
     wet neighbours, which span about half a corner range (refine above half
-    of --adapt-tol, coarsen below a sixteenth).
+    of --adapt-tol, coarsen below a sixteenth); each step evaluates them on
+    the state it reads and leaves them for the next remeshing, as the
+    reference's Euler traversal leaves its flags.
     Split cells resample b from the raster, merged cells average it
     (SWE_adapt.f90:95-169).  Throughput counts the Euler cell updates over the
     wall time of the whole phase, remeshing included, as the reference's
@@ -147,14 +149,15 @@ def tsunami_adaptive(args, torch):
     state0 = op.download()
     t, fwd = 0.0, False
     step_ev = []
+    # every step leaves flags from the elevation jumps of the state it reads
+    # (a jump to a neighbour's centroid spans about half a corner range)
+    op.keep_eta_jump_flags(tol / 2.0, tol / 16.0)
 
     def one_step(k, timed):
         nonlocal t, dt, fwd
         if (k + 1) % args.nadapt == 0:  # time step k + 1 of SWE.f90:385-390
             if k > 0:  # before the first step the flags are all 0: nothing changes
-                # a jump to a neighbour's centroid spans about half a corner range
-                op.flag_eta_jumps(tol / 2.0, tol / 16.0)
-                op.adapt()
+                op.adapt()  # the flags the last step left
             fwd = not fwd  # the adaption traversal flips the direction
         n = op.n
         if timed:

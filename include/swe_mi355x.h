@@ -193,8 +193,21 @@ int swe_set_bathymetry_raster(swe_handle *h, int64_t nx, int64_t ny, double x0, 
 int swe_flag_eta_jumps(swe_handle *h, double refine_tol, double coarsen_tol);
 
 /* on != 0: every step of swe_step and swe_run leaves its refinement flags on
- * the device for swe_adapt(h, NULL, ...). */
+ * the device for swe_adapt(h, NULL, ...), with volume_op's |dQ_h| criterion
+ * (SWE_euler_timestep.f90:336-344).  Either value turns the elevation-jump
+ * criterion of swe_keep_eta_jump_flags off. */
 int swe_keep_refinement_flags(swe_handle *h, int on);
+
+/* Every step of swe_step and swe_run leaves refinement flags on the device
+ * for swe_adapt(h, NULL, ...) that follow the criterion of
+ * swe_flag_eta_jumps (refine_tol, coarsen_tol >= 0), evaluated on the state
+ * the step reads, as volume_op evaluates its criterion on the state of its
+ * traversal; no counterpart in the reference.  The traversal finds both
+ * states of every edge anyway, so no separate pass over the mesh is made.
+ * The flags of step k drive the remeshing before step k + 1, as in the
+ * reference's time loop (SWE.f90:385-393).  Single-mesh handles only;
+ * swe_step's refine output returns these flags too. */
+int swe_keep_eta_jump_flags(swe_handle *h, double refine_tol, double coarsen_tol);
 
 /* The record of the last swe_adapt, for the adjoint of a run with
  * remeshing (no counterpart in the reference): code[n_old] (0 keep, 1 first

@@ -58,6 +58,7 @@ SIGNATURES = {
     "swe_set_stream": (_I, [_P, _P]),
     "swe_adapt": (_I, [_P, _P, C.POINTER(_I64), C.POINTER(_I)]),
     "swe_keep_refinement_flags": (_I, [_P, _I]),
+    "swe_keep_eta_jump_flags": (_I, [_P, _D, _D]),
     "swe_flag_eta_jumps": (_I, [_P, _D, _D]),
     "swe_set_bathymetry_raster": (_I, [_P, _I64, _I64, _D, _D, _D, _D, _D, _D, _P]),
     "swe_cells": (_I64, [_P]),

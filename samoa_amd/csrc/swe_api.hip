@@ -125,7 +125,8 @@ int ensure_tables(int device) {
 // buffer it started from, so the graph stays valid across run calls.
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
-    int cur = -1, forward = -1, chunk = 0, keep = -1;
+    int cur = -1, forward = -1, chunk = 0, keep = -1, jumps = -1;
+    double jump_refine = 0.0, jump_coarsen = 0.0;
     int64_t grid = -1;
 };
 
@@ -238,6 +239,8 @@ struct Batch {
     int64_t rnx = 0, rny = 0;
     double rx0 = 0, ry0 = 0, rdx = 1, rdy = 1, roffset_x = 0, roffset_y = 0;
     int keep_flags = 0;       // every step writes its refinement flags to `refine`
+    int jump_flags = 0;       // ... from elevation jumps (swe_keep_eta_jump_flags)
+    double jump_refine = 0.0, jump_coarsen = 0.0;
     bool flags_valid = false; // `refine` holds the flags of the current state's last step
 };
 
@@ -459,6 +462,9 @@ swe::StepArgs make_args(const Batch &B, int from, int to, int forward) {
     a.inv16g = 1.0 / (16.0 * B.g);
     a.bc = B.bc;
     a.forward = forward ? 1 : 0;
+    a.flag_jumps = B.jump_flags;
+    a.jump_refine = B.jump_refine;
+    a.jump_coarsen = B.jump_coarsen;
     return a;
 }
 
@@ -557,7 +563,8 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
     if (pairs > 0) {
         GraphCache &gc = B.graph;
         if (!gc.exec || gc.cur != B.cur || gc.forward != fwd || gc.chunk != B.chunk ||
-            gc.grid != B.grid || gc.keep != B.keep_flags) {
+            gc.grid != B.grid || gc.keep != B.keep_flags || gc.jumps != B.jump_flags ||
+            gc.jump_refine != B.jump_refine || gc.jump_coarsen != B.jump_coarsen) {
             if (gc.exec) {
                 (void)hipGraphExecDestroy(gc.exec);
                 gc.exec = nullptr;
@@ -580,6 +587,9 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
             gc.chunk = B.chunk;
             gc.grid = B.grid;
             gc.keep = B.keep_flags;
+            gc.jumps = B.jump_flags;
+            gc.jump_refine = B.jump_refine;
+            gc.jump_coarsen = B.jump_coarsen;
         }
         for (int64_t i = 0; i < pairs; ++i) HIP_TRY(hipGraphLaunch(gc.exec, B.stream));
     }
@@ -1009,6 +1019,19 @@ int64_t swe_cells(swe_handle *h) { return h ? h->B.ncells : -1; }
 int swe_keep_refinement_flags(swe_handle *h, int on) {
     if (!h) return fail(SWE_ERR_ARG, "swe_keep_refinement_flags: null handle");
     h->B.keep_flags = on ? 1 : 0;
+    h->B.jump_flags = 0;
+    return SWE_OK;
+}
+
+int swe_keep_eta_jump_flags(swe_handle *h, double refine_tol, double coarsen_tol) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_keep_eta_jump_flags: null handle");
+    if (h->B.nscen != 1) return fail(SWE_ERR_ARG, "swe_keep_eta_jump_flags: single-mesh handles only");
+    if (!(refine_tol >= 0.0) || !(coarsen_tol >= 0.0))
+        return fail(SWE_ERR_ARG, "swe_keep_eta_jump_flags: tolerances must be >= 0");
+    h->B.keep_flags = 1;
+    h->B.jump_flags = 1;
+    h->B.jump_refine = refine_tol;
+    h->B.jump_coarsen = coarsen_tol;
     return SWE_OK;
 }
 

@@ -60,6 +60,11 @@ struct StepArgs {
     int bc, forward;
     int device_dt;                         // 1: read dt from slots[s].dt and write the next one
     int skip_finalize;                     // 1: nobody reads this step's CFL results
+    // refinement flags from surface-elevation jumps between wet neighbours
+    // of the state the step reads (refine above jump_refine, coarsen below
+    // jump_coarsen) instead of volume_op's |dQ_h| criterion
+    int flag_jumps;
+    double jump_refine, jump_coarsen;
 };
 
 // Discrete adjoint of one traversal (swe_adjoint_step.hip).

@@ -58,19 +58,12 @@ SWE_HD void riemanntype(double hL, double hR, double uL, double uR,
     const double F_min = delu + 2.0 * (sqh_min - sqh_max);
     const double F_max =
         delu + (h_max - h_min) * sqrt(((0.5 * g) * (h_max + h_min)) / (h_max * h_min));
-    // Every branch ends in sqm = sqrt(g hm), s1m = A1 - k1 sqm and
-    // s2m = A2 + k2 sqm with k1, k2 in {3, 1, -1}: multiplying by +-1 and
-    // adding a negated term are exact, so the shared tail rounds as each of
-    // the reference's branch expressions does, and a wavefront whose edges
-    // take different branches evaluates the square root once.
-    double A1, A2, k1, k2;
     if (F_min > 0.0) {  // two rarefactions
         const double t = dmax2(0.0, -delu + 2.0 * (sqL + sqR));
         hm = inv16g * (t * t);
-        A1 = uL + 2.0 * sqL;  // s1m = (uL + 2 sqL) - 3 sqm
-        k1 = 3.0;
-        A2 = uR - 2.0 * sqR;  // s2m = (uR - 2 sqR) + 3 sqm
-        k2 = 3.0;
+        const double sqm = sqrt(g * hm);
+        s1m = (uL + 2.0 * sqL) - 3.0 * sqm;
+        s2m = (uR - 2.0 * sqR) + 3.0 * sqm;
     } else if (F_max <= 0.0) {  // two shocks: one Newton step on sqrt(h)
         const double h0 = h_max;
         const double gL = sqrt((0.5 * g) * (1.0 / h0 + 1.0 / hL));
@@ -83,10 +76,11 @@ SWE_HD void riemanntype(double hL, double hR, double uL, double uR,
         const double slope = (2.0 * sq0) * dfdh;
         const double q = sq0 - F0 / slope;
         hm = q * q;
-        A1 = uL - (hm - hL) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hL));  // u1m; s1m = u1m - sqm
-        k1 = 1.0;
-        A2 = uR + (hm - hR) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hR));  // u2m; s2m = u2m + sqm
-        k2 = 1.0;
+        const double u1m = uL - (hm - hL) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hL));
+        const double u2m = uR + (hm - hR) * sqrt((0.5 * g) * (1.0 / hm + 1.0 / hR));
+        const double sqm = sqrt(g * hm);
+        s1m = u1m - sqm;
+        s2m = u2m + sqm;
     } else {  // one shock, one rarefaction: one secant step
         // The reference evaluates F0 at h0 = h_min as
         //   (delu + 2 (sqrt(g h_min) - sqrt(g h_max))) + (h0 - h_min) sqrt(g/2 (1/h0 + 1/h_min)),
@@ -96,21 +90,15 @@ SWE_HD void riemanntype(double hL, double hR, double uL, double uR,
         const double F0 = F_min + 0.0;
         const double slope = (F_max - F0) / (h_max - h_min);
         hm = h0 - F0 / slope;
+        const double sqm = sqrt(g * hm);
         if (hL > hR) {
-            A1 = uL + 2.0 * sqL;  // s1m = (uL + 2 sqL) - 3 sqm
-            k1 = 3.0;
-            A2 = A1;  // s2m = (uL + 2 sqL) - sqm
-            k2 = -1.0;
+            s1m = (uL + 2.0 * sqL) - 3.0 * sqm;
+            s2m = (uL + 2.0 * sqL) - sqm;
         } else {
-            A2 = uR - 2.0 * sqR;  // s2m = (uR - 2 sqR) + 3 sqm
-            k2 = 3.0;
-            A1 = A2;  // s1m = (uR - 2 sqR) + sqm
-            k1 = -1.0;
+            s2m = (uR - 2.0 * sqR) + 3.0 * sqm;
+            s1m = (uR - 2.0 * sqR) + sqm;
         }
     }
-    const double sqm = sqrt(g * hm);
-    s1m = A1 - k1 * sqm;
-    s2m = A2 + k2 * sqm;
 }
 
 // hm of riemanntype(h, h, a, c) for the wall test of a wet cell of depth h

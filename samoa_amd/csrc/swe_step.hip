@@ -56,7 +56,10 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 
 // Four waves per SIMD (at most 128 VGPRs, a few spilled) outrun three
 // unspilled ones: the LDS of four 256-cell workgroups fits a CU.
-template <int SOLVER, int CH>
+// JUMPS: the refinement flags follow the elevation-jump criterion
+// (StepArgs::flag_jumps); a template parameter so that the plain kernel's
+// code and registers do not pay for it.
+template <int SOLVER, int CH, bool JUMPS>
 __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void euler_step_kernel(
     StepArgs a) {
     constexpr int NW = CH / 64;
@@ -231,12 +234,50 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
             sv = sv + lleg * s_f[3][k2][t];
 
             int8_t rf = 0;
-            const double dq = fabs(dh);
-            if (d < sc.dmax && dq > sc.refine_thr) {
-                rf = 1;
-                refined = 1;
-            } else if (d > sc.dmin && dq < sc.refine_thr / 8.0) {
-                rf = -1;
+            if (JUMPS) {
+                // elevation jumps to the wet neighbours of a wet cell, in
+                // the state the step reads: bit 0 if one is above the
+                // refinement tolerance, bit 1 if one is not below the
+                // coarsening tolerance (a NaN jump sets neither, as fmax
+                // ignores it).  j = the largest jump (0 without a wet
+                // neighbour): j > tol if some edge's is, j < tol if every
+                // edge's is and tol > 0.
+                int jb = 0;
+                const double qe = s_q[0][t];
+                if (qe - s_q[3][t] > drytol) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const int32_t p = s_nbr[k][t];
+                        if (p < 0) continue;
+                        const int64_t o = (int64_t)(p >> 2);
+                        double oe, ob;
+                        if (o >= lbase && o < lbase + cnt) {
+                            oe = s_q[0][o - lbase];
+                            ob = s_q[3][o - lbase];
+                        } else {
+                            oe = a.eta[gs + o];
+                            ob = a.b[gs + o];
+                        }
+                        if (oe - ob > drytol) {
+                            const double jd = fabs(qe - oe);
+                            jb |= (jd > a.jump_refine ? 1 : 0) | (jd >= a.jump_coarsen ? 2 : 0);
+                        }
+                    }
+                }
+                if (d < sc.dmax && (jb & 1)) {
+                    rf = 1;
+                    refined = 1;
+                } else if (d > sc.dmin && !(jb & 2) && a.jump_coarsen > 0.0) {
+                    rf = -1;
+                }
+            } else {
+                const double dq = fabs(dh);
+                if (d < sc.dmax && dq > sc.refine_thr) {
+                    rf = 1;
+                    refined = 1;
+                } else if (d > sc.dmin && dq < sc.refine_thr / 8.0) {
+                    rf = -1;
+                }
             }
             if (a.refine) a.refine[gs + lc_t] = rf;
             cand = volume / sv;
@@ -368,8 +409,12 @@ template <int SOLVER, int CH>
 static void launch_t(const StepArgs &a, int64_t grid_blocks, hipStream_t s) {
     const int64_t nvirt = 8 * ((a.nchunks + 7) / 8);
     const int64_t blocks = grid_blocks > 0 && grid_blocks < nvirt ? grid_blocks : nvirt;
-    hipLaunchKernelGGL((euler_step_kernel<SOLVER, CH>), dim3((unsigned)blocks), dim3(CH), 0, s,
-                       a);
+    if (a.flag_jumps)
+        hipLaunchKernelGGL((euler_step_kernel<SOLVER, CH, true>), dim3((unsigned)blocks), dim3(CH),
+                           0, s, a);
+    else
+        hipLaunchKernelGGL((euler_step_kernel<SOLVER, CH, false>), dim3((unsigned)blocks),
+                           dim3(CH), 0, s, a);
     if (!a.skip_finalize)
         hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)a.n_scen), dim3(FIN_THREADS), 0, s, a);
 }

@@ -122,7 +122,9 @@ contains
 
     !> SWE.f90:380-429 with remeshing before every nadapt-th step
     !> (:385-390), the flags taken from the surface-elevation jumps between
-    !> wet neighbours (refine above refine_tol, coarsen below coarsen_tol).
+    !> wet neighbours (refine above refine_tol, coarsen below coarsen_tol) of
+    !> the state each step reads, which the step leaves on the device as the
+    !> reference's Euler traversal leaves its flags (swe_keep_eta_jump_flags).
     !> k0 numbers the first step from 0, so that calls continue one another:
     !> step k remeshes when mod(k + 1, nadapt) = 0, as time step k + 1 of the
     !> reference does; before step 0 the flags are the initial phase's, all 0,
@@ -144,12 +146,14 @@ contains
         dt_first = dt
         cells = 0
         rc = SWE_OK
+        if (nadapt > 0) then
+            rc = swe_keep_eta_jump_flags(handle, refine_tol, coarsen_tol)
+            if (rc /= SWE_OK) return
+        end if
         do k = k0, k0 + nsteps - 1
             if (nadapt > 0) then
                 if (mod(k + 1, nadapt) == 0) then
                     if (k > 0) then
-                        rc = swe_flag_eta_jumps(handle, refine_tol, coarsen_tol)
-                        if (rc /= SWE_OK) return
                         rc = swe_adapt(handle, c_null_ptr, n_new, sweeps)
                         if (rc /= SWE_OK) return
                     end if

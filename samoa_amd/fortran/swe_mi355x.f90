@@ -63,6 +63,14 @@ module swe_mi355x
             integer(c_int), value   :: on
         end function
 
+        !> every step leaves flags from the elevation jumps of the state it reads
+        integer(c_int) function swe_keep_eta_jump_flags(handle, refine_tol, coarsen_tol) &
+                bind(c, name='swe_keep_eta_jump_flags')
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value      :: handle
+            real(c_double), value   :: refine_tol, coarsen_tol
+        end function
+
         integer(c_int64_t) function swe_cells(handle) bind(c, name='swe_cells')
             import :: c_int64_t, c_ptr
             type(c_ptr), value :: handle

@@ -201,8 +201,15 @@ program swe_mi355x_main
         call check(rc, 'swe_upload')
     end if
     if (nadapt > 0) then
-        rc = swe_keep_refinement_flags(h, 1_c_int)
-        call check(rc, 'swe_keep_refinement_flags')
+        if (trim(scenario) == 'tsunami') then
+            ! |grad eta| of the state each step reads: a jump to a neighbour
+            ! spans about half a corner range
+            rc = swe_keep_eta_jump_flags(h, refine_tol / 2.0_c_double, refine_tol / 16.0_c_double)
+            call check(rc, 'swe_keep_eta_jump_flags')
+        else
+            rc = swe_keep_refinement_flags(h, 1_c_int)
+            call check(rc, 'swe_keep_refinement_flags')
+        end if
     end if
 
     ! init_grid leaves the grid reversed; init_b runs backward, init_dofs
@@ -237,11 +244,6 @@ program swe_mi355x_main
                     refine = 0
                     rc = swe_adapt(h, c_loc(refine), n_new, sweeps)
                 else
-                    if (trim(scenario) == 'tsunami') then
-                        ! |grad eta|: a jump to a neighbour spans about half a corner range
-                        rc = swe_flag_eta_jumps(h, refine_tol / 2.0_c_double, refine_tol / 16.0_c_double)
-                        call check(rc, 'swe_flag_eta_jumps')
-                    end if
                     rc = swe_adapt(h, c_null_ptr, n_new, sweeps)
                 end if
                 call check(rc, 'swe_adapt')

@@ -112,6 +112,14 @@ class EulerTimestep:
         _native.check("swe_keep_refinement_flags",
                       self._lib.swe_keep_refinement_flags(self._h, int(bool(on))))
 
+    def keep_eta_jump_flags(self, refine_tol: float, coarsen_tol: float) -> None:
+        """Every step leaves flags from the surface-elevation jumps to wet
+        neighbours of the state it reads (the criterion of flag_eta_jumps)
+        on the device for adapt() (swe_keep_eta_jump_flags)."""
+        _native.check("swe_keep_eta_jump_flags",
+                      self._lib.swe_keep_eta_jump_flags(self._h, float(refine_tol),
+                                                        float(coarsen_tol)))
+
     def flag_eta_jumps(self, refine_tol: float, coarsen_tol: float) -> None:
         """Device refinement flags from surface-elevation jumps to wet
         neighbours (swe_flag_eta_jumps), for adapt() with refine=None."""

@@ -25,8 +25,9 @@ def load():
             subprocess.run(["hipcc", "-O2", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off",
                             "--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang",
                             "-packed-fp32-ops", f"-I{os.path.join(ROOT, 'samoa_amd', 'csrc')}",
-                            SRC, "-o", OUT + ".tmp"], check=True, capture_output=True)
-            os.replace(OUT + ".tmp", OUT)
+                            SRC, "-o", f"{OUT}.tmp{os.getpid()}"], check=True,
+                           capture_output=True)
+            os.replace(f"{OUT}.tmp{os.getpid()}", OUT)  # parallel test workers may race here
         L = C.CDLL(OUT)
         L.shim_edge_flux.argtypes = [C.c_int, C.c_double, C.c_double, _dp, _dp, C.c_double,
                                      C.c_double, _dp, _dp]

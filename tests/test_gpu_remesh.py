@@ -163,6 +163,63 @@ def test_split_cells_sample_the_bathymetry_raster():
     op.close()
 
 
+def eta_jump_flags(mesh, st, drytol, refine_tol, coarsen_tol, dmin, dmax):
+    """The elevation-jump criterion restated in numpy: j = the largest
+    |eta - eta_n| over the wet neighbours n of a wet cell (0 otherwise)."""
+    eta, b = st[0], st[3]
+    wet = eta - b > drytol
+    j = np.zeros(mesh.n)
+    for k in range(3):
+        o = mesh.nbr[:, k]
+        ok = (o >= 0) & wet & wet[np.maximum(o, 0)]
+        j = np.where(ok, np.maximum(j, np.abs(eta - eta[np.maximum(o, 0)])), j)
+    return np.where((mesh.depth < dmax) & (j > refine_tol), 1,
+                    np.where((mesh.depth > dmin) & (j < coarsen_tol), -1, 0)).astype(np.int8)
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("coarsen", [True, False])
+def test_steps_leave_eta_jump_flags(seed, coarsen):
+    """With swe_keep_eta_jump_flags every step's flags follow the
+    elevation-jump criterion on the state it reads (numpy restatement), on
+    adaptive meshes with dry land, in both directions; the remeshing they
+    drive equals the oracle's with those flags."""
+    mesh, rng = random_case(seed, n_rounds=4)
+    print("case: seed", seed, "coarsen", coarsen, "cells", mesh.n, flush=True)
+    st = np.stack(host.tsunami(mesh, seed=seed, drytol=1e-3))
+    # a shelf of dry land along one side
+    cen = mesh.centroids()
+    st[3] = np.where(cen[:, 0] > 0.85, 50.0, st[3])
+    st[0] = np.maximum(st[0], np.minimum(st[3], 0.0))
+    dmin, dmax = 1, 14
+    j = eta_jump_flags(mesh, st, 1e-3, 0.0, np.inf, 0, 99)  # any wet jump refines
+    assert (j == 1).any()
+    eta, b = st[0], st[3]
+    jumps = np.abs(eta[:, None] - eta[np.maximum(mesh.nbr, 0)])
+    rtol = float(np.quantile(jumps[jumps > 0], 0.7))
+    ctol = rtol / 8 if coarsen else 0.0
+    op = swe.EulerTimestep(mesh, scaling=1e6, dry_tolerance=1e-3, min_depth=dmin, max_depth=dmax)
+    op.upload(*st)
+    op.keep_eta_jump_flags(rtol, ctol)
+    dt, fwd = 5.0, bool(seed % 2)
+    for step in range(3):
+        cur = np.stack(op.download())
+        res = op.traverse(dt, fwd, flags=True)
+        want = eta_jump_flags(mesh, cur, 1e-3, rtol, ctol, dmin, dmax)
+        assert np.array_equal(res.refine, want), step
+        assert res.n_refine == int((want == 1).sum())
+        fwd = not fwd
+    after = np.stack(op.download())
+    d2, p2, s2, nb2, _, _ = po.adapt(mesh.depth, mesh.plotter, mesh.nbr, want, after, dmin=dmin,
+                                     dmax=dmax, coords=mesh.coords)
+    n, _ = op.adapt()
+    assert n == len(d2)
+    d, p, nb = op.download_mesh()
+    assert np.array_equal(d, d2) and np.array_equal(p, p2) and np.array_equal(nb, nb2)
+    assert np.array_equal(np.stack(op.download()), s2)
+    op.close()
+
+
 def test_eta_jump_flags_drive_adapt():
     """swe_flag_eta_jumps against its definition restated in numpy, through
     the remeshing it drives (compared with the oracle's remeshing)."""
