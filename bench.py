@@ -294,7 +294,7 @@ def tsunami_frozen(args, torch):
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": traffic,
-            "kernel": "swe::euler_step_kernel<0> + swe::finalize_kernel (one step)",
+            "kernel": "swe::euler_step_kernel<0, 256, false> + swe::finalize_kernel (one step)",
             "bytes_per_cell_update": BYTES_PER_CELL,
             "valu_issue_frac_profiled": valu,
         },
@@ -575,7 +575,7 @@ def main():
         "roofline": {
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": traffic,
-            "kernel": "swe::euler_step_kernel<0> + swe::finalize_kernel (one step)",
+            "kernel": "swe::euler_step_kernel<0, 256, true> + swe::finalize_kernel (one step)",
             "bytes_per_cell_update": BYTES_PER_CELL,
             "timing": "HIP events around every step launch of the adaptive run",
         },
