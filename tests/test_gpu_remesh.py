@@ -91,6 +91,38 @@ def test_random_requests_match_oracle(seed):
     op.close()
 
 
+def test_short_sweep_batch_is_completed():
+    """A call without requests changes no word, so the next call starts with
+    a single sweep; requests that need more make the count pass find words
+    off the fixpoint, and further sweeps and a recount follow (more than one
+    sweep reported).  Mesh and state still equal the oracle's, in every case
+    and in at least one that takes that path."""
+    taken = 0
+    for seed in range(10, 16):
+        mesh, rng = random_case(seed)
+        while mesh.n < 4000:  # tens of sweep tiles
+            grow = rng.choice(np.array([0, 1], np.int8), size=mesh.n, p=[0.5, 0.5])
+            mesh = host.mesh_from_depths(po.adapt(mesh.depth, mesh.plotter, mesh.nbr, grow,
+                                                  np.zeros((4, mesh.n)), dmin=1, dmax=16)[0])
+        st = rng.normal(size=(4, mesh.n))
+        op = swe.EulerTimestep(mesh, scaling=1.0, dry_tolerance=1e-3, min_depth=1, max_depth=16)
+        op.upload(*st)
+        n, _ = op.adapt(np.zeros(mesh.n, np.int8))
+        assert n == mesh.n
+        flags = rng.choice(np.array([-1, 0, 1], np.int8), size=mesh.n, p=[0.3, 0.3, 0.4])
+        d2, p2, s2, nb2, _, _ = po.adapt(mesh.depth, mesh.plotter, mesh.nbr, flags, st, dmin=1,
+                                         dmax=16, coords=mesh.coords)
+        n, sweeps = op.adapt(flags)
+        print("case: seed", seed, "cells", mesh.n, "sweeps", sweeps, flush=True)
+        taken += sweeps > 1
+        assert n == len(d2)
+        d, p, nb = op.download_mesh()
+        assert np.array_equal(d, d2) and np.array_equal(p, p2) and np.array_equal(nb, nb2)
+        assert np.array_equal(np.stack(op.download()), s2)
+        op.close()
+    assert taken > 0, "no case needed more than the first sweep"
+
+
 def test_growth_and_reuse_of_buffers():
     """Refine everything from depth 4 to 12 (buffers grow), coarsen back
     (spare buffers reused), against the uniform meshes and the oracle."""
