@@ -170,14 +170,19 @@ int swe_adjoint_step_device(swe_handle *h, const double *eta, const double *hu, 
 int swe_adapt(swe_handle *h, const int8_t *refine, int64_t *n_cells, int *sweeps);
 
 /* Bathymetry of the cells that swe_adapt splits, in place of the parent's
- * value: bilinear interpolation in a raster of nx x ny values over the world
- * points (x0 + i dx, y0 + j dy), values[j * nx + i], at the child's centroid
- * in world coordinates, scaling * (x1 + x2 + x3) / 3 + (offset_x, offset_y)
- * (unit-square corners; cfg%scaling and cfg%offset), clamped to the raster's
- * last cell.  This stands for
- * get_bathymetry_at_element (SWE_initialize.f90:120-147) with gridded data
- * (the ASAGI path, SWE_initialize.f90:149-187).  The library copies the
- * values; NULL removes the raster. */
+ * value, as get_bathymetry_at_element (SWE_initialize.f90:120-147) computes
+ * it with gridded data (the ASAGI path, SWE_initialize.f90:149-187) in the
+ * reference's _ADAPT_INTEGRATE build.  The raster is bilinear in nx x ny
+ * values over the world points (x0 + i dx, y0 + j dy), values[j * nx + i],
+ * clamped to its last cell; a unit-square point x lies at world
+ * scaling * x + (offset_x, offset_y) (cfg%scaling, cfg%offset).  A child at
+ * depth d averages it over L = min(16, min(max_depth + 3, data_depth + 3) - d)
+ * halvings, data_depth = nint(log2(scaling^2 / (dx dy))): each level
+ * 0.5 (value of the half (x1, m, x2) + value of the half (x2, m, x3)),
+ * m = 0.5 (x1 + x3), and at the bottom the raster at the centroid
+ * (x1 + x2 + x3) / 3 (refine_2D_recursive, :106-116); L <= 0 is that
+ * centroid sample alone.  The library copies the values; NULL removes the
+ * raster. */
 int swe_set_bathymetry_raster(swe_handle *h, int64_t nx, int64_t ny, double x0, double y0,
                               double dx, double dy, double offset_x, double offset_y,
                               const double *values);

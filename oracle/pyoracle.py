@@ -178,6 +178,54 @@ def initial_dt(depth, state, courant=0.45, g=G):
 _VTK_TYPES = {"Float64": ">f8", "Int32": ">i4", "Int8": ">i1"}
 
 
+def integrate_levels(depth: int, dmax: int, data_depth=None) -> int:
+    """Recursion depth of get_bathymetry_at_element under _ADAPT_INTEGRATE
+    (SWE_initialize.f90:127-133): min(16, dmax - d) for a bathymetry
+    function, min(16, min(dmax + 3, data_depth + 3) - d) for gridded data
+    (the ASAGI branch); <= 0 means one sample at the centroid."""
+    if data_depth is None:
+        return min(16, dmax - depth)
+    return min(16, min(dmax + 3, data_depth + 3) - depth)
+
+
+def data_depth(scaling: float, dx: float, dy: float) -> int:
+    """nint(log(scaling**2 / (dx dy)) / log(2)) of SWE_initialize.f90:129;
+    nint rounds half away from zero."""
+    import math
+
+    v = math.log(scaling ** 2 / (dx * dy)) / math.log(2.0)
+    return int(math.floor(abs(v) + 0.5)) * (1 if v >= 0 else -1)
+
+
+def integrate_bathymetry(f, x1, x2, x3, levels: int) -> float:
+    """refine_2D_recursive (SWE_initialize.f90:106-116): over `levels`
+    halvings of the triangle (x1, x2, x3) (x2 the right angle, unit-square
+    points), 0.5 (value of (x1, m, x2) + value of (x2, m, x3)) with
+    m = 0.5 (x1 + x3); at the bottom f at the centroid (x1 + x2 + x3) / 3.
+    Pure-Python recursion for small cases."""
+    if levels > 0:
+        m = (0.5 * (x1[0] + x3[0]), 0.5 * (x1[1] + x3[1]))
+        return 0.5 * (integrate_bathymetry(f, x1, m, x2, levels - 1) +
+                      integrate_bathymetry(f, x2, m, x3, levels - 1))
+    return f((x1[0] + x2[0] + x3[0]) / 3.0, (x1[1] + x2[1] + x3[1]) / 3.0)
+
+
+def bilinear(values, x0, y0, dx, dy):
+    """The bilinear raster sample of the library's remesher (clamped to the
+    last cell) as a function of one world point, for integrate_bathymetry."""
+    ny, nx = values.shape
+
+    def f(X, Y):
+        fx, fy = (X - x0) / dx, (Y - y0) / dy
+        ix = min(max(int(fx), 0), nx - 2)
+        iy = min(max(int(fy), 0), ny - 2)
+        wx, wy = fx - ix, fy - iy
+        return ((1.0 - wx) * (1.0 - wy) * values[iy, ix] + wx * (1.0 - wy) * values[iy, ix + 1] +
+                (1.0 - wx) * wy * values[iy + 1, ix] + wx * wy * values[iy + 1, ix + 1])
+
+    return f
+
+
 def read_vtu(path: str) -> dict:
     raw = open(path, "rb").read()
     head_end = raw.index(b"<AppendedData")

@@ -102,9 +102,54 @@ def sample(r: Raster, x, y) -> np.ndarray:
             (1.0 - wx) * wy * v[iy + 1, ix] + wx * wy * v[iy + 1, ix + 1])
 
 
-def cell_bathymetry(r: Raster, coords: np.ndarray, scaling: float, offset) -> np.ndarray:
-    """b of each cell: the raster at the centroid scaling * (x1 + x2 + x3) / 3
-    + offset of its unit-square corners coords[n, 3, 2]."""
-    c = (coords[:, 0] + coords[:, 1] + coords[:, 2]) / 3.0
+def data_depth(r: Raster, scaling: float) -> int:
+    """The refinement depth at which a cell is as large as a grid cell,
+    nint(log(scaling^2 / (dx dy)) / log(2)) (SWE_initialize.f90:129; nint
+    rounds half away from zero)."""
+    v = np.log(scaling ** 2 / (r.dx * r.dy)) / np.log(2.0)
+    return int(np.sign(v) * np.floor(abs(v) + 0.5))
+
+
+def integrate_levels(depth: np.ndarray, dmax: int, ddepth: int) -> np.ndarray:
+    """Halvings over which get_bathymetry_at_element averages gridded data
+    under _ADAPT_INTEGRATE (SWE_initialize.f90:127-133, the ASAGI branch):
+    min(16, min(dmax + 3, ddepth + 3) - d); <= 0 means the centroid."""
+    return np.minimum(16, min(dmax + 3, ddepth + 3) - depth.astype(np.int64))
+
+
+def cell_bathymetry(r: Raster, coords: np.ndarray, scaling: float, offset, depth=None,
+                    dmax=None) -> np.ndarray:
+    """b of each cell of unit-square corners coords[n, 3, 2] (entry, right
+    angle, exit), world point = scaling * x + offset.
+
+    Without depth: the raster at the centroid (x1 + x2 + x3) / 3, as the
+    reference's _ADAPT_SAMPLE build evaluates it.  With depth and dmax: the
+    _ADAPT_INTEGRATE average of get_bathymetry_at_element
+    (SWE_initialize.f90:106-147), which the reference is built with and the
+    device's remesher applies to split cells: over L = integrate_levels(...)
+    halvings of the cell, 0.5 (value of the first half + value of the
+    second) with the halves (x1, m, x2), (x2, m, x3), m = 0.5 (x1 + x3), and
+    the raster at the centroid at the bottom."""
     ox, oy = (offset, offset) if np.isscalar(offset) else offset
-    return sample(r, scaling * c[:, 0] + ox, scaling * c[:, 1] + oy)
+    if depth is None:
+        c = (coords[:, 0] + coords[:, 1] + coords[:, 2]) / 3.0
+        return sample(r, scaling * c[:, 0] + ox, scaling * c[:, 1] + oy)
+    levels = np.maximum(integrate_levels(np.asarray(depth), int(dmax), data_depth(r, scaling)), 0)
+    out = np.empty(coords.shape[0])
+    for L in np.unique(levels):
+        sel = np.nonzero(levels == L)[0]
+        out[sel] = _integrate(r, coords[sel], int(L), scaling, ox, oy)
+    return out
+
+
+def _integrate(r: Raster, co: np.ndarray, L: int, scaling: float, ox: float, oy: float):
+    """Leaves of L halvings in depth-first order, each level's pairs combined
+    as the recursion does (0.5 (first + second))."""
+    if L == 0:
+        c = (co[:, 0] + co[:, 1] + co[:, 2]) / 3.0
+        return sample(r, scaling * c[:, 0] + ox, scaling * c[:, 1] + oy)
+    m = 0.5 * (co[:, 0] + co[:, 2])
+    first = np.stack([co[:, 0], m, co[:, 1]], axis=1)
+    second = np.stack([co[:, 1], m, co[:, 2]], axis=1)
+    return 0.5 * (_integrate(r, first, L - 1, scaling, ox, oy) +
+                  _integrate(r, second, L - 1, scaling, ox, oy))

@@ -466,11 +466,11 @@ __device__ __forceinline__ int split_type(int p, int rule) {
     }
 }
 
-// Centroid (unit square) of the cell at tree offset `off` and depth d: the
-// corners of the root, then d halvings along the path the offset spells
-// (bit 60 - l chooses the second child at level l), with the arithmetic of
-// the host's mesh builders: m = 0.5 (x1 + x3), centroid (x1 + x2 + x3) / 3.
-__device__ void tree_centroid(int64_t off, int d, double *cx, double *cy) {
+// Corners (unit square; entry, right angle, exit) of the cell at tree offset
+// `off` and depth d: the corners of its root, then d halvings along the path
+// the offset spells (bit 60 - l chooses the second child at level l), with
+// the host's arithmetic, m = 0.5 (x1 + x3).
+__device__ void tree_corners(int64_t off, int d, double c[6]) {
     double ax, ay, bx, by, ex, ey;  // corners 1, 2, 3
     if ((off >> 60) == 0) {
         ax = 0.0; ay = 0.0; bx = 1.0; by = 0.0; ex = 1.0; ey = 1.0;
@@ -489,8 +489,7 @@ __device__ void tree_centroid(int64_t off, int d, double *cx, double *cy) {
         bx = mx;
         by = my;
     }
-    *cx = (ax + bx + ex) / 3.0;
-    *cy = (ay + by + ey) / 3.0;
+    c[0] = ax; c[1] = ay; c[2] = bx; c[3] = by; c[4] = ex; c[5] = ey;
 }
 
 // Bilinear sample of the raster at the world point (X, Y), clamped to its
@@ -503,6 +502,45 @@ __device__ double raster_sample(const AdaptArgs &a, double X, double Y) {
     const double *r0 = a.raster + iy * a.rnx + ix, *r1 = r0 + a.rnx;
     return (1.0 - wx) * (1.0 - wy) * r0[0] + wx * (1.0 - wy) * r0[1] + (1.0 - wx) * wy * r1[0] +
            wx * wy * r1[1];
+}
+
+// b of a new cell at tree offset `off` and depth d from the raster, as
+// get_bathymetry_at_element computes it under _ADAPT_INTEGRATE
+// (SWE_initialize.f90:106-147) for gridded data: L = min(16, raster_cap - d)
+// halvings of the cell (raster_cap = min(dmax + 3, data_depth + 3)), each
+// level 0.5 (value of the first half (x1, m, x2) + value of the second
+// (x2, m, x3)), the raster at the centroid scaling * (x1 + x2 + x3) / 3 +
+// offset at the bottom; L <= 0 is that centroid sample alone.  The 2^L
+// leaves run in depth-first order, a leaf's corners found by L halvings
+// along its index; a stack of partial values combines each finished pair as
+// the recursion does.
+__device__ double raster_cell(const AdaptArgs &a, int64_t off, int d) {
+    double c[6];
+    tree_corners(off, d, c);
+    const int levels = min(16, a.raster_cap - d);
+    double stack[17];
+    int sp = 0;
+    const int leaves = levels > 0 ? 1 << levels : 1;
+    for (int i = 0; i < leaves; ++i) {
+        double ax = c[0], ay = c[1], bx = c[2], by = c[3], ex = c[4], ey = c[5];
+        for (int l = levels - 1; l >= 0; --l) {
+            const double mx = 0.5 * (ax + ex), my = 0.5 * (ay + ey);
+            if ((i >> l) & 1) {  // (x2, m, x3)
+                ax = bx;
+                ay = by;
+            } else {  // (x1, m, x2)
+                ex = bx;
+                ey = by;
+            }
+            bx = mx;
+            by = my;
+        }
+        const double cx = (ax + bx + ex) / 3.0, cy = (ay + by + ey) / 3.0;
+        double v = raster_sample(a, a.scaling * cx + a.offset_x, a.scaling * cy + a.offset_y);
+        for (int j = i; j & 1; j >>= 1) v = 0.5 * (stack[--sp] + v);
+        stack[sp++] = v;
+    }
+    return stack[0];
 }
 
 __device__ __forceinline__ void put_slot(int32_t *eslot, int64_t c, int k, int half, int32_t v) {
@@ -586,9 +624,7 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
             a.hu_out[idx] = a.hu[c];
             a.hv_out[idx] = a.hv[c];
             if (a.raster) {
-                double cx, cy;
-                tree_centroid(coff, dc, &cx, &cy);
-                a.b_out[idx] = raster_sample(a, a.scaling * cx + a.offset_x, a.scaling * cy + a.offset_y);
+                a.b_out[idx] = raster_cell(a, coff, dc);
             } else {
                 a.b_out[idx] = a.b[c];
             }

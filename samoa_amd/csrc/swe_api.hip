@@ -716,6 +716,13 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.scaling = sc.scaling;
     a.offset_x = B.roffset_x;
     a.offset_y = B.roffset_y;
+    if (B.raster) {
+        // SWE_initialize.f90:127-130: the depth at which a cell is as large as
+        // a grid cell, nint(log(scaling^2 / (dx dy)) / log(2))
+        const long data_depth = std::lround(std::log(sc.scaling * sc.scaling / (B.rdx * B.rdy)) /
+                                            std::log(2.0));
+        a.raster_cap = (int)std::min<long>(sc.dmax + 3, data_depth + 3);
+    }
     swe::AdaptPlan plan{};
     HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, 1 << 16, B.ad_host,
                             B.ad_host_dev, &B.ad_seq, &plan));

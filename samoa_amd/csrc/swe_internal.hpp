@@ -112,11 +112,14 @@ struct AdaptArgs {
     int32_t *nbr_out;                      // [3n']
     double *eta_out, *hu_out, *hv_out, *b_out;
     // bathymetry of split cells: bilinear in a raster of nx x ny values at
-    // world points (x0 + i dx, y0 + j dy), values[j nx + i], sampled at the
-    // child's centroid scaling * c + offset; null: the parent's b
+    // world points (x0 + i dx, y0 + j dy), values[j nx + i], at world points
+    // scaling * c + offset of unit-square points c; null: the parent's b
     const double *raster;
     int64_t rnx, rny;
     double rx0, ry0, rdx, rdy, scaling, offset_x, offset_y;
+    // a new cell at depth d averages the raster over min(16, raster_cap - d)
+    // halvings (_ADAPT_INTEGRATE); raster_cap = min(dmax + 3, data_depth + 3)
+    int raster_cap;
 };
 
 // bsum holds adapt_scan_tiles(n) + 1 scan words and 8 result words.

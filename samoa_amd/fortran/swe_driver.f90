@@ -224,7 +224,7 @@ contains
             if (rc /= SWE_OK) return
         end do
         allocate(eta(n), hu(n), hv(n), b(n), grid(TSU_G, TSU_G))
-        call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+        call tsunami_state(n, coords, depth, int(dmax), seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
         rc = swe_upload(handle, eta, hu, hv, b)
         if (rc /= SWE_OK) return
         call tsunami_raster(seed, grid)
@@ -330,13 +330,15 @@ contains
         swe_host_tsunami_raster = TSU_G
     end function
 
-    subroutine swe_host_tsunami(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b) &
-            bind(c, name='swe_host_tsunami')
+    subroutine swe_host_tsunami(n, coords, depth, dmax, seed, amp, ratio, x0, y0, drytol, eta, hu, &
+                                hv, b) bind(c, name='swe_host_tsunami')
         integer(c_int64_t), value   :: n, seed
         real(c_double), intent(in)  :: coords(2, 3, n)
+        integer(c_int8_t), intent(in) :: depth(n)
+        integer(c_int), value       :: dmax
         real(c_double), value       :: amp, ratio, x0, y0, drytol
         real(c_double), intent(out) :: eta(n), hu(n), hv(n), b(n)
-        call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+        call tsunami_state(n, coords, depth, int(dmax), seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
     end subroutine
 
     !> The time loop for a handle created by the caller.  forward is in/out

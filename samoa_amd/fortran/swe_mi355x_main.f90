@@ -193,7 +193,7 @@ program swe_mi355x_main
     case ('tsunami')
         if (dstart == dmax) then
             call tsunami_draw(seed, amp, ratio, x0, y0, depth_i, dmax, dmax)
-            call tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+            call tsunami_state(n, coords, depth, dmax, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
         end if
     end select
     if (.not. (trim(scenario) == 'tsunami' .and. dstart < dmax)) then

@@ -22,7 +22,8 @@ module swe_scenarios
     implicit none
     private
     public :: rng_t, rng_seed, rng_uniform, rng_normal, tsunami_draw, tsunami_state, &
-              grf_grid, centroid, tsunami_raster, raster_sample, TSU_G, TSU_DX, TSU_SCALING
+              grf_grid, centroid, tsunami_raster, raster_sample, raster_integrate, integrate_levels, &
+              TSU_G, TSU_DX, TSU_SCALING
 
     integer(c_int64_t), parameter :: M1 = 4294967087_c_int64_t, M2 = 4294944443_c_int64_t
     real(c_double), parameter :: PI = 3.14159265358979323846_c_double
@@ -169,12 +170,47 @@ contains
             + (1.0_c_double - wx) * wy * grid(ix + 1, iy + 2) + wx * wy * grid(ix + 2, iy + 2)
     end function
 
-    !> Tsunami state at the centroids of n cells (unit-square coordinates):
-    !> b from the raster, eta the elliptical Gaussian source (dry where it
-    !> lies below b + drytol).
-    subroutine tsunami_state(n, coords, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
+    !> get_bathymetry_at_element under _ADAPT_INTEGRATE for gridded data
+    !> (SWE_initialize.f90:106-147): over `levels` halvings of the cell
+    !> (x1, x2, x3) (unit square, x2 the right angle), 0.5 (value of
+    !> (x1, m, x2) + value of (x2, m, x3)) with m = 0.5 (x1 + x3); at the
+    !> bottom the raster at the world point scaling * (x1 + x2 + x3) / 3.
+    pure recursive real(c_double) function raster_integrate(nx, ny, grid, x0, y0, dx, dy, scaling, &
+                                                            x1, x2, x3, levels) result(v)
+        integer, intent(in)        :: nx, ny, levels
+        real(c_double), intent(in) :: grid(nx, ny), x0, y0, dx, dy, scaling, x1(2), x2(2), x3(2)
+        real(c_double) :: m(2)
+        if (levels > 0) then
+            m = 0.5_c_double * (x1 + x3)
+            v = 0.5_c_double * (raster_integrate(nx, ny, grid, x0, y0, dx, dy, scaling, x1, m, x2, &
+                                                 levels - 1) &
+                                + raster_integrate(nx, ny, grid, x0, y0, dx, dy, scaling, x2, m, x3, &
+                                                   levels - 1))
+        else
+            v = raster_sample(nx, ny, grid, x0, y0, dx, dy, scaling * ((x1 + x2 + x3) / 3.0_c_double))
+        end if
+    end function
+
+    !> Halvings of a cell at depth d for raster_integrate: the ASAGI branch of
+    !> SWE_initialize.f90:127-133, min(16, min(dmax + 3, data_depth + 3) - d),
+    !> with data_depth = nint(log(scaling**2 / (dx dy)) / log(2)).
+    pure integer function integrate_levels(d, dmax, scaling, dx, dy) result(l)
+        integer, intent(in)        :: d, dmax
+        real(c_double), intent(in) :: scaling, dx, dy
+        integer :: data_depth
+        data_depth = nint(log(scaling ** 2 / (dx * dy)) / log(2.0_c_double))
+        l = min(16, min(dmax + 3, data_depth + 3) - d)
+    end function
+
+    !> Tsunami state of n cells (unit-square corners, depths; dmax the mesh's
+    !> maximum depth): b from the raster averaged over the cell as the
+    !> reference's _ADAPT_INTEGRATE build does, eta the elliptical Gaussian
+    !> source at the centroid (dry where it lies below b + drytol).
+    subroutine tsunami_state(n, coords, depth, dmax, seed, amp, ratio, x0, y0, drytol, eta, hu, hv, b)
         integer(c_int64_t), intent(in) :: n, seed
         real(c_double), intent(in)     :: coords(2, 3, n), amp, ratio, x0, y0, drytol
+        integer(c_int8_t), intent(in)  :: depth(n)
+        integer, intent(in)            :: dmax
         real(c_double), intent(out)    :: eta(n), hu(n), hv(n), b(n)
         real(c_double), allocatable :: grid(:, :)
         real(c_double) :: x(2), a, c
@@ -185,7 +221,9 @@ contains
         c = a / ratio
         do i = 1, n
             x = TSU_SCALING * centroid(coords(:, :, i)) + 0.0_c_double
-            b(i) = raster_sample(TSU_G, TSU_G, grid, 0.0_c_double, 0.0_c_double, TSU_DX, TSU_DX, x)
+            b(i) = raster_integrate(TSU_G, TSU_G, grid, 0.0_c_double, 0.0_c_double, TSU_DX, TSU_DX, &
+                                    TSU_SCALING, coords(:, 1, i), coords(:, 2, i), coords(:, 3, i), &
+                                    integrate_levels(int(depth(i)), dmax, TSU_SCALING, TSU_DX, TSU_DX))
             eta(i) = amp * exp(-(((x(1) - x0) / a) ** 2 + ((x(2) - y0) / c) ** 2))
             hu(i) = 0.0_c_double
             hv(i) = 0.0_c_double

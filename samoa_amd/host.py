@@ -43,7 +43,8 @@ def load():
         L.swe_host_tsunami_draw.argtypes = [C.c_int64, C.c_int, C.c_int] + [C.POINTER(C.c_double)] * 4 + [
             C.POINTER(C.c_int)]
         L.swe_host_tsunami.restype = None
-        L.swe_host_tsunami.argtypes = [C.c_int64, _dp, C.c_int64] + [C.c_double] * 5 + [_dp] * 4
+        L.swe_host_tsunami.argtypes = [C.c_int64, _dp, _i8p, C.c_int, C.c_int64] + \
+            [C.c_double] * 5 + [_dp] * 4
         L.swe_host_time_loop.restype = C.c_int
         L.swe_host_time_loop.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int64,
                                          C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -157,16 +158,21 @@ def tsunami_draw(seed: int, dlo: int = 14, dhi: int = 20) -> dict:
 
 
 def tsunami(mesh: Mesh, seed: int, drytol: float = 1e-3, amplitude=None, ratio=None, x0=None,
-            y0=None):
-    """Tsunami scenario on the 1000 km square (use scaling 1e6)."""
+            y0=None, dmax=None):
+    """Tsunami scenario on the 1000 km square (use scaling 1e6).  b is the
+    raster averaged over each cell as the reference's _ADAPT_INTEGRATE build
+    does for gridded data, with dmax (default: the mesh's deepest cell) the
+    run's maximum depth."""
     p = tsunami_draw(seed)
     amp = p["amplitude"] if amplitude is None else amplitude
     rat = p["ratio"] if ratio is None else ratio
     xx = p["x0"] if x0 is None else x0
     yy = p["y0"] if y0 is None else y0
+    depth = np.ascontiguousarray(mesh.depth, np.int8)
+    dm = int(depth.max()) if dmax is None else int(dmax)
     out = [np.empty(mesh.n) for _ in range(4)]
-    load().swe_host_tsunami(mesh.n, np.ascontiguousarray(mesh.coords.reshape(-1)), seed, amp, rat,
-                            xx, yy, drytol, *out)
+    load().swe_host_tsunami(mesh.n, np.ascontiguousarray(mesh.coords.reshape(-1)), depth, dm, seed,
+                            amp, rat, xx, yy, drytol, *out)
     return tuple(out)
 
 

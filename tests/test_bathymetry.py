@@ -76,3 +76,62 @@ def test_cell_bathymetry_uses_centroids():
     b = bt.cell_bathymetry(r, mesh.coords, 10.0, (0.0, 0.0))
     c = 10.0 * mesh.coords.mean(axis=1)
     assert np.allclose(b, 10.0 * c[:, 0] / 2.5 + c[:, 1] / 2.0, atol=1e-12)
+
+
+def test_integrate_restatement_matches_reference_adaptive_run():
+    """The oracle's refine_2D_recursive (SWE_initialize.f90:106-147) with the
+    bathymetry function of the reference's patched scenario reproduces, bit
+    for bit, the b of every cell of the reference's adaptive run (depths
+    3-11, built with _ADAPT_INTEGRATE): 256 samples per depth-3 cell down to
+    the centroid at depth 11."""
+    import math
+
+    import golden_io
+    import pyoracle as po
+
+    z = golden_io.load("adapt_bath_d3_11.npz")
+    import json
+
+    meta = json.loads(str(z["meta"]))
+    S, off, dmax = meta["scaling"], meta["offset"], meta["dmax"]
+    depth = z["depth"]
+    b = z["states"][0, 3]
+    mesh = host.mesh_from_depths(depth)
+
+    def f(x, y):  # SWE_initialize.f90:185 as the reference's bath variant patches it
+        return -0.5 + 0.25 * (S * x + off) + 0.1 * math.cos(2.0 * (S * y + off))
+
+    got = np.array([po.integrate_bathymetry(f, *(tuple(c) for c in mesh.coords[i]),
+                                            po.integrate_levels(int(depth[i]), dmax))
+                    for i in range(mesh.n)])
+    assert (depth < dmax).sum() > 100
+    assert np.array_equal(got, b)
+    # the centroid alone misses the coarse cells: the recursion is what is pinned
+    cen = np.array([f(*(mesh.coords[i].sum(axis=0) / 3.0)) for i in range(mesh.n)])
+    assert not np.array_equal(cen[depth < dmax], b[depth < dmax])
+
+
+def test_raster_integration_of_the_host_paths():
+    """The tsunami scenario (Fortran) and cell_bathymetry (numpy) average the
+    raster as the oracle's recursion does, with the gridded-data levels
+    min(16, min(dmax + 3, data_depth + 3) - d)."""
+    import pyoracle as po
+
+    seed, S = 4, 1.0e6
+    r = host.tsunami_raster(seed)
+    rast = bt.Raster(r["values"], r["x0"], r["y0"], r["dx"], r["dy"])
+    dd = bt.data_depth(rast, S)
+    assert dd == po.data_depth(S, r["dx"], r["dy"]) == 18
+    f = po.bilinear(r["values"], r["x0"], r["y0"], r["dx"], r["dy"])
+    for d, dmax in ((12, 12), (15, 17), (6, 6)):
+        mesh = host.mesh_uniform(d)
+        idx = np.random.default_rng(d).choice(mesh.n, size=min(mesh.n, 40), replace=False)
+        levels = po.integrate_levels(d, dmax, dd)
+        assert levels == min(16, min(dmax + 3, 21) - d)
+        want = np.array([po.integrate_bathymetry(lambda x, y: f(S * x, S * y),
+                                                 *(tuple(c) for c in mesh.coords[i]), levels)
+                         for i in idx])
+        b = host.tsunami(mesh, seed=seed, dmax=dmax)[3]
+        assert np.array_equal(b[idx], want), d
+        cb = bt.cell_bathymetry(rast, mesh.coords[idx], S, 0.0, depth=mesh.depth[idx], dmax=dmax)
+        assert np.array_equal(cb, want), d

@@ -175,9 +175,10 @@ def test_adapt_errors():
 
 def test_split_cells_sample_the_bathymetry_raster():
     """With the tsunami scenario's raster set, the cells a refinement creates
-    take b from it at their centroids, bit-identical to the host's scenario
-    code on the finer mesh (the same bilinear expression), while a merge
-    still averages its pair."""
+    take b from it averaged over the cell as the reference's _ADAPT_INTEGRATE
+    build does (8 samples per cell here), bit-identical to the host's
+    scenario code on the finer mesh, while a merge still averages its
+    pair."""
     d = 10
     mesh = host.mesh_uniform(d)
     st = host.tsunami(mesh, seed=3, drytol=1e-3)
@@ -301,10 +302,14 @@ def test_run_continues_the_device_chain():
         op.close()
 
 
-def test_split_cells_sample_a_netcdf_grid():
+@pytest.mark.parametrize("d,dmax", [(9, 10), (4, 16)])
+def test_split_cells_sample_a_netcdf_grid(d, dmax):
     """A netCDF bathymetry grid placed with the reference's domain rule:
-    the cells a refinement creates take b from it, bit-identical to the
-    host's sampling at their centroids on the finer mesh."""
+    the cells a refinement creates take b from it averaged as the
+    reference's _ADAPT_INTEGRATE build does for gridded data, bit-identical
+    to the host's average on the finer mesh and, for some cells, to the
+    oracle's recursion; with a coarse mesh and a deep maximum depth each new
+    cell averages 2^9 samples."""
     import os
     import tempfile
 
@@ -320,14 +325,23 @@ def test_split_cells_sample_a_netcdf_grid():
         write_grid(f, x, y, z)
         r = bt.read_netcdf_grid(f)
     S, off = bt.asagi_domain(r)
-    d = 9
     mesh = host.mesh_uniform(d)
-    b = bt.cell_bathymetry(r, mesh.coords, S, off)
+    print("case: netCDF grid, depth", d, "dmax", dmax, flush=True)
+    b = bt.cell_bathymetry(r, mesh.coords, S, off, depth=mesh.depth, dmax=dmax)
     eta = np.zeros(mesh.n)
-    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3, min_depth=d, max_depth=d + 1)
+    op = swe.EulerTimestep(mesh, scaling=S, dry_tolerance=1e-3, min_depth=d, max_depth=dmax)
     op.upload(eta, eta, eta, b)
     op.set_bathymetry_raster(r.values, r.x0, r.y0, r.dx, r.dy, off)
     op.adapt(np.ones(mesh.n, np.int8))
     fine = host.mesh_uniform(d + 1)
-    assert np.array_equal(op.download()[3], bt.cell_bathymetry(r, fine.coords, S, off))
+    got = op.download()[3]
+    assert np.array_equal(got, bt.cell_bathymetry(r, fine.coords, S, off, depth=fine.depth,
+                                                  dmax=dmax))
+    levels = po.integrate_levels(d + 1, dmax, bt.data_depth(r, S))
+    assert levels == (3 if d == 9 else 9)
+    f = po.bilinear(r.values, r.x0, r.y0, r.dx, r.dy)
+    for i in (0, fine.n // 3, fine.n - 1):
+        want = po.integrate_bathymetry(lambda x, y: f(S * x + off[0], S * y + off[1]),
+                                       *(tuple(c) for c in fine.coords[i]), levels)
+        assert got[i] == want
     op.close()
