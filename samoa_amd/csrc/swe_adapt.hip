@@ -786,8 +786,8 @@ static hipError_t wait_results(const int64_t *host, int64_t seq, hipStream_t s) 
 }
 
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
-                      int max_sweeps, int64_t *host, int64_t *host_dev, int64_t *seq,
-                      AdaptPlan *plan) {
+                      bool prefix_ahead, int max_sweeps, int64_t *host, int64_t *host_dev,
+                      int64_t *seq, AdaptPlan *plan) {
     const unsigned nb = blocks_for(a.n, AT);
     const int64_t tiles = adapt_scan_tiles(a.n);
     int64_t *res = a.bsum + tiles + 1;
@@ -814,6 +814,12 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
             count_reduce_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(
                 a.n, a.depth, a.nbr, a.slots, a.off, a.code, a.bsum, a.status);
         plan_top_kernel<<<1, TOP_T, 0, s>>>(a.bsum, tiles, a.status, res, host_dev, ++*seq);
+        // the prefix pass needs only the count pass: behind it, it runs
+        // while the host reads the results (a batch short of the fixpoint
+        // is counted again, and so is its prefix)
+        if (prefix_ahead && tiles > 0)
+            scan_codes_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(a.n, a.code, a.bsum, a.base,
+                                                                 a.kidx);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = wait_results(host, *seq, s)) != hipSuccess) return e;
         if (!(host[2] & 4)) {
@@ -823,6 +829,7 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
         batch = 2;
     }
     if (plan->sweeps < 0) return hipSuccess;
+    plan->prefix_done = prefix_ahead;
     plan->last_change = (int)host[3];
     plan->tree_total = offsets_valid ? (int64_t(1) << 61) : host[0];
     plan->n_new = host[1];
@@ -853,9 +860,9 @@ hipError_t identity_record(int64_t n, int8_t *code, int64_t *base, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s) {
+hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s, bool prefix_done) {
     const int64_t tiles = adapt_scan_tiles(a.n);
-    if (tiles > 0)
+    if (tiles > 0 && !prefix_done)
         scan_codes_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(a.n, a.code, a.bsum, a.base, a.kidx);
     emit_kernel<<<blocks_for(a.n, AT), AT, 0, s>>>(a);
     hipError_t e = hipGetLastError();

@@ -233,6 +233,7 @@ struct Batch {
     int64_t *ad_host_dev = nullptr;  // its device address
     int64_t ad_seq = 0;         // number of the last results
     int ad_last_sweeps = 2;     // sweeps that changed a word in the last remeshing
+    bool ad_last_changed = false;  // the last remeshing changed the mesh
     int64_t ad_record_n = -1;      // old cells of the last remeshing (its record in ad_code/ad_base)
     bool ad_record_sampled = false;  // its split cells sampled the raster
     double *raster = nullptr;  // bathymetry raster for split cells (swe_set_bathymetry_raster)
@@ -724,8 +725,9 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
         a.raster_cap = (int)std::min<long>(sc.dmax + 3, data_depth + 3);
     }
     swe::AdaptPlan plan{};
-    HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, 1 << 16, B.ad_host,
-                            B.ad_host_dev, &B.ad_seq, &plan));
+    HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, B.ad_last_changed,
+                            1 << 16, B.ad_host, B.ad_host_dev, &B.ad_seq, &plan));
+    B.ad_last_changed = (plan.status & 2u) != 0;
     const int sweeps = plan.sweeps;
     if (sweeps < 0) return fail(SWE_ERR_STATE, "adapt: the conformity sweeps did not converge");
     // the next call starts with as many sweeps as changed a word this time
@@ -817,7 +819,7 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.nbr_out = fresh ? no : B.nbr2;
     a.plotter_out = fresh ? po : B.plotter2;
     a.depth_out = fresh ? dpo : B.depth2;
-    hipError_t e = swe::adapt_emit(a, B.stream);
+    hipError_t e = swe::adapt_emit(a, B.stream, plan.prefix_done);
     if (e != hipSuccess) {
         if (fresh) drop();
         return fail(SWE_ERR_HIP, "adapt: %s", hipGetErrorString(e));

@@ -127,6 +127,7 @@ int64_t adapt_scan_tiles(int64_t n);
 struct AdaptPlan {
     int sweeps;          // conformity sweeps run, -1 if the fixpoint was not reached in max_sweeps
     int last_change;     // the last of them that changed a word (0: none did)
+    bool prefix_done;    // base and kidx are written (prefix_ahead)
     int64_t n_new;       // cells of the new mesh
     int64_t tree_total;  // covered part of the two root trees (2^61 when the mesh is whole)
     uint32_t status;     // bit 0: an inconsistent merge; bit 1: the mesh changes
@@ -135,14 +136,16 @@ struct AdaptPlan {
 // Marks the edges, sweeps to the conformity fixpoint (first_batch sweeps,
 // then 2 at a time), decides every cell (code) and sums the new-cell counts
 // per scan tile; the count pass confirms the fixpoint.  offsets_valid: a.off
-// still holds this mesh's tree offsets.  host: 8 words of coherent pinned
+// still holds this mesh's tree offsets.  prefix_ahead: launch the prefix of
+// the counts behind every count pass, before the host knows whether the
+// mesh changes (it runs while the host reads).  host: 8 words of coherent pinned
 // memory for the results, host_dev its device address; *seq numbers the
 // results (incremented per batch).  Waits for the device once per batch.
 // adapt_emit then writes the prefix of the counts (base; its high word
 // counts the cells that change) and the new mesh.
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
-                      int max_sweeps, int64_t *host, int64_t *host_dev, int64_t *seq,
-                      AdaptPlan *plan);
+                      bool prefix_ahead, int max_sweeps, int64_t *host, int64_t *host_dev,
+                      int64_t *seq, AdaptPlan *plan);
 // Refinement flags from surface-elevation jumps to wet neighbours.
 hipError_t flag_eta_jumps(int64_t n, const double *eta, const double *b, const int32_t *nbr,
                           const int8_t *depth, double drytol, double refine_tol,
@@ -155,7 +158,8 @@ hipError_t transfer_adjoint(int64_t n_old, const int8_t *code, const int64_t *ba
 // The record of a remeshing that changed nothing.
 hipError_t identity_record(int64_t n, int8_t *code, int64_t *base, hipStream_t s);
 // Writes the new mesh and state (outputs sized for n_new).
-hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s);
+// prefix_done: adapt_plan has launched the prefix pass already.
+hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s, bool prefix_done);
 
 hipError_t upload_tables(const double *normal, const double *leg);
 hipError_t upload_tables_adjoint(const double *normal, const double *leg);
