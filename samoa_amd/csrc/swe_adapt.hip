@@ -563,6 +563,7 @@ __device__ __forceinline__ int32_t edge_slot(const AdaptArgs &a, int64_t c, int 
 __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
     const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
     if (c >= a.n) return;
+    if (c == 0 && a.scen_dev) *a.scen_dev = a.scen_new;  // no host upload of the table
     const int32_t kc = a.kidx[c];
     const int d = a.depth[c];
     const int p = a.plotter[c];

@@ -297,7 +297,9 @@ bool all_uploaded(const Batch &B) {
 
 // Chunk tables for the current chunk size: chunk offsets per scenario and the
 // chunk -> scenario map (not needed for a single scenario).
-int rebuild_chunks(Batch &B) {
+// upload = false: the device copy of the scenario table is already
+// current (the remesher's emit kernel wrote it).
+int rebuild_chunks(Batch &B, bool upload = true) {
     int64_t off = 0;
     for (auto &s : B.scen) {
         s.chunk_off = off;
@@ -332,8 +334,9 @@ int rebuild_chunks(Batch &B) {
         HIP_TRY(hipMemcpy(B.d_chunk_scen, map.data(), sizeof(int32_t) * map.size(),
                           hipMemcpyHostToDevice));
     }
-    HIP_TRY(hipMemcpyAsync(B.d_scen, B.scen.data(), sizeof(swe::Scen) * B.scen.size(),
-                           hipMemcpyHostToDevice, B.stream));
+    if (upload)
+        HIP_TRY(hipMemcpyAsync(B.d_scen, B.scen.data(), sizeof(swe::Scen) * B.scen.size(),
+                               hipMemcpyHostToDevice, B.stream));
     if (B.graph.exec) {
         (void)hipGraphExecDestroy(B.graph.exec);
         B.graph = GraphCache{};
@@ -819,6 +822,14 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.nbr_out = fresh ? no : B.nbr2;
     a.plotter_out = fresh ? po : B.plotter2;
     a.depth_out = fresh ? dpo : B.depth2;
+    // the new mesh's table entry, as rebuild_chunks lays it out for one mesh
+    swe::Scen sn = B.scen[0];
+    sn.n = n_new;
+    sn.depth = -1;
+    sn.chunk_off = 0;
+    sn.nchunks = (n_new + B.chunk - 1) / B.chunk;
+    a.scen_dev = B.d_scen;
+    a.scen_new = sn;
     hipError_t e = swe::adapt_emit(a, B.stream, plan.prefix_done);
     if (e != hipSuccess) {
         if (fresh) drop();
@@ -864,11 +875,10 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     std::swap(B.ad_off_cap, B.ad_off2_cap);
     B.ad_off_valid = true;
     B.ncells = B.ntopo = n_new;
-    B.scen[0].n = n_new;
-    B.scen[0].depth = -1;
+    B.scen[0] = sn;
     B.uniform_depth[0] = -1;
     B.flags_valid = false;
-    SWE_TRY(rebuild_chunks(B));
+    SWE_TRY(rebuild_chunks(B, false));
     if (n_out) *n_out = n_new;
     if (sweeps_out) *sweeps_out = sweeps;
     return SWE_OK;

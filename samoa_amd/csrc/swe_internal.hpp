@@ -111,6 +111,8 @@ struct AdaptArgs {
     int8_t *depth_out, *plotter_out;       // [n'] new mesh
     int32_t *nbr_out;                      // [3n']
     double *eta_out, *hu_out, *hv_out, *b_out;
+    Scen *scen_dev;                        // non-null: emit stores scen_new there (the
+    Scen scen_new;                         // new mesh's entry of the step kernels' table)
     // bathymetry of split cells: bilinear in a raster of nx x ny values at
     // world points (x0 + i dx, y0 + j dy), values[j nx + i], at world points
     // scaling * c + offset of unit-square points c; null: the parent's b
