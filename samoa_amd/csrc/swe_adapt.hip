@@ -98,29 +98,17 @@ __device__ __forceinline__ uint32_t edge_value(const uint32_t *slots, const int3
     return ((s | t) & 1u) | (s & t & 2u);
 }
 
-// mark_edges (Conformity.f90:1244-1268): the hypotenuse is never coarsened,
-// a cell that keeps its depth clears its legs' coarsen bits, a refinement
-// request marks the hypotenuse.  Requests beyond [dmin, dmax] are dropped as
-// volume_op (SWE_euler_timestep.f90:339-342) never issues them.
-__global__ __launch_bounds__(AT) void conf_init_kernel(int64_t n, const int8_t *depth,
-                                                       const int8_t *flags, int dmin, int dmax,
-                                                       uint32_t *slots, uint32_t *status,
-                                                       uint32_t *stamp) {
-    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
-    if (c == 0) {
-        status[0] = 0u;
-        status[1] = 0u;
-        status[2] = 0u;
-    }
-    if (threadIdx.x == 0) stamp[blockIdx.x] = 0u;  // the sweeps' tiles are these workgroups
-    if (c >= n) return;
-    int f = flags[c];
-    const int d = depth[c];
+// mark_edges (Conformity.f90:1244-1268): the word a request f gives a cell
+// of depth d.  The hypotenuse is never coarsened, a cell that keeps its
+// depth clears its legs' coarsen bits, a refinement request marks the
+// hypotenuse.  Requests beyond [dmin, dmax] are dropped as volume_op
+// (SWE_euler_timestep.f90:339-342) never issues them.
+__device__ __forceinline__ uint32_t request_word(int f, int d, int dmin, int dmax) {
     if (f > 0 && d >= dmax) f = 0;
     if (f < 0 && d <= dmin) f = 0;
     const uint32_t legs = f != 0 ? 2u : 0u;
     const uint32_t hyp = f > 0 ? 1u : 0u;
-    slots[c] = legs | (hyp << 2) | (legs << 4);
+    return legs | (hyp << 2) | (legs << 4);
 }
 
 // The word the integrity rule gives a cell from the values (bit 0 refine,
@@ -139,18 +127,26 @@ __device__ __forceinline__ uint32_t integrity_word(uint32_t v0, uint32_t v1, uin
 // tile settle in one sweep.  Every update is monotone, which keeps the global
 // fixpoint the same.
 //
-// stamp[T] is the last sweep in which a cell next to tile T but outside it
-// changed (or T itself did not settle).  From the second sweep on, a tile
-// whose stamp is older than the previous sweep has read every neighbour word
-// since it last changed and is at its fixpoint: it is skipped.  *changed gets
-// the sweep's number if a word changed.
+// The first sweep of a remeshing starts from the requests: every word, its
+// own and an outside neighbour's, is request_word of the flags, an earlier
+// state than any other tile may have written; it writes every word of its
+// tile.
+//
+// Sweep numbers carry the call's tag in their high 16 bits.  stamp[T] is the
+// last sweep in which a cell next to tile T but outside it changed (or T
+// itself did not settle).  From the second sweep on, a tile whose stamp is
+// older than the previous sweep (a stamp of an earlier call included) has
+// read every neighbour word since it last changed and is at its fixpoint: it
+// is skipped.  *changed gets the sweep if a word changed.
 __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t *depth,
                                                         const int32_t *nbr, uint32_t *slots,
                                                         uint32_t *stamp, uint32_t *changed,
-                                                        uint32_t sweep) {
+                                                        uint32_t sweep, const int8_t *flags,
+                                                        int dmin, int dmax) {
     __shared__ uint32_t w[AT];
     __shared__ uint32_t s_skip;
-    if (sweep > 1) {
+    const bool first = (sweep & 0xffffu) == 1u;
+    if (!first) {
         if (threadIdx.x == 0) s_skip = ld_relaxed(&stamp[blockIdx.x]) + 1u < sweep ? 1u : 0u;
         __syncthreads();
         if (s_skip) return;  // uniform across the workgroup
@@ -162,11 +158,11 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
     int d = 0;
     uint32_t own = 0u;
     if (live) {
-        own = ld_relaxed(&slots[c]);
         q[0] = nbr[3 * c];
         q[1] = nbr[3 * c + 1];
         q[2] = nbr[3 * c + 2];
         d = depth[c];
+        own = first ? request_word(flags[c], d, dmin, dmax) : ld_relaxed(&slots[c]);
     }
     const uint32_t start = own;
     w[threadIdx.x] = own;
@@ -184,7 +180,12 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
                 v[k] = sk;
                 if (q[k] >= 0) {
                     const int64_t o = q[k] >> 2;
-                    const uint32_t ow = (o >= t0 && o < t0 + AT) ? w[o - t0] : ld_relaxed(&slots[o]);
+                    uint32_t ow;
+                    if (o >= t0 && o < t0 + AT)
+                        ow = w[o - t0];
+                    else
+                        ow = first ? request_word(flags[o], depth[o], dmin, dmax)
+                                   : ld_relaxed(&slots[o]);
                     const uint32_t tk = (ow >> (2 * (q[k] & 3))) & 3u;
                     v[k] = ((sk | tk) & 1u) | (sk & tk & 2u);
                 }
@@ -203,8 +204,8 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
         }
     }
     const bool diff = own != start;
+    if (live && (diff || first)) st_relaxed(&slots[c], own);
     if (diff) {
-        st_relaxed(&slots[c], own);
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (q[k] < 0) continue;
@@ -787,15 +788,12 @@ static hipError_t wait_results(const int64_t *host, int64_t seq, hipStream_t s) 
 }
 
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
-                      bool prefix_ahead, int max_sweeps, int64_t *host, int64_t *host_dev,
-                      int64_t *seq, AdaptPlan *plan) {
+                      bool prefix_ahead, int max_sweeps, uint32_t epoch, int64_t *host,
+                      int64_t *host_dev, int64_t *seq, AdaptPlan *plan) {
     const unsigned nb = blocks_for(a.n, AT);
     const int64_t tiles = adapt_scan_tiles(a.n);
     int64_t *res = a.bsum + tiles + 1;
     hipError_t e;
-    conf_init_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.flags, a.dmin, a.dmax, a.slots, a.status,
-                                       a.stamp);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
     if (!offsets_valid) {  // else the offsets of the unchanged mesh are in place
         tree_size_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.off);
         if ((e = scan_i64(a.off, a.off, a.n, a.bsum, s)) != hipSuccess) return e;
@@ -805,11 +803,14 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
     // read: the count pass also tells whether the words are at the fixpoint,
     // so no sweep is spent on confirming it.
     plan->sweeps = -1;
+    const uint32_t tag = (epoch & 0xffffu) << 16;
+    if (max_sweeps > 0xffff) max_sweeps = 0xffff;
     int batch = first_batch < 1 ? 1 : first_batch, done = 0;
     while (done < max_sweeps) {
         for (int j = 1; j <= batch; ++j)
             conf_sweep_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.stamp,
-                                                a.status + 1, (uint32_t)(done + j));
+                                                a.status + 1, tag | (uint32_t)(done + j),
+                                                a.flags, a.dmin, a.dmax);
         done += batch;
         if (tiles > 0)
             count_reduce_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(
@@ -831,7 +832,8 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
     }
     if (plan->sweeps < 0) return hipSuccess;
     plan->prefix_done = prefix_ahead;
-    plan->last_change = (int)host[3];
+    // a last change of an earlier call (another tag) means none in this one
+    plan->last_change = ((uint32_t)host[3] >> 16) == (tag >> 16) ? (int)(host[3] & 0xffff) : 0;
     plan->tree_total = offsets_valid ? (int64_t(1) << 61) : host[0];
     plan->n_new = host[1];
     plan->status = (uint32_t)host[2] & 3u;

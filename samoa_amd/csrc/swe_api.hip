@@ -234,6 +234,7 @@ struct Batch {
     int64_t ad_seq = 0;         // number of the last results
     int ad_last_sweeps = 2;     // sweeps that changed a word in the last remeshing
     bool ad_last_changed = false;  // the last remeshing changed the mesh
+    uint32_t ad_epoch = 0;      // number of the current remeshing
     int64_t ad_record_n = -1;      // old cells of the last remeshing (its record in ad_code/ad_base)
     bool ad_record_sampled = false;  // its split cells sampled the raster
     double *raster = nullptr;  // bathymetry raster for split cells (swe_set_bathymetry_raster)
@@ -648,6 +649,8 @@ int ensure_adapt_scratch(Batch &B, int64_t n) {
     B.ad_off_valid = false;
     SWE_TRY(dev_alloc(&B.ad_slots, cap));
     SWE_TRY(dev_alloc(&B.ad_status, 4 + cap / 256 + 1));  // status words, then sweep stamps
+    HIP_TRY(hipMemsetAsync(B.ad_status, 0, sizeof(uint32_t) * (size_t)(4 + cap / 256 + 1),
+                           B.stream));
     SWE_TRY(dev_alloc(&B.ad_off, cap));
     SWE_TRY(dev_alloc(&B.ad_off2, cap));
     B.ad_off_cap = B.ad_off2_cap = cap;
@@ -729,7 +732,7 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     }
     swe::AdaptPlan plan{};
     HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, B.ad_last_changed,
-                            1 << 16, B.ad_host, B.ad_host_dev, &B.ad_seq, &plan));
+                            1 << 16, ++B.ad_epoch, B.ad_host, B.ad_host_dev, &B.ad_seq, &plan));
     B.ad_last_changed = (plan.status & 2u) != 0;
     const int sweeps = plan.sweeps;
     if (sweeps < 0) return fail(SWE_ERR_STATE, "adapt: the conformity sweeps did not converge");

@@ -103,7 +103,7 @@ struct AdaptArgs {
     int32_t *kidx;                         // [n] new index of a kept cell, -1 if it changes
     int32_t *eslot;                        // [6n] new cell-edge on each half of each old edge
                                            //      (changing cells only)
-    uint32_t *status;                      // [3]
+    uint32_t *status;                      // [3], zero before the first call
     uint32_t *stamp;                       // [n / 256 + 1] per tile of the conformity sweeps
     int32_t *list;                         // [n] the changing cells (may alias slots)
     int64_t n_changing;                    // their number (from adapt_plan)
@@ -135,19 +135,21 @@ struct AdaptPlan {
     uint32_t status;     // bit 0: an inconsistent merge; bit 1: the mesh changes
     int64_t n_changing;  // cells that split or merge
 };
-// Marks the edges, sweeps to the conformity fixpoint (first_batch sweeps,
-// then 2 at a time), decides every cell (code) and sums the new-cell counts
-// per scan tile; the count pass confirms the fixpoint.  offsets_valid: a.off
-// still holds this mesh's tree offsets.  prefix_ahead: launch the prefix of
-// the counts behind every count pass, before the host knows whether the
-// mesh changes (it runs while the host reads).  host: 8 words of coherent pinned
-// memory for the results, host_dev its device address; *seq numbers the
-// results (incremented per batch).  Waits for the device once per batch.
-// adapt_emit then writes the prefix of the counts (base; its high word
-// counts the cells that change) and the new mesh.
+// Marks the edges and sweeps to the conformity fixpoint (first_batch sweeps,
+// then 2 at a time; the first sweep starts from the requests), decides every
+// cell (code) and sums the new-cell counts per scan tile; the count pass
+// confirms the fixpoint.  offsets_valid: a.off still holds this mesh's tree
+// offsets.  prefix_ahead: launch the prefix of the counts behind every count
+// pass, before the host knows whether the mesh changes (it runs while the
+// host reads).  epoch: a number that differs from the last few calls' (it
+// tags their sweep stamps, so that no pass has to clear them).  host: 8
+// words of coherent pinned memory for the results, host_dev its device
+// address; *seq numbers the results (incremented per batch).  Waits for the
+// device once per batch.  adapt_emit then writes the prefix of the counts
+// (base; its high word counts the cells that change) and the new mesh.
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
-                      bool prefix_ahead, int max_sweeps, int64_t *host, int64_t *host_dev,
-                      int64_t *seq, AdaptPlan *plan);
+                      bool prefix_ahead, int max_sweeps, uint32_t epoch, int64_t *host,
+                      int64_t *host_dev, int64_t *seq, AdaptPlan *plan);
 // Refinement flags from surface-elevation jumps to wet neighbours.
 hipError_t flag_eta_jumps(int64_t n, const double *eta, const double *b, const int32_t *nbr,
                           const int8_t *depth, double drytol, double refine_tol,
