@@ -406,12 +406,13 @@ __global__ __launch_bounds__(SCAN_T) void count_reduce_kernel(int64_t n, const i
 }
 
 // base[c] = exclusive prefix of the counts, rebuilt from the decision codes
-// (8 consecutive cells per thread, bsum holding the scanned tile sums), and
+// (8 consecutive cells per thread, bsum holding the scanned tile sums),
 // kidx[c] = the new index of a cell that keeps its place, -1 for one that
-// changes.  The prefixes go through LDS so that the stores are coalesced.
+// changes, and list = the changing cells in curve order.  The prefixes go
+// through LDS so that the stores are coalesced.
 __global__ __launch_bounds__(SCAN_T) void scan_codes_kernel(int64_t n, const int8_t *code,
                                                             const int64_t *bsum, int64_t *base,
-                                                            int32_t *kidx) {
+                                                            int32_t *kidx, int32_t *list) {
     __shared__ int64_t lds[SCAN_T / 64 + 1];
     __shared__ int64_t stage[SCAN_TILE + SCAN_TILE / SCAN_ITEMS];  // item i at i + i / 8
     __shared__ uint8_t kstage[SCAN_TILE + SCAN_TILE / SCAN_ITEMS];  // 1: the cell is kept
@@ -439,6 +440,8 @@ __global__ __launch_bounds__(SCAN_T) void scan_codes_kernel(int64_t n, const int
     for (int j = 0; j < SCAN_ITEMS; ++j) {
         stage[threadIdx.x * (SCAN_ITEMS + 1) + j] = run;
         kstage[threadIdx.x * (SCAN_ITEMS + 1) + j] = v[j] == 1 ? 1 : 0;
+        // the changing cells, in order: the high word counts those before
+        if (v[j] >> 32) list[run >> 32] = (int32_t)(first + j);
         run += v[j];
     }
     __syncthreads();
@@ -468,17 +471,21 @@ __device__ __forceinline__ int split_type(int p, int rule) {
 }
 
 // Corners (unit square; entry, right angle, exit) of the cell at tree offset
-// `off` and depth d: the corners of its root, then d halvings along the path
-// the offset spells (bit 60 - l chooses the second child at level l), with
-// the host's arithmetic, m = 0.5 (x1 + x3).
-__device__ void tree_corners(int64_t off, int d, double c[6]) {
-    double ax, ay, bx, by, ex, ey;  // corners 1, 2, 3
-    if ((off >> 60) == 0) {
-        ax = 0.0; ay = 0.0; bx = 1.0; by = 0.0; ex = 1.0; ey = 1.0;
-    } else {
-        ax = 1.0; ay = 1.0; bx = 0.0; by = 1.0; ex = 0.0; ey = 0.0;
+// `off` and depth d, from the corners c of its ancestor at depth d0 (or of
+// its root, d0 = 0): one halving per level along the path the offset spells
+// (bit 60 - l chooses the second child at level l), with the host's
+// arithmetic, m = 0.5 (x1 + x3).  Starting from an ancestor runs the same
+// operations as starting from the root.
+__device__ void tree_corners(int64_t off, int d0, int d, double c[6]) {
+    if (d0 == 0) {
+        if ((off >> 60) == 0) {
+            c[0] = 0.0; c[1] = 0.0; c[2] = 1.0; c[3] = 0.0; c[4] = 1.0; c[5] = 1.0;
+        } else {
+            c[0] = 1.0; c[1] = 1.0; c[2] = 0.0; c[3] = 1.0; c[4] = 0.0; c[5] = 0.0;
+        }
     }
-    for (int l = 1; l <= d; ++l) {
+    double ax = c[0], ay = c[1], bx = c[2], by = c[3], ex = c[4], ey = c[5];
+    for (int l = d0 + 1; l <= d; ++l) {
         const double mx = 0.5 * (ax + ex), my = 0.5 * (ay + ey);
         if ((off >> (60 - l)) & 1) {  // (x2, m, x3)
             ax = bx;
@@ -505,7 +512,7 @@ __device__ double raster_sample(const AdaptArgs &a, double X, double Y) {
            wx * wy * r1[1];
 }
 
-// b of a new cell at tree offset `off` and depth d from the raster, as
+// b of a new cell of corners c and depth d from the raster, as
 // get_bathymetry_at_element computes it under _ADAPT_INTEGRATE
 // (SWE_initialize.f90:106-147) for gridded data: L = min(16, raster_cap - d)
 // halvings of the cell (raster_cap = min(dmax + 3, data_depth + 3)), each
@@ -515,10 +522,12 @@ __device__ double raster_sample(const AdaptArgs &a, double X, double Y) {
 // leaves run in depth-first order, a leaf's corners found by L halvings
 // along its index; a stack of partial values combines each finished pair as
 // the recursion does.
-__device__ double raster_cell(const AdaptArgs &a, int64_t off, int d) {
-    double c[6];
-    tree_corners(off, d, c);
+__device__ double raster_cell(const AdaptArgs &a, const double c[6], int d) {
     const int levels = min(16, a.raster_cap - d);
+    if (levels <= 0) {  // a cell finer than the data: its centroid (no stack)
+        const double cx = (c[0] + c[2] + c[4]) / 3.0, cy = (c[1] + c[3] + c[5]) / 3.0;
+        return raster_sample(a, a.scaling * cx + a.offset_x, a.scaling * cy + a.offset_y);
+    }
     double stack[17];
     int sp = 0;
     const int leaves = levels > 0 ? 1 << levels : 1;
@@ -561,38 +570,51 @@ __device__ __forceinline__ int32_t edge_slot(const AdaptArgs &a, int64_t c, int 
     return a.eslot[6 * c + 2 * k + s];
 }
 
-__global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
+// The cells that keep their place: a shifted copy of depth, plotter type,
+// tree offset and state, and the neighbours that keep their place too.
+__global__ __launch_bounds__(AT) void emit_keep_kernel(AdaptArgs a) {
     const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
     if (c >= a.n) return;
     if (c == 0 && a.scen_dev) *a.scen_dev = a.scen_new;  // no host upload of the table
+    // every load is issued before the test, so that they are in flight
+    // together (a changing cell, one in twenty, reads them for nothing)
     const int32_t kc = a.kidx[c];
+    const int8_t dc = a.depth[c], pc = a.plotter[c];
+    const int64_t oc = a.off[c];
+    const double ec = a.eta[c], uc = a.hu[c], vc = a.hv[c], bc = a.b[c];
+    const int32_t q3[3] = {a.nbr[3 * c], a.nbr[3 * c + 1], a.nbr[3 * c + 2]};
+    if (kc < 0) return;  // a changing cell: emit_change_kernel
+    const int64_t b0 = kc;
+    a.depth_out[b0] = dc;
+    a.plotter_out[b0] = pc;
+    a.off_out[b0] = oc;
+    a.eta_out[b0] = ec;
+    a.hu_out[b0] = uc;
+    a.hv_out[b0] = vc;
+    a.b_out[b0] = bc;
+    // an edge to a cell that keeps its place too keeps its pairing; the
+    // others are written by link_kernel from the changing side
+    for (int e = 0; e < 3; ++e) {
+        const int32_t q = q3[e];
+        if (q < 0) {
+            a.nbr_out[3 * b0 + e] = -1;
+        } else {
+            const int32_t ko = a.kidx[q >> 2];
+            if (ko >= 0) a.nbr_out[3 * b0 + e] = (ko << 2) | (q & 3);
+        }
+    }
+}
+
+// The cells that split or merge (list[i], in curve order): their new cells,
+// and for every half of every old edge the new cell-edge that lies on it.
+__global__ __launch_bounds__(AT) void emit_change_kernel(AdaptArgs a) {
+    const int64_t i = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (i >= a.n_changing) return;
+    const int64_t c = a.list[i];
     const int d = a.depth[c];
     const int p = a.plotter[c];
-    if (kc >= 0) {
-        const int64_t b0 = kc;
-        a.depth_out[b0] = (int8_t)d;
-        a.plotter_out[b0] = (int8_t)p;
-        a.off_out[b0] = a.off[c];
-        a.eta_out[b0] = a.eta[c];
-        a.hu_out[b0] = a.hu[c];
-        a.hv_out[b0] = a.hv[c];
-        a.b_out[b0] = a.b[c];
-        // an edge to a cell that keeps its place too keeps its pairing; the
-        // others are written by link_kernel from the changing side
-        for (int e = 0; e < 3; ++e) {
-            const int32_t q = a.nbr[3 * c + e];
-            if (q < 0) {
-                a.nbr_out[3 * b0 + e] = -1;
-            } else {
-                const int32_t ko = a.kidx[q >> 2];
-                if (ko >= 0) a.nbr_out[3 * b0 + e] = (ko << 2) | (q & 3);
-            }
-        }
-        return;
-    }
     const int8_t k = a.code[c];
     const int64_t b0 = a.base[c] & 0xffffffffLL;  // the high word counts changing cells
-    a.list[a.base[c] >> 32] = (int32_t)c;  // the changing cells, in order
     if (k == CODE_MERGE_FIRST) {
         a.depth_out[b0] = (int8_t)(d - 1);
         a.plotter_out[b0] = c_child[p + 8];  // the parent's type
@@ -615,6 +637,8 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
         const int pat = k - CODE_SPLIT;
         const int cells = 2 + (pat & 1) + (pat >> 1);
         int64_t coff = a.off[c];  // tree offset of the next child
+        double pc[6];             // the parent's corners, for the raster
+        if (a.raster) tree_corners(coff, 0, d, pc);
         for (int j = 0; j < cells; ++j) {
             const SplitCell sc = c_split[pat][j];
             const int64_t idx = b0 + j;
@@ -626,7 +650,9 @@ __global__ __launch_bounds__(AT) void emit_kernel(AdaptArgs a) {
             a.hu_out[idx] = a.hu[c];
             a.hv_out[idx] = a.hv[c];
             if (a.raster) {
-                a.b_out[idx] = raster_cell(a, coff, dc);
+                double cc[6] = {pc[0], pc[1], pc[2], pc[3], pc[4], pc[5]};
+                tree_corners(coff, d, dc, cc);
+                a.b_out[idx] = raster_cell(a, cc, dc);
             } else {
                 a.b_out[idx] = a.b[c];
             }
@@ -821,7 +847,7 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
         // is counted again, and so is its prefix)
         if (prefix_ahead && tiles > 0)
             scan_codes_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(a.n, a.code, a.bsum, a.base,
-                                                                 a.kidx);
+                                                                 a.kidx, a.list);
         if ((e = hipGetLastError()) != hipSuccess) return e;
         if ((e = wait_results(host, *seq, s)) != hipSuccess) return e;
         if (!(host[2] & 4)) {
@@ -866,10 +892,12 @@ hipError_t identity_record(int64_t n, int8_t *code, int64_t *base, hipStream_t s
 hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s, bool prefix_done) {
     const int64_t tiles = adapt_scan_tiles(a.n);
     if (tiles > 0 && !prefix_done)
-        scan_codes_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(a.n, a.code, a.bsum, a.base, a.kidx);
-    emit_kernel<<<blocks_for(a.n, AT), AT, 0, s>>>(a);
+        scan_codes_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(a.n, a.code, a.bsum, a.base, a.kidx,
+                                                             a.list);
+    emit_keep_kernel<<<blocks_for(a.n, AT), AT, 0, s>>>(a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.n_changing == 0) return e;
+    emit_change_kernel<<<blocks_for(a.n_changing, AT), AT, 0, s>>>(a);
     link_kernel<<<blocks_for(a.n_changing, AT), AT, 0, s>>>(a);
     return hipGetLastError();
 }

@@ -226,6 +226,7 @@ struct Batch {
     int64_t ad_off_cap = 0, ad_off2_cap = 0;
     int8_t *ad_code = nullptr;
     int32_t *ad_kidx = nullptr;  // new index of each kept cell, -1 for a changing one
+    int32_t *ad_list = nullptr;  // the changing cells
     int32_t *ad_eslot = nullptr;
     int64_t ad_cap = 0;
     bool ad_off_valid = false;  // ad_off holds the current mesh's tree offsets
@@ -286,6 +287,7 @@ void release(Batch &B) {
     (void)hipFree(B.ad_bsum);
     (void)hipFree(B.ad_code);
     (void)hipFree(B.ad_kidx);
+    (void)hipFree(B.ad_list);
     (void)hipFree(B.ad_eslot);
     (void)hipFree(B.raster);
     if (B.ad_host) (void)hipHostFree(B.ad_host);
@@ -638,8 +640,10 @@ int ensure_adapt_scratch(Batch &B, int64_t n) {
     (void)hipFree(B.ad_bsum);
     (void)hipFree(B.ad_code);
     (void)hipFree(B.ad_kidx);
+    (void)hipFree(B.ad_list);
     (void)hipFree(B.ad_eslot);
     B.ad_kidx = nullptr;
+    B.ad_list = nullptr;
     B.ad_slots = B.ad_status = nullptr;
     B.ad_off = B.ad_off2 = B.ad_base = B.ad_bsum = nullptr;
     B.ad_off_cap = B.ad_off2_cap = 0;
@@ -658,6 +662,7 @@ int ensure_adapt_scratch(Batch &B, int64_t n) {
     SWE_TRY(dev_alloc(&B.ad_bsum, swe::adapt_scan_tiles(cap) + 1 + 8));
     SWE_TRY(dev_alloc(&B.ad_code, cap));
     SWE_TRY(dev_alloc(&B.ad_kidx, cap));
+    SWE_TRY(dev_alloc(&B.ad_list, cap));
     SWE_TRY(dev_alloc(&B.ad_eslot, 6 * cap));
     B.ad_cap = cap;
     return SWE_OK;
@@ -710,6 +715,7 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
     a.bsum = B.ad_bsum;
     a.code = B.ad_code;
     a.kidx = B.ad_kidx;
+    a.list = B.ad_list;
     a.eslot = B.ad_eslot;
     a.status = B.ad_status;
     a.stamp = B.ad_status + 4;
@@ -816,7 +822,6 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
         B.ad_off2_cap = n_new + n_new / 4;
     }
     a.off_out = B.ad_off2;
-    a.list = reinterpret_cast<int32_t *>(B.ad_slots);  // the conformity words are spent
     a.n_changing = plan.n_changing;
     a.eta_out = fresh ? eo : B.eta[1 - B.cur];
     a.hu_out = fresh ? ho : B.hu[1 - B.cur];

@@ -105,7 +105,7 @@ struct AdaptArgs {
                                            //      (changing cells only)
     uint32_t *status;                      // [3], zero before the first call
     uint32_t *stamp;                       // [n / 256 + 1] per tile of the conformity sweeps
-    int32_t *list;                         // [n] the changing cells (may alias slots)
+    int32_t *list;                         // [n] the changing cells (written by the prefix pass)
     int64_t n_changing;                    // their number (from adapt_plan)
     int64_t *off_out;                      // [n'] tree offsets of the new mesh
     int8_t *depth_out, *plotter_out;       // [n'] new mesh
