@@ -366,16 +366,39 @@ __global__ __launch_bounds__(SCAN_T) void count_reduce_kernel(int64_t n, const i
     const int64_t tile = blockIdx.x * (int64_t)SCAN_TILE;
     int64_t sum = 0;
     bool moved = false, bad = false;
+    // the sweeps are over: the words are read with plain loads, and every
+    // cell's loads are issued before any is used (own word, depth and
+    // neighbours, then the neighbours' words)
+    uint32_t own[SCAN_ITEMS], nw[SCAN_ITEMS][3];
+    int32_t q[SCAN_ITEMS][3];
+    int dep[SCAN_ITEMS];
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const int64_t c = tile + j * SCAN_T + threadIdx.x;
+        const bool live = c < n;
+        own[j] = live ? slots[c] : 0u;
+        dep[j] = live ? depth[c] : 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[j][k] = live ? nbr[3 * c + k] : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) nw[j][k] = q[j][k] >= 0 ? slots[q[j][k] >> 2] : 0u;
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; ++j) {
         const int64_t c = tile + j * SCAN_T + threadIdx.x;
         if (c >= n) continue;
-        const uint32_t own = slots[c];
-        const uint32_t v0 = edge_value(slots, nbr, c, own, 0);
-        const uint32_t v1 = edge_value(slots, nbr, c, own, 1);
-        const uint32_t v2 = edge_value(slots, nbr, c, own, 2);
-        const int d = depth[c];
-        moved |= integrity_word(v0, v1, v2, d) != own;
+        uint32_t v[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t sk = (own[j] >> (2 * k)) & 3u;
+            const uint32_t tk = (nw[j][k] >> (2 * (q[j][k] & 3))) & 3u;
+            v[k] = q[j][k] < 0 ? sk : (((sk | tk) & 1u) | (sk & tk & 2u));
+        }
+        const uint32_t v0 = v[0], v1 = v[1], v2 = v[2];
+        const int d = dep[j];
+        moved |= integrity_word(v0, v1, v2, d) != own[j];
         int k = CODE_KEEP;
         if (v1 & 1u) {
             k = CODE_SPLIT + ((int)(v2 & 1u) | ((int)(v0 & 1u) << 1));
