@@ -253,6 +253,57 @@ def test_steps_leave_eta_jump_flags(seed, coarsen):
     op.close()
 
 
+def test_adaptive_loop_with_step_flags_matches_oracle():
+    """Fifteen time steps of the adaptive loop, remeshing before every step
+    from the elevation-jump flags the step before left: the oracle's step and
+    remeshing with the numpy restatement of the flags, against the device's
+    loop, mesh and state bit for bit after every remeshing and step."""
+    mesh, rng = random_case(21)
+    while mesh.n < 3000:
+        grow = rng.choice(np.array([0, 1], np.int8), size=mesh.n, p=[0.4, 0.6])
+        mesh = host.mesh_from_depths(po.adapt(mesh.depth, mesh.plotter, mesh.nbr, grow,
+                                              np.zeros((4, mesh.n)), dmin=1, dmax=14)[0])
+    st = np.stack(host.tsunami(mesh, seed=7, drytol=1e-3))
+    st[0] *= 50.0  # a source strong enough to move the mesh within a few steps
+    cen = mesh.centroids()
+    st[3] = np.where(cen[:, 0] > 0.85, 50.0, st[3])
+    st[0] = np.maximum(st[0], np.minimum(st[3], 0.0))
+    dmin, dmax, drytol = 2, 14, 1e-3
+    jumps = np.abs(st[0][:, None] - st[0][np.maximum(mesh.nbr, 0)])
+    rtol = float(np.quantile(jumps[jumps > 0], 0.9))
+    ctol = rtol / 20
+    op = swe.EulerTimestep(mesh, scaling=1e6, dry_tolerance=drytol, min_depth=dmin, max_depth=dmax)
+    op.upload(*st)
+    op.keep_eta_jump_flags(rtol, ctol)
+    cur, fwd, dt = mesh, False, 20.0
+    flags = None
+    changed = 0
+    for k in range(15):
+        print("case: step", k, "cells", cur.n, flush=True)
+        if k > 0:
+            d2, p2, s2, nb2, _, _ = po.adapt(cur.depth, cur.plotter, cur.nbr, flags, st, dmin=dmin,
+                                             dmax=dmax, coords=cur.coords)
+            n, _ = op.adapt()
+            assert n == len(d2), k
+            d, p, nb = op.download_mesh()
+            assert np.array_equal(d, d2) and np.array_equal(p, p2) and np.array_equal(nb, nb2), k
+            assert np.array_equal(np.stack(op.download()), s2), k
+            changed += n != cur.n or not np.array_equal(d2, cur.depth)
+            cur, st = host.mesh_from_depths(d2), s2
+            fwd = not fwd
+        flags = eta_jump_flags(cur, st, drytol, rtol, ctol, dmin, dmax)
+        eo, ho, vo, _, cand, _ = po.step(cur.depth, cur.plotter, cur.nbr, st, dt, scaling=1e6,
+                                         drytol=drytol, dmin=dmin, dmax=dmax, forward=fwd)
+        res = op.traverse(dt, fwd, flags=True)
+        assert np.array_equal(res.refine, flags), k
+        assert res.dt_cfl == cand, k
+        st = np.stack([eo, ho, vo, st[3]])
+        assert np.array_equal(np.stack(op.download()), st), k
+        fwd = not fwd
+    assert changed >= 5
+    op.close()
+
+
 def test_eta_jump_flags_drive_adapt():
     """swe_flag_eta_jumps against its definition restated in numpy, through
     the remeshing it drives (compared with the oracle's remeshing)."""
