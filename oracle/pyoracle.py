@@ -212,10 +212,15 @@ def integrate_bathymetry(f, x1, x2, x3, levels: int) -> float:
 
 def bilinear(values, x0, y0, dx, dy):
     """The bilinear raster sample of the library's remesher (clamped to the
-    last cell) as a function of one world point, for integrate_bathymetry."""
+    last cell, -5000 m outside the nodes' extent as get_bathymetry_at_position
+    gives outside its grid) as a function of one world point, for
+    integrate_bathymetry."""
     ny, nx = values.shape
 
     def f(X, Y):
+        if not (x0 <= X and y0 <= Y and X <= x0 + float(nx - 1) * dx and
+                Y <= y0 + float(ny - 1) * dy):
+            return -5000.0  # SWE_initialize.f90:164-171
         fx, fy = (X - x0) / dx, (Y - y0) / dy
         ix = min(max(int(fx), 0), nx - 2)
         iy = min(max(int(fy), 0), ny - 2)

@@ -9,7 +9,8 @@ scipy's reader.  A grid is node-registered: values at the points of its
 coordinate variables, the extent from the first to the last of them.
 
 Sampling is bilinear in the grid cell that holds the point, clamped to the
-last cell, with the arithmetic of the device's remesher
+last cell, and -5000 m outside the nodes' extent (the reference's assumption
+outside its ASAGI data), with the arithmetic of the device's remesher
 (samoa_amd/csrc/swe_adapt.hip raster_sample), so that b sampled here for an
 initial state and b the device samples for a cell it splits agree bit for
 bit.
@@ -85,12 +86,19 @@ def asagi_domain(bath: Raster, displ: Raster | None = None):
     return scaling, (float(off[0]), float(off[1]))
 
 
+OUTSIDE = -5000.0  # the sea floor the reference assumes outside its data
+
+
 def sample(r: Raster, x, y) -> np.ndarray:
     """Bilinear samples at world points (x, y) (arrays), clamped to the last
-    cell; the expression of the device's raster_sample, term for term."""
+    cell; the expression of the device's raster_sample, term for term.  A
+    point outside the nodes' extent gets OUTSIDE, as get_bathymetry_at_position
+    gives outside the ASAGI grid (SWE_initialize.f90:164-171)."""
     x = np.asarray(x, np.float64)
     y = np.asarray(y, np.float64)
     ny, nx = r.values.shape
+    inside = ((r.x0 <= x) & (r.y0 <= y) & (x <= r.x0 + float(nx - 1) * r.dx) &
+              (y <= r.y0 + float(ny - 1) * r.dy))
     fx = (x - r.x0) / r.dx
     fy = (y - r.y0) / r.dy
     ix = np.clip(np.trunc(fx).astype(np.int64), 0, nx - 2)
@@ -98,8 +106,9 @@ def sample(r: Raster, x, y) -> np.ndarray:
     wx = fx - ix.astype(np.float64)
     wy = fy - iy.astype(np.float64)
     v = r.values
-    return ((1.0 - wx) * (1.0 - wy) * v[iy, ix] + wx * (1.0 - wy) * v[iy, ix + 1] +
-            (1.0 - wx) * wy * v[iy + 1, ix] + wx * wy * v[iy + 1, ix + 1])
+    b = ((1.0 - wx) * (1.0 - wy) * v[iy, ix] + wx * (1.0 - wy) * v[iy, ix + 1] +
+         (1.0 - wx) * wy * v[iy + 1, ix] + wx * wy * v[iy + 1, ix + 1])
+    return np.where(inside, b, OUTSIDE)
 
 
 def data_depth(r: Raster, scaling: float) -> int:

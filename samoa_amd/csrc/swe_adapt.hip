@@ -524,8 +524,13 @@ __device__ void tree_corners(int64_t off, int d0, int d, double c[6]) {
 }
 
 // Bilinear sample of the raster at the world point (X, Y), clamped to its
-// last cell (the host's scenario code evaluates the same expression).
+// last cell (the host's scenario code evaluates the same expression); a point
+// outside the nodes' extent gets -5000 m, the sea floor the reference assumes
+// outside its data (SWE_initialize.f90:164-171).
 __device__ double raster_sample(const AdaptArgs &a, double X, double Y) {
+    if (!(a.rx0 <= X && a.ry0 <= Y && X <= a.rx0 + (double)(a.rnx - 1) * a.rdx &&
+          Y <= a.ry0 + (double)(a.rny - 1) * a.rdy))
+        return -5000.0;
     const double fx = (X - a.rx0) / a.rdx, fy = (Y - a.ry0) / a.rdy;
     const int64_t ix = min(max((int64_t)fx, (int64_t)0), a.rnx - 2);
     const int64_t iy = min(max((int64_t)fy, (int64_t)0), a.rny - 2);

@@ -159,6 +159,13 @@ contains
         real(c_double), intent(in) :: grid(nx, ny), x0, y0, dx, dy, p(2)
         real(c_double) :: fx, fy, wx, wy
         integer :: ix, iy
+        ! outside the nodes' extent: the sea floor the reference assumes
+        ! outside its data (SWE_initialize.f90:164-171)
+        if (.not. (x0 <= p(1) .and. y0 <= p(2) .and. p(1) <= x0 + real(nx - 1, c_double) * dx .and. &
+                   p(2) <= y0 + real(ny - 1, c_double) * dy)) then
+            v = -5000.0_c_double
+            return
+        end if
         fx = (p(1) - x0) / dx
         fy = (p(2) - y0) / dy
         ix = min(max(int(fx), 0), nx - 2)

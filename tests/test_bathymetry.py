@@ -66,13 +66,16 @@ def test_sample_reproduces_bilinear_functions():
     y = rng.uniform(2.0, 2.0 + 1.5, 1000)
     assert np.allclose(bt.sample(r, x, y), f(x, y), rtol=0, atol=1e-12)
     assert np.array_equal(bt.sample(r, X.ravel(), Y.ravel()), r.values.ravel())
-    # outside the grid: the edge cell's bilinear extended
-    assert np.allclose(bt.sample(r, np.array([-4.0]), np.array([2.5])), f(-4.0, 2.5), atol=1e-12)
+    # outside the nodes' extent: the reference's constant sea floor
+    # (SWE_initialize.f90:164-171); the extent's own edges are inside
+    out = bt.sample(r, np.array([-4.0, -3.0 + 6.0 + 1e-9, -2.0, 0.0]), np.array([2.5, 3.0, 1.9, 3.5]))
+    assert np.array_equal(out[:3], np.full(3, bt.OUTSIDE))
+    assert out[3] == f(0.0, 3.5)
 
 
 def test_cell_bathymetry_uses_centroids():
     mesh = host.mesh_uniform(4)
-    r = bt.Raster(np.add.outer(np.arange(5.0), 10.0 * np.arange(6.0)), 0.0, 0.0, 2.5, 2.0)
+    r = bt.Raster(np.add.outer(np.arange(6.0), 10.0 * np.arange(6.0)), 0.0, 0.0, 2.5, 2.0)
     b = bt.cell_bathymetry(r, mesh.coords, 10.0, (0.0, 0.0))
     c = 10.0 * mesh.coords.mean(axis=1)
     assert np.allclose(b, 10.0 * c[:, 0] / 2.5 + c[:, 1] / 2.0, atol=1e-12)

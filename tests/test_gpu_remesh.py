@@ -388,6 +388,9 @@ def test_split_cells_sample_a_netcdf_grid(d, dmax):
     got = op.download()[3]
     assert np.array_equal(got, bt.cell_bathymetry(r, fine.coords, S, off, depth=fine.depth,
                                                   dmax=dmax))
+    # the domain's square reaches past the grid in y: cells there take the
+    # reference's constant sea floor (SWE_initialize.f90:164-171)
+    assert (got == bt.OUTSIDE).any() and (got != bt.OUTSIDE).any()
     levels = po.integrate_levels(d + 1, dmax, bt.data_depth(r, S))
     assert levels == (3 if d == 9 else 9)
     f = po.bilinear(r.values, r.x0, r.y0, r.dx, r.dy)
