@@ -559,11 +559,12 @@ def main():
 
     ad = tsunami_adaptive(args, torch)
     achieved = ad["step_kernel_bytes_per_s"]
-    traffic = None
+    traffic, valu = None, None
     if os.path.exists(PROFILE_TRAFFIC):
         prof = json.load(open(PROFILE_TRAFFIC)).get("euler_step_tsunami_adaptive_d16_26")
         if prof:  # bytes per cell of the profiled launches, at this run's mean mesh size
             traffic = prof["hbm_bytes_per_cell"] * ad["cell_updates"] / args.steps
+            valu = prof.get("valu_issue_frac")
     out = {
         "metric": METRIC,
         "value": ad["value"],
@@ -589,6 +590,7 @@ def main():
             "bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
             "frac": achieved / HBM_PEAK, "traffic": traffic,
             "kernel": "swe::euler_step_kernel<0, 256, true> + swe::finalize_kernel (one step)",
+            "valu_issue_frac_profiled": valu,
             "bytes_per_cell_update": BYTES_PER_CELL,
             "timing": "HIP events around every step launch of the adaptive run",
         },
