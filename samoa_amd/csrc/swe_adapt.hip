@@ -14,11 +14,12 @@
 //   3. counts of new cells (split 2..4, merge 1 for the first child of a
 //      pair and 0 for the second, else 1), an exclusive scan of them.
 //   4. emit: depth, plotter type, tree offset and state of every new cell (a
-//      split copies eta, hu, hv and b, or samples b from a raster at the
-//      child's centroid as get_bathymetry_at_element does from its data; a
+//      split copies eta, hu, hv and b, or averages b over the child from a
+//      raster as get_bathymetry_at_element does under _ADAPT_INTEGRATE; a
 //      merge averages all four, SWE_adapt.f90:95-169); the neighbours inside
 //      each old cell, and between two cells that keep their place; for a
-//      changing cell, the new cell-edge on each half of each old edge.
+//      changing cell, the new cell-edge on each half of each old edge.  One
+//      kernel streams over the kept cells, another over the changing ones.
 //   5. link, over the changing cells only: a new edge on half s of old edge k
 //      of cell A meets the new edge on the same half of the same old edge in
 //      A's neighbour B; the same end of the shared edge is half s in B when A
@@ -591,7 +592,7 @@ __device__ __forceinline__ void put_slot(int32_t *eslot, int64_t c, int k, int h
 }
 
 // The new cell-edge on half s of old edge k of cell c: computed for a cell
-// that keeps its place, read from the slots emit_kernel wrote otherwise.
+// that keeps its place, read from the slots emit_change_kernel wrote otherwise.
 __device__ __forceinline__ int32_t edge_slot(const AdaptArgs &a, int64_t c, int k, int s) {
     const int32_t kc = a.kidx[c];
     if (kc >= 0) return (kc << 2) | k;
