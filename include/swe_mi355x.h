@@ -46,6 +46,14 @@ extern "C" {
 
 #define SWE_SOLVER_AUG_RIEMANN 0 /* riemannsolvers.f:2-258 (flux_solver=aug_riemann) */
 #define SWE_SOLVER_FWAVE 1       /* riemannsolvers.f:446-500 (flux_solver=fwave) */
+/* compute_lf_flux, SWE_euler_timestep.f90:359-446: global and local
+ * Lax-Friedrichs (flux_solver=lf, llf; a flat bed b = 0) and their
+ * well-balanced variants with bathymetry and wet/dry walls (lfbath,
+ * llfbath).  Forward steps only: the adjoint takes the GeoClaw solvers. */
+#define SWE_SOLVER_LF 2
+#define SWE_SOLVER_LLF 3
+#define SWE_SOLVER_LF_BATH 4
+#define SWE_SOLVER_LLF_BATH 5
 
 #define SWE_BC_OUTFLOW 0 /* ghost = copy of the cell (SWE_euler_timestep.f90:248) */
 #define SWE_BC_WALL 1    /* ghost momentum mirrored, p - 2 (p.n) n */

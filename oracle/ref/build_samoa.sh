@@ -21,6 +21,7 @@
 # to (10 m | 5 m), a wet shock strong enough to trigger refinement while the
 # mesh adapts every step (-nadapt 1):
 #   SWE_initialize.f90:354   hL = 1, hR = 0  ->  hL = 10, hR = 5
+# FLUX=... builds one of the reference's other flux solvers (see below).
 # Only the linked binary is written to $OUT (oracle/_ref/).
 set -euo pipefail
 : "${REF:?}" "${REFTMP:?}" "${OUT:?}"
@@ -41,7 +42,11 @@ elif [ "$VARIANT" != "plain" ]; then
     echo "build_samoa.sh: unknown VARIANT $VARIANT" >&2
     exit 1
 fi
-FL="-cpp -D_SWE -D_AUG_RIEMANN_FLUX -D_DOUBLE_PRECISION -D_ADAPT_INTEGRATE -D_DARCY_LAYERS=0 -D_DEBUG_LEVEL=1 -D_DARCY_MOB_QUADRATIC -D_PERM_MEAN_GEOMETRIC -fopenmp -O2 -ffp-contract=off -I$REFTMP/src -I$REFTMP/src/Samoa -module-dir $REFTMP/mod -I$REFTMP/mod"
+# FLUX selects the flux solver as SCons flux_solver= does (SConstruct:224-237):
+# AUG_RIEMANN (default), FWAVE, LF, LLF, LF_BATH or LLF_BATH.
+FLUX="${FLUX:-AUG_RIEMANN}"
+case "$FLUX" in AUG_RIEMANN|FWAVE|LF|LLF|LF_BATH|LLF_BATH) ;; *) echo "build_samoa.sh: unknown FLUX $FLUX" >&2; exit 1;; esac
+FL="-cpp -D_SWE -D_${FLUX}_FLUX -D_DOUBLE_PRECISION -D_ADAPT_INTEGRATE -D_DARCY_LAYERS=0 -D_DEBUG_LEVEL=1 -D_DARCY_MOB_QUADRATIC -D_PERM_MEAN_GEOMETRIC -fopenmp -O2 -ffp-contract=off -I$REFTMP/src -I$REFTMP/src/Samoa -module-dir $REFTMP/mod -I$REFTMP/mod"
 SRCS=$(python3 -c "
 import re
 t = open('$REF/src/SConscript').read()

@@ -380,9 +380,83 @@ void swo_geoclaw(int solver, double hL, double hR, double huL, double huR,
     *maxs = mx;
 }
 
+/* compute_lf_flux, SWE_euler_timestep.f90:359-446, in world coordinates:
+ * Q = (eta, hu, hv, b), n the left cell's outward normal, f = (h, hu, hv,
+ * max wave speed) per side.  Fortran's vector expressions are written out per
+ * component in the order the source gives them (left to right; a unary minus
+ * binds looser than a product, which does not change a value). */
+static void lf_flux(int solver, const double n[2], const double QL[4], const double QR[4],
+                    double drytol, double g, double fL[4], double fR[4]) {
+    const int llf = solver == SWO_SOLVER_LLF || solver == SWO_SOLVER_LLF_BATH;
+    double hL, hR, vL, vR, alpha;
+    if (solver == SWO_SOLVER_LF_BATH || solver == SWO_SOLVER_LLF_BATH) {
+        const int dryL = QL[0] - QL[3] < drytol, dryR = QR[0] - QR[3] < drytol;
+        if (dryL || dryR) { /* :367-392, a wall against the dry side */
+            for (int i = 0; i < 4; ++i) fL[i] = fR[i] = 0.0;
+            if (dryL && dryR) {
+            } else if (dryL) {
+                hR = fmax(QR[0] - QR[3], 0.0);
+                vR = n[0] * (QR[1] / (QR[0] - QR[3])) + n[1] * (QR[2] / (QR[0] - QR[3]));
+                fR[3] = sqrt(g * hR) + fabs(vR);
+                for (int i = 0; i < 2; ++i)
+                    fR[1 + i] = (-((0.5 * vR) * QR[1 + i]) - (((0.5 * g) * hR) * hR) * n[i]) +
+                                (0.5 * fR[3]) * QR[1 + i];
+            } else {
+                hL = fmax(QL[0] - QL[3], 0.0);
+                vL = n[0] * (QL[1] / (QL[0] - QL[3])) + n[1] * (QL[2] / (QL[0] - QL[3]));
+                fL[3] = sqrt(g * hL) + fabs(vL);
+                for (int i = 0; i < 2; ++i)
+                    fL[1 + i] = (((0.5 * vL) * QL[1 + i]) + (((0.5 * g) * hL) * hL) * n[i]) +
+                                (0.5 * fL[3]) * QL[1 + i];
+            }
+            return;
+        }
+        hL = fmax(QL[0] - QL[3], 0.0); /* :394-418 */
+        hR = fmax(QR[0] - QR[3], 0.0);
+        vL = n[0] * (QL[1] / (QL[0] - QL[3])) + n[1] * (QL[2] / (QL[0] - QL[3]));
+        vR = n[0] * (QR[1] / (QR[0] - QR[3])) + n[1] * (QR[2] / (QR[0] - QR[3]));
+        fL[3] = sqrt(g * hL) + fabs(vL);
+        fR[3] = sqrt(g * hR) + fabs(vR);
+        alpha = llf ? (fL[3] > fR[3] ? fL[3] : fR[3]) : 100.0;
+        fL[0] = 0.5 * ((hL * vL + hR * vR) + alpha * (QL[0] - QR[0]));
+        fR[0] = -fL[0];
+        for (int i = 0; i < 2; ++i) {
+            fL[1 + i] = 0.5 * (((QL[1 + i] * vL + QR[1 + i] * vR) +
+                                ((0.5 * g) * (hL * hL + hR * hR)) * n[i]) +
+                               alpha * (QL[1 + i] - QR[1 + i]));
+            fR[1 + i] = -fL[1 + i];
+            const double src = (((0.25 * g) * (hL + hR)) * (QR[3] - QL[3])) * n[i];
+            fL[1 + i] = fL[1 + i] + src;
+            fR[1 + i] = fR[1 + i] + src;
+        }
+        return;
+    }
+    /* :422-445, the constant bed b = 0 */
+    hL = fmax(QL[0] - 0.0, 0.0);
+    hR = fmax(QR[0] - 0.0, 0.0);
+    vL = n[0] * (QL[1] / (QL[0] - 0.0)) + n[1] * (QL[2] / (QL[0] - 0.0));
+    vR = n[0] * (QR[1] / (QR[0] - 0.0)) + n[1] * (QR[2] / (QR[0] - 0.0));
+    fL[3] = sqrt(g * (QL[0] - 0.0)) + fabs(vL);
+    fR[3] = sqrt(g * (QR[0] - 0.0)) + fabs(vR);
+    alpha = llf ? (fL[3] > fR[3] ? fL[3] : fR[3]) : 100.0;
+    fL[0] = 0.5 * ((vL * hL + vR * hR) + alpha * (QL[0] - QR[0]));
+    fR[0] = -fL[0];
+    for (int i = 0; i < 2; ++i) {
+        /* vL vL hL + vR vR hR is a scalar, added to both components */
+        fL[1 + i] = 0.5 * ((((vL * vL) * hL + (vR * vR) * hR) +
+                            ((0.5 * g) * (hL * hL + hR * hR)) * n[i]) +
+                           alpha * (QL[1 + i] - QR[1 + i]));
+        fR[1 + i] = -fL[1 + i];
+    }
+}
+
 /* SWE_euler_timestep.f90:449-481 */
 void swo_flux(int solver, const double n[2], const double QL[4], const double QR[4],
               double drytol, double g, double fL[4], double fR[4]) {
+    if (solver >= SWO_SOLVER_LF) {
+        lf_flux(solver, n, QL, QR, drytol, g, fL, fR);
+        return;
+    }
     /* transform_matrix(1,:) = n, (2,:) = (-n2, n1); p' = T p */
     double pL1 = n[0] * QL[1] + n[1] * QL[2];
     double pL2 = (-n[1]) * QL[1] + n[0] * QL[2];

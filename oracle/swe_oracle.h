@@ -29,6 +29,12 @@ extern "C" {
 #define SWO_BC_WALL 1
 #define SWO_SOLVER_AUG 0
 #define SWO_SOLVER_FWAVE 1
+/* compute_lf_flux (SWE_euler_timestep.f90:359-446) under _LF_FLUX, _LLF_FLUX,
+ * _LF_BATH_FLUX, _LLF_BATH_FLUX */
+#define SWO_SOLVER_LF 2
+#define SWO_SOLVER_LLF 3
+#define SWO_SOLVER_LF_BATH 4
+#define SWO_SOLVER_LLF_BATH 5
 
 /* l(k) = sqrt(0.5)**k as flang folds it, k = -1..62, stored at index k+1. */
 void swo_leg_sizes(double out[64]);

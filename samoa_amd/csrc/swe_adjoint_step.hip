@@ -273,6 +273,7 @@ static void launch_adj_t(const AdjArgs &a, int64_t grid_blocks, hipStream_t s) {
 hipError_t launch_adjoint(const AdjArgs &a, int solver, int chunk, int64_t grid_blocks,
                           hipStream_t s) {
     if (grid_blocks < 0 || grid_blocks % 8) return hipErrorInvalidValue;
+    if (solver != 0 && solver != 1) return hipErrorInvalidValue;  // the GeoClaw solvers only
     switch (chunk * 2 + (solver == 1 ? 1 : 0)) {
         case 128 * 2: launch_adj_t<0, 128>(a, grid_blocks, s); break;
         case 128 * 2 + 1: launch_adj_t<1, 128>(a, grid_blocks, s); break;

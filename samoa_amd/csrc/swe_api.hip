@@ -373,7 +373,7 @@ int batch_init(Batch &B, int32_t nscen, const swe_mesh_desc *descs) {
         if (!(d.scaling > 0.0) || !(d.g > 0.0) || !(d.dry_tolerance >= swe::MIN_DRY_TOLERANCE))
             return fail(SWE_ERR_ARG, "scenario %d: scaling and g must be > 0, dry_tolerance >= %g",
                         s, swe::MIN_DRY_TOLERANCE);
-        if (d.solver != SWE_SOLVER_AUG_RIEMANN && d.solver != SWE_SOLVER_FWAVE)
+        if (d.solver < SWE_SOLVER_AUG_RIEMANN || d.solver > SWE_SOLVER_LLF_BATH)
             return fail(SWE_ERR_ARG, "scenario %d: unknown solver %d", s, d.solver);
         if (d.boundary != SWE_BC_OUTFLOW && d.boundary != SWE_BC_WALL)
             return fail(SWE_ERR_ARG, "scenario %d: unknown boundary %d", s, d.boundary);
@@ -954,6 +954,9 @@ int batch_adjoint_device(Batch &B, const double *eta, const double *hu, const do
     a.inv16g = 1.0 / (16.0 * B.g);
     a.bc = B.bc;
     a.forward = forward ? 1 : 0;
+    if (B.solver != SWE_SOLVER_AUG_RIEMANN && B.solver != SWE_SOLVER_FWAVE)
+        return fail(SWE_ERR_ARG, "adjoint: the augmented and f-wave solvers only (solver %d)",
+                    B.solver);
     HIP_TRY(swe::launch_adjoint(a, B.solver, B.chunk, B.grid, st));
     return SWE_OK;
 }

@@ -339,6 +339,77 @@ SWE_HD void geoclaw(double hL, double hR, double huL, double huR,
     maxs = dmax2(dmax2(fabs(sw0), fabs(sw1)), fabs(sw2));
 }
 
+// Solver ids beyond GeoClaw's: compute_lf_flux (SWE_euler_timestep.f90:359-446)
+// under _LF_FLUX, _LLF_FLUX, _LF_BATH_FLUX and _LLF_BATH_FLUX.
+constexpr int SOLVER_LF = 2, SOLVER_LLF = 3, SOLVER_LF_BATH = 4, SOLVER_LLF_BATH = 5;
+
+// compute_lf_flux in world coordinates: Q = (eta, hu, hv, b), n the left
+// cell's outward normal; each side gets its own wave speed.  Fortran's vector
+// expressions are written out per component in the source's order (left to
+// right; a unary minus binds looser than a product, which changes no value).
+template <int SOLVER>
+SWE_HD void lf_flux(double nx, double ny, double etaL, double huL, double hvL, double bL,
+                    double etaR, double huR, double hvR, double bR, double drytol, double g,
+                    Flux &fL, Flux &fR) {
+    constexpr bool llf = SOLVER == SOLVER_LLF || SOLVER == SOLVER_LLF_BATH;
+    if (SOLVER == SOLVER_LF_BATH || SOLVER == SOLVER_LLF_BATH) {
+        const bool dryL = etaL - bL < drytol, dryR = etaR - bR < drytol;
+        if (dryL || dryR) {  // :367-392, a wall against the dry side
+            fL = Flux{0.0, 0.0, 0.0, 0.0};
+            fR = Flux{0.0, 0.0, 0.0, 0.0};
+            if (dryL && dryR) return;
+            if (dryL) {
+                const double hR = dmax2(etaR - bR, 0.0);
+                const double vR = nx * (huR / (etaR - bR)) + ny * (hvR / (etaR - bR));
+                fR.s = sqrt(g * hR) + fabs(vR);
+                const double pr = ((0.5 * g) * hR) * hR;
+                fR.hu = (-((0.5 * vR) * huR) - pr * nx) + (0.5 * fR.s) * huR;
+                fR.hv = (-((0.5 * vR) * hvR) - pr * ny) + (0.5 * fR.s) * hvR;
+            } else {
+                const double hL = dmax2(etaL - bL, 0.0);
+                const double vL = nx * (huL / (etaL - bL)) + ny * (hvL / (etaL - bL));
+                fL.s = sqrt(g * hL) + fabs(vL);
+                const double pr = ((0.5 * g) * hL) * hL;
+                fL.hu = ((0.5 * vL) * huL + pr * nx) + (0.5 * fL.s) * huL;
+                fL.hv = ((0.5 * vL) * hvL + pr * ny) + (0.5 * fL.s) * hvL;
+            }
+            return;
+        }
+        const double hL = dmax2(etaL - bL, 0.0), hR = dmax2(etaR - bR, 0.0);  // :394-418
+        const double vL = nx * (huL / (etaL - bL)) + ny * (hvL / (etaL - bL));
+        const double vR = nx * (huR / (etaR - bR)) + ny * (hvR / (etaR - bR));
+        fL.s = sqrt(g * hL) + fabs(vL);
+        fR.s = sqrt(g * hR) + fabs(vR);
+        const double alpha = llf ? dmax2(fL.s, fR.s) : 100.0;
+        fL.h = 0.5 * ((hL * vL + hR * vR) + alpha * (etaL - etaR));
+        fR.h = -fL.h;
+        const double pr = (0.5 * g) * (hL * hL + hR * hR);
+        const double u = 0.5 * (((huL * vL + huR * vR) + pr * nx) + alpha * (huL - huR));
+        const double v = 0.5 * (((hvL * vL + hvR * vR) + pr * ny) + alpha * (hvL - hvR));
+        const double src = ((0.25 * g) * (hL + hR)) * (bR - bL);  // well-balancing, :414-418
+        fL.hu = u + src * nx;
+        fL.hv = v + src * ny;
+        fR.hu = -u + src * nx;
+        fR.hv = -v + src * ny;
+    } else {  // :422-445, the constant bed b = 0
+        const double hL = dmax2(etaL - 0.0, 0.0), hR = dmax2(etaR - 0.0, 0.0);
+        const double vL = nx * (huL / (etaL - 0.0)) + ny * (hvL / (etaL - 0.0));
+        const double vR = nx * (huR / (etaR - 0.0)) + ny * (hvR / (etaR - 0.0));
+        fL.s = sqrt(g * (etaL - 0.0)) + fabs(vL);
+        fR.s = sqrt(g * (etaR - 0.0)) + fabs(vR);
+        const double alpha = llf ? dmax2(fL.s, fR.s) : 100.0;
+        fL.h = 0.5 * ((vL * hL + vR * hR) + alpha * (etaL - etaR));
+        fR.h = -fL.h;
+        // vL vL hL + vR vR hR is a scalar, added to both components
+        const double k = (vL * vL) * hL + (vR * vR) * hR;
+        const double pr = (0.5 * g) * (hL * hL + hR * hR);
+        fL.hu = 0.5 * ((k + pr * nx) + alpha * (huL - huR));
+        fL.hv = 0.5 * ((k + pr * ny) + alpha * (hvL - hvR));
+        fR.hu = -fL.hu;
+        fR.hv = -fL.hv;
+    }
+}
+
 // compute_geoclaw_flux, SWE_euler_timestep.f90:449-481.
 // QL/QR = (eta, hu, hv, b) in world coordinates, n the left cell's outward
 // normal.
@@ -347,6 +418,10 @@ SWE_HD void edge_flux(double nx, double ny, double etaL, double huL,
                                           double hvL, double bL, double etaR, double huR,
                                           double hvR, double bR, double drytol, double g,
                                           double inv16g, Flux &fL, Flux &fR) {
+    if constexpr (SOLVER >= SOLVER_LF) {
+        lf_flux<SOLVER>(nx, ny, etaL, huL, hvL, bL, etaR, huR, hvR, bR, drytol, g, fL, fR);
+        return;
+    }
     const double pL1 = nx * huL + ny * hvL;
     const double pL2 = (-ny) * huL + nx * hvL;
     const double pR1 = nx * huR + ny * hvR;

@@ -419,19 +419,29 @@ static void launch_t(const StepArgs &a, int64_t grid_blocks, hipStream_t s) {
         hipLaunchKernelGGL(finalize_kernel, dim3((unsigned)a.n_scen), dim3(FIN_THREADS), 0, s, a);
 }
 
-hipError_t launch_step(const StepArgs &a, int solver, int chunk, int64_t grid_blocks,
-                       hipStream_t s) {
-    if (grid_blocks < 0 || grid_blocks % 8) return hipErrorInvalidValue;
-    switch (chunk * 2 + (solver == 1 ? 1 : 0)) {
-        case 128 * 2: launch_t<0, 128>(a, grid_blocks, s); break;
-        case 128 * 2 + 1: launch_t<1, 128>(a, grid_blocks, s); break;
-        case 256 * 2: launch_t<0, 256>(a, grid_blocks, s); break;
-        case 256 * 2 + 1: launch_t<1, 256>(a, grid_blocks, s); break;
-        case 512 * 2: launch_t<0, 512>(a, grid_blocks, s); break;
-        case 512 * 2 + 1: launch_t<1, 512>(a, grid_blocks, s); break;
+template <int SOLVER>
+static hipError_t launch_chunk(const StepArgs &a, int chunk, int64_t grid_blocks, hipStream_t s) {
+    switch (chunk) {
+        case 128: launch_t<SOLVER, 128>(a, grid_blocks, s); break;
+        case 256: launch_t<SOLVER, 256>(a, grid_blocks, s); break;
+        case 512: launch_t<SOLVER, 512>(a, grid_blocks, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
+}
+
+hipError_t launch_step(const StepArgs &a, int solver, int chunk, int64_t grid_blocks,
+                       hipStream_t s) {
+    if (grid_blocks < 0 || grid_blocks % 8) return hipErrorInvalidValue;
+    switch (solver) {
+        case 0: return launch_chunk<0>(a, chunk, grid_blocks, s);
+        case 1: return launch_chunk<1>(a, chunk, grid_blocks, s);
+        case SOLVER_LF: return launch_chunk<SOLVER_LF>(a, chunk, grid_blocks, s);
+        case SOLVER_LLF: return launch_chunk<SOLVER_LLF>(a, chunk, grid_blocks, s);
+        case SOLVER_LF_BATH: return launch_chunk<SOLVER_LF_BATH>(a, chunk, grid_blocks, s);
+        case SOLVER_LLF_BATH: return launch_chunk<SOLVER_LLF_BATH>(a, chunk, grid_blocks, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace swe

@@ -11,6 +11,8 @@ module swe_mi355x
     integer(c_int), parameter :: SWE_OK = 0
     integer(c_int), parameter :: SWE_ERR_ARG = -1, SWE_ERR_HIP = -2, SWE_ERR_NOMEM = -3, SWE_ERR_STATE = -4
     integer(c_int32_t), parameter :: SWE_SOLVER_AUG_RIEMANN = 0, SWE_SOLVER_FWAVE = 1
+    integer(c_int32_t), parameter :: SWE_SOLVER_LF = 2, SWE_SOLVER_LLF = 3, SWE_SOLVER_LF_BATH = 4, &
+                                     SWE_SOLVER_LLF_BATH = 5
     integer(c_int32_t), parameter :: SWE_BC_OUTFLOW = 0, SWE_BC_WALL = 1
 
     type, bind(c) :: swe_mesh_desc

@@ -21,6 +21,11 @@ from . import _native
 
 SOLVER_AUG_RIEMANN = 0
 SOLVER_FWAVE = 1
+# compute_lf_flux variants (SWE_euler_timestep.f90:359-446); forward steps only
+SOLVER_LF = 2
+SOLVER_LLF = 3
+SOLVER_LF_BATH = 4
+SOLVER_LLF_BATH = 5
 BC_OUTFLOW = 0
 BC_WALL = 1
 G = 9.80665  # SWE_data_types.f90:34

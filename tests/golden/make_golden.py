@@ -154,7 +154,8 @@ def run_reference(binary: str, depth: int, steps: int, nout: int) -> list[dict]:
         shutil.rmtree(od, ignore_errors=True)
 
 
-def write_run(name: str, binary: str, depth: int, steps: int, nout: int, bc: int) -> None:
+def write_run(name: str, binary: str, depth: int, steps: int, nout: int, bc: int,
+              solver: int = 0) -> None:
     outs = run_reference(binary, depth, steps, nout)
     S, off = 10.0, -5.0  # SWE.f90:169-170 (non-ASAGI domain [-5,5]^2)
     n = 1 << (depth + 1)
@@ -180,8 +181,9 @@ def write_run(name: str, binary: str, depth: int, steps: int, nout: int, bc: int
         refine.append(rf)
         order_fwd.append(bool(np.array_equal(perm, np.arange(n))))
         assert order_fwd[-1] or np.array_equal(perm, np.arange(n)[::-1])
+    assert all(np.isfinite(st).all() for st in states), f"{name}: non-finite output"
     meta = dict(binary=binary, depth=depth, steps=steps, nout=nout, bc=bc, scaling=S,
-                offset=off, courant=F32_COURANT, drytol=F32_DRYTOL,
+                offset=off, courant=F32_COURANT, drytol=F32_DRYTOL, solver=solver,
                 cmd="-threads 1 -sections 1 -dstart d -dmin d -dmax d -nadapt -1 -tmax -1 "
                     f"-nmax {steps} -xmloutput .true. -nout {nout}")
     np.savez_compressed(os.path.join(HERE, f"run_{name}.npz"), states=np.asarray(states),
@@ -269,6 +271,24 @@ def write_adaptive_run(name: str, binary: str, dstart: int, dmin: int, dmax: int
     print(f"adapt_{name}: {len(outs)} outputs, n={n}, depths {np.bincount(outs[0]['depth'])}")
 
 
+# The reference built with its Lax-Friedrichs flux solvers (oracle/Makefile:
+# samoa_swe_*_lf, _llf, _lfbath, _llfbath): solver ids of the library.
+LF_RUNS = [
+    # the global variant's fixed alpha = 100 outruns the CFL step: the
+    # reference's own run turns non-finite after three steps
+    ("lf_dam_d7_n1", "samoa_swe_dam_lf", 7, 3, 1, 0, 2),
+    ("llf_dam_d8_n3", "samoa_swe_dam_llf", 8, 24, 3, 0, 3),
+    ("lfbath_bath_d7_n2", "samoa_swe_bath_lfbath", 7, 12, 2, 1, 4),
+    ("llfbath_bath_d8_n3", "samoa_swe_bath_llfbath", 8, 24, 3, 1, 5),
+    ("llfbath_plain_d7_n2", "samoa_swe_llfbath", 7, 16, 2, 0, 5),
+]
+
+
+def write_lf_runs() -> None:
+    for args in LF_RUNS:
+        write_run(*args)
+
+
 def main() -> None:
     if not os.path.exists(os.path.join(REF, "ref_kat")):
         sys.exit("build the reference first: make -C oracle ref")
@@ -277,6 +297,9 @@ def main() -> None:
         return
     if sys.argv[1:] == ["adapt"]:
         write_adaptive_fixtures()
+        return
+    if sys.argv[1:] == ["lf"]:
+        write_lf_runs()
         return
     if sys.argv[1:] == ["remesh"]:
         write_remesh_fixtures()

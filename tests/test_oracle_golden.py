@@ -59,6 +59,7 @@ def test_full_run_bit_exact(name):
     for k in range(1, meta["steps"] + 1):
         eo, ho, vo, rf, cand, _ = po.step(depth, plotter, nbr, st, dt, scaling=S,
                                           drytol=meta["drytol"], bc=meta["bc"],
+                                          solver=meta.get("solver", po.SOLVER_AUG),
                                           forward=sched[k - 1])
         t += dt
         st = np.stack([eo, ho, vo, st[3]])
