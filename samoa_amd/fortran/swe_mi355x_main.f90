@@ -5,7 +5,8 @@
 !   swe_mi355x -dmax 14 -nmax 100 [-dmin d] [-dstart d] [-nadapt k]
 !              [-tmax t] [-tout t] [-nout k]
 !              [-courant c] [-drytolerance d] [-scenario dambreak|radial|tsunami]
-!              [-seed s] [-boundary outflow|wall] [-solver aug_riemann|fwave]
+!              [-seed s] [-boundary outflow|wall]
+!              [-solver aug_riemann|fwave|lf|llf|lfbath|llfbath]
 !              [-dump prefix] [-hoststeps] [-xmloutput .true.] [-output_dir dir]
 !              [-stestpoints "x y,x y,..."] [-pointoutput_full .true.]
 !              [-dam_left hL] [-dam_right hR] [-refine_tol m]
@@ -145,7 +146,18 @@ program swe_mi355x_main
     desc%courant = courant
     desc%min_depth = dmin
     desc%max_depth = dmax
-    desc%solver = merge(SWE_SOLVER_FWAVE, SWE_SOLVER_AUG_RIEMANN, trim(solvername) == 'fwave')
+    ! the reference's flux_solver names (SConstruct:45-46, :224-237)
+    select case (trim(solvername))
+    case ('aug_riemann'); desc%solver = SWE_SOLVER_AUG_RIEMANN
+    case ('fwave'); desc%solver = SWE_SOLVER_FWAVE
+    case ('lf'); desc%solver = SWE_SOLVER_LF
+    case ('llf'); desc%solver = SWE_SOLVER_LLF
+    case ('lfbath'); desc%solver = SWE_SOLVER_LF_BATH
+    case ('llfbath'); desc%solver = SWE_SOLVER_LLF_BATH
+    case default
+        write(*, '(A, A)') 'unknown -solver ', trim(solvername)
+        stop 2
+    end select
     desc%boundary = merge(SWE_BC_WALL, SWE_BC_OUTFLOW, trim(bcname) == 'wall')
     rc = swe_create(desc, h)
     call check(rc, 'swe_create')

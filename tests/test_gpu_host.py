@@ -41,6 +41,28 @@ def test_cli_matches_reference_run(name, hoststeps):
             assert np.array_equal(arr, z["states"][i]), f"output {i}"
 
 
+@pytest.mark.parametrize("name,solver,dam", [("llfbath_plain_d7_n2", "llfbath", None),
+                                              ("llf_dam_d8_n3", "llf", (10, 5)),
+                                              ("lf_dam_d7_n1", "lf", (10, 5))])
+def test_cli_lax_friedrichs_runs_match_reference(name, solver, dam):
+    """-solver with the reference's flux_solver names: the reference's runs
+    of those builds, output for output."""
+    _, z, meta = {r[0]: r for r in golden_io.runs()}[name]
+    with tempfile.TemporaryDirectory() as td:
+        cmd = [host.CLI, "-scenario", "dambreak", "-dmax", str(meta["depth"]), "-nmax",
+               str(meta["steps"]), "-nout", str(meta["nout"]), "-tmax", "-1", "-courant", "0.45",
+               "-drytolerance", "0.01", "-solver", solver, "-dump", os.path.join(td, "s")]
+        if dam:
+            cmd += ["-dam_left", str(dam[0]), "-dam_right", str(dam[1])]
+        print("case:", " ".join(cmd[1:]), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        for i in range(len(z["times"])):
+            t, arr = read_dump(os.path.join(td, f"s_{i}.bin"))
+            assert t == z["times"][i], f"output {i}"
+            assert np.array_equal(arr, z["states"][i]), f"output {i}"
+
+
 @pytest.mark.parametrize("d", [3, 4])
 def test_cli_vtk_output_identical_to_reference_files(d):
     """The drop-in end to end: the CLI on the GPU writes the same VTK files,
