@@ -28,7 +28,10 @@ shape, 500 steps) under "ensemble"; the differentiable inversion (depth-20
 mesh, 200 steps, 32 gauges, sqrt(200) checkpoints) with its adjoint/forward
 cost ratio under "adjoint"; and 16 batched source inversions (20 lockstep
 L-BFGS iterations) with their recovery error under "inversion".  Inputs are
-resident in HBM when the timed region starts.  Prints ONE JSON line.
+resident in HBM when the timed region starts.  "frozen_uniform" also carries
+"host_buffers": ten of those steps through the host-buffer boundary
+(swe_upload, swe_step, swe_download each step), which PCIe transfers bound.
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -278,6 +281,25 @@ def tsunami_frozen(args, torch):
             traffic = prof[key]["hbm_bytes_per_launch"]
             valu = prof[key].get("valu_issue_frac")
 
+    # The same steps through the host-buffer boundary (swe_upload, swe_step,
+    # swe_download): every step takes the state from host memory and hands
+    # it back, so PCIe transfers bound it.  Reported beside the resident
+    # rate, never as the value.
+    r_host = 10
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(r_host):
+        op.upload(eta, hu, hv, b)
+        op.traverse(dt_next, forward=(fwd + args.steps + k) % 2 == 1)
+        eta, hu, hv, b = op.download()
+    s_host = time.perf_counter() - t0
+    host_buffers = {
+        "value": cells * r_host / s_host, "unit": "cell-updates/s", "steps": r_host,
+        "ms_per_step": 1e3 * s_host / r_host,
+        "per_step": "swe_upload of eta, hu, hv, b from pageable host memory, swe_step, "
+                    "swe_download of the four arrays",
+    }
+
     if args.calibrate_copy:
         a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         bb = torch.empty_like(a)
@@ -312,6 +334,7 @@ def tsunami_frozen(args, torch):
             "valu_issue_frac_profiled": valu,
         },
         "tuning": {"chunk": tuned["chunk"], "grid": tuned["grid"], "trials": tuned["trials"]},
+        "host_buffers": host_buffers,
     }
     out["roofline"]["achieved"] = achieved / 1e9
     out["roofline"]["peak"] = HBM_PEAK / 1e9
