@@ -4,6 +4,13 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, as the checker.  The product (samoa_amd/) never
 imports it.
 
+Parity pins of the functions written here in Python: integrate_levels and
+integrate_bathymetry are pinned bit for bit by the reference's adaptive run
+built with _ADAPT_INTEGRATE (tests/test_bathymetry.py).  bilinear and
+data_depth are parity unpinned: the reference samples its grids through
+ASAGI, a third-party library absent from it, so they are checked only
+against closed forms (bilinear functions, a worked example).
+
 Also holds the readers for the reference's VTU dumps (SWE_xml_output.f90:220-241,
 LIB_VTK_IO raw appended big-endian data) used by tests/golden/make_golden.py.
 """

@@ -6,10 +6,15 @@
  * expression rounds exactly as written, in the reference's order.
  *
  * Parity pins for this file (see tests/test_oracle_*.py):
- *   - swo_geoclaw / swo_riemanntype: known-answer vectors produced by the
- *     reference's own riemannsolvers.f + c_bind_riemannsolvers.f90 compiled
- *     with amdflang (oracle/_ref/ref_kat), plus invariants (dry-dry, identical
- *     states, lake at rest, mirror symmetry).
+ *   - swo_geoclaw / swo_riemanntype, augmented and f-wave solvers:
+ *     known-answer vectors produced by the reference's own riemannsolvers.f +
+ *     c_bind_riemannsolvers.f90 compiled with amdflang (oracle/_ref/ref_kat),
+ *     plus invariants (dry-dry, identical states, lake at rest, mirror
+ *     symmetry).
+ *   - lf_flux (global and local Lax-Friedrichs, flat-bed and well-balanced):
+ *     VTU dumps of the reference SWE binary built with each of _LF_FLUX,
+ *     _LLF_FLUX, _LF_BATH_FLUX and _LLF_BATH_FLUX (oracle/_ref/samoa_swe_*lf*),
+ *     on wet dam breaks, a partly dry sloped bed and the half-dry dam break.
  *   - swo_leg_sizes: table dumped by oracle/_ref/ref_tables (the reference's
  *     parameter expressions compiled by amdflang).
  *   - swo_mesh_uniform / swo_step / swo_initial_dt: VTU dumps of the full

@@ -138,3 +138,19 @@ def test_raster_integration_of_the_host_paths():
         assert np.array_equal(b[idx], want), d
         cb = bt.cell_bathymetry(rast, mesh.coords[idx], S, 0.0, depth=mesh.depth[idx], dmax=dmax)
         assert np.array_equal(cb, want), d
+
+
+def test_data_depth_closed_forms():
+    """nint(log2(S^2 / (dx dy))) (SWE_initialize.f90:129): a grid of 2^k x 2^k
+    samples over the square is depth 2k; rounding is to the nearest level,
+    both above and below the square's own scale."""
+    import pyoracle as po
+
+    for k in range(0, 13):
+        assert po.data_depth(1.0e6, 1.0e6 / 2 ** k, 1.0e6 / 2 ** k) == 2 * k
+    assert po.data_depth(1.0, 0.6, 0.6) == 1      # log2(2.78) = 1.47
+    assert po.data_depth(1.0, 0.55, 0.55) == 2    # log2(3.31) = 1.73
+    assert po.data_depth(1.0, 3.0, 3.0) == -3     # log2(1/9) = -3.17
+    assert po.data_depth(1.0, 4.0, 5.0) == -4     # log2(1/20) = -4.32
+    r = bt.Raster(np.zeros((3, 5)), 0.0, 0.0, 1.0e6 / 64, 1.0e6 / 64)
+    assert bt.data_depth(r, 1.0e6) == 12
