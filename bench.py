@@ -300,12 +300,22 @@ def tsunami_frozen(args, torch):
                     "swe_download of the four arrays",
     }
 
+    copy_rate = None
     if args.calibrate_copy:
+        # three 1 GiB copies for the counters, then twenty timed ones: the
+        # rate a plain device copy reaches (bytes read plus bytes written)
         a = torch.empty(1 << 27, dtype=torch.float64, device=dev)
         bb = torch.empty_like(a)
         for _ in range(3):
             bb.copy_(a)
+        c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        c0.record()
+        for _ in range(20):
+            bb.copy_(a)
+        c1.record()
         torch.cuda.synchronize()
+        copy_rate = 2 * a.numel() * 8 * 20 / (c0.elapsed_time(c1) * 1e-3)
+        del a, bb
 
     out = {
         "metric": METRIC,
@@ -336,6 +346,8 @@ def tsunami_frozen(args, torch):
         "tuning": {"chunk": tuned["chunk"], "grid": tuned["grid"], "trials": tuned["trials"]},
         "host_buffers": host_buffers,
     }
+    if copy_rate is not None:
+        out["copy_1gib_bytes_per_s"] = copy_rate
     out["roofline"]["achieved"] = achieved / 1e9
     out["roofline"]["peak"] = HBM_PEAK / 1e9
     op.close()
