@@ -61,7 +61,7 @@ def test_golden_run_bit_exact(name):
     op.close()
 
 
-@pytest.mark.parametrize("d", [1, 4, 9, 12, 15])
+@pytest.mark.parametrize("d", [0, 1, 4, 9, 12, 15])
 # the local Lax-Friedrichs flux with bathymetry takes wet/dry states too; the
 # other Lax-Friedrichs variants are pinned by the reference's own runs (their
 # fixed dissipation or flat bed does not suit these random states)
