@@ -91,6 +91,35 @@ def test_random_requests_match_oracle(seed):
     op.close()
 
 
+@pytest.mark.parametrize("pattern", ["all", "random"])
+def test_tiny_meshes_down_to_the_two_root_cells(pattern):
+    """Meshes smaller than one tile, from the two-cell mesh: four rounds that
+    refine (every cell, or a random half), then rounds that coarsen every
+    cell back to the two root cells, which cannot coarsen further.  Mesh and
+    state equal the oracle's after every round."""
+    rng = np.random.default_rng(11)
+    cur = host.mesh_uniform(0)
+    st = np.stack([5.0 + rng.random(2), rng.normal(size=2), rng.normal(size=2), -rng.random(2)])
+    op = swe.EulerTimestep(cur, scaling=1.0, dry_tolerance=1e-3, min_depth=0, max_depth=6)
+    op.upload(*st)
+    for r, f in enumerate([1] * 4 + [-1] * 7):
+        flags = np.full(cur.n, f, np.int8)
+        if pattern == "random" and f == 1:
+            flags = rng.choice(np.array([0, 1], np.int8), size=cur.n)
+        print("case:", pattern, "round", r, "cells", cur.n, flush=True)
+        d2, p2, s2, nb2, _, _ = po.adapt(cur.depth, cur.plotter, cur.nbr, flags, st, dmin=0,
+                                         dmax=6, coords=cur.coords)
+        n, _ = op.adapt(flags)
+        assert n == len(d2), r
+        d, p, nb = op.download_mesh()
+        assert np.array_equal(d, d2) and np.array_equal(p, p2) and np.array_equal(nb, nb2), r
+        assert np.array_equal(np.stack(op.download()), s2), r
+        cur = host.mesh_from_depths(d2)
+        st = s2
+    assert cur.n == 2
+    op.close()
+
+
 def test_short_sweep_batch_is_completed():
     """A call without requests changes no word, so the next call starts with
     a single sweep; requests that need more make the count pass find words
