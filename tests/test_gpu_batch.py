@@ -8,7 +8,7 @@ from samoa_amd import host, swe
 
 pytestmark = pytest.mark.gpu
 
-DEPTHS = [6, 9, 12, 9, 4, 13]
+DEPTHS = [6, 9, 12, 9, 4, 13, 0]  # 0: the two-cell mesh
 
 
 def build():
