@@ -74,7 +74,7 @@ def P(t):
 @pytest.mark.parametrize("forward", [True, False])
 @pytest.mark.parametrize("bc", [0, 1])
 @pytest.mark.parametrize("solver", [0, 1])
-@pytest.mark.parametrize("d", [3, 7, 10, 12])
+@pytest.mark.parametrize("d", [1, 3, 7, 10, 12])
 def test_adjoint_step_matches_autograd(d, solver, bc, forward):
     S = 100.0
     mesh = make_mesh(d)
