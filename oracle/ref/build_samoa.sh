@@ -21,6 +21,13 @@
 # to (10 m | 5 m), a wet shock strong enough to trigger refinement while the
 # mesh adapts every step (-nadapt 1):
 #   SWE_initialize.f90:354   hL = 1, hR = 0  ->  hL = 10, hR = 5
+# VARIANT=radial is the radial dam break in a closed basin: the unit square
+# (no offset), eta = 10 m inside the circle of radius 0.1 around (0.5, 0.5)
+# and 5 m outside, a flat bed and reflecting walls:
+#   SWE.f90:169-170          scaling = 10, offset = -5 (5, 5)  ->  scaling = 1, offset = 0
+#   SWE_initialize.f90:354   hL = 1, hR = 0  ->  hL = 10, hR = 5
+#   SWE_initialize.f90:362   xs(1) < 0  ->  norm2(xs - 0.5) < 0.1
+#   SWE_euler_timestep.f90:248 OUTFLOW copy -> mirror ghost p - 2 (p.n) n
 # FLUX=... builds one of the reference's other flux solvers (see below).
 # Only the linked binary is written to $OUT (oracle/_ref/).
 set -euo pipefail
@@ -38,6 +45,17 @@ if [ "$VARIANT" = "bath" ]; then
 elif [ "$VARIANT" = "dam" ]; then
     sed -i '354s/hL = 1.0_SR, hR = 0.0_SR/hL = 10.0_SR, hR = 5.0_SR/' "$REFTMP/src/SWE/SWE_initialize.f90"
     grep -q "hL = 10.0_SR, hR = 5.0_SR" "$REFTMP/src/SWE/SWE_initialize.f90"
+elif [ "$VARIANT" = "radial" ]; then
+    sed -i '169s/cfg%scaling = 10.0_GRID_SR$/cfg%scaling = 1.0_GRID_SR/' "$REFTMP/src/SWE/SWE.f90"
+    sed -i '170s/cfg%offset = cfg%scaling \* \[-0.5_GRID_SR, -0.5_GRID_SR\]$/cfg%offset = [0.0_GRID_SR, 0.0_GRID_SR]/' "$REFTMP/src/SWE/SWE.f90"
+    sed -i '354s/hL = 1.0_SR, hR = 0.0_SR/hL = 10.0_SR, hR = 5.0_SR/' "$REFTMP/src/SWE/SWE_initialize.f90"
+    sed -i '362s/if (xs(1) < 0.0_SR) then/if (norm2(xs - 0.5_SR) < 0.1_SR) then/' "$REFTMP/src/SWE/SWE_initialize.f90"
+    sed -i '248s/^\(\s*\)bnd_rep = rep%Q(1)$/\1bnd_rep = t_state(rep%Q(1)%h, rep%Q(1)%p - 2.0_GRID_SR * dot_product(rep%Q(1)%p, edge%transform_data%normal) * edge%transform_data%normal, rep%Q(1)%b)/' "$REFTMP/src/SWE/SWE_euler_timestep.f90"
+    grep -q "cfg%scaling = 1.0_GRID_SR" "$REFTMP/src/SWE/SWE.f90"
+    grep -q "cfg%offset = \[0.0_GRID_SR, 0.0_GRID_SR\]" "$REFTMP/src/SWE/SWE.f90"
+    grep -q "hL = 10.0_SR, hR = 5.0_SR" "$REFTMP/src/SWE/SWE_initialize.f90"
+    grep -q "norm2(xs - 0.5_SR) < 0.1_SR" "$REFTMP/src/SWE/SWE_initialize.f90"
+    grep -q "2.0_GRID_SR \* dot_product" "$REFTMP/src/SWE/SWE_euler_timestep.f90"
 elif [ "$VARIANT" != "plain" ]; then
     echo "build_samoa.sh: unknown VARIANT $VARIANT" >&2
     exit 1
@@ -46,7 +64,11 @@ fi
 # AUG_RIEMANN (default), FWAVE, LF, LLF, LF_BATH or LLF_BATH.
 FLUX="${FLUX:-AUG_RIEMANN}"
 case "$FLUX" in AUG_RIEMANN|FWAVE|LF|LLF|LF_BATH|LLF_BATH) ;; *) echo "build_samoa.sh: unknown FLUX $FLUX" >&2; exit 1;; esac
-FL="-cpp -D_SWE -D_${FLUX}_FLUX -D_DOUBLE_PRECISION -D_ADAPT_INTEGRATE -D_DARCY_LAYERS=0 -D_DEBUG_LEVEL=1 -D_DARCY_MOB_QUADRATIC -D_PERM_MEAN_GEOMETRIC -fopenmp -O2 -ffp-contract=off -I$REFTMP/src -I$REFTMP/src/Samoa -module-dir $REFTMP/mod -I$REFTMP/mod"
+# OPT: optimisation flags.  The parity builds keep every rounding as written
+# (-O2 -ffp-contract=off); the timing builds take SConstruct:306-310's
+# release flags (-O3, or -Ofast -march=native).
+OPT="${OPT:--O2 -ffp-contract=off}"
+FL="-cpp -D_SWE -D_${FLUX}_FLUX -D_DOUBLE_PRECISION -D_ADAPT_INTEGRATE -D_DARCY_LAYERS=0 -D_DEBUG_LEVEL=1 -D_DARCY_MOB_QUADRATIC -D_PERM_MEAN_GEOMETRIC -fopenmp $OPT -I$REFTMP/src -I$REFTMP/src/Samoa -module-dir $REFTMP/mod -I$REFTMP/mod"
 SRCS=$(python3 -c "
 import re
 t = open('$REF/src/SConscript').read()

@@ -4,9 +4,12 @@ Tries every compiled chunk size (cells per workgroup, i.e. the LDS tile) and a
 few grid sizes (one workgroup per chunk, or persistent grids of k workgroups
 per CU) on the operator's own meshes and state, timing graph-replayed steps
 of each with HIP events on the stream the kernels run on (the median of a
-few repetitions).  The state is saved before and restored after, so tuning
-does not advance the simulation.  Results are cached per device and mesh
-signature in ~/.cache/samoa_amd/autotune.json.
+few repetitions).  The state is saved before the trials and restored before
+every candidate, and every candidate's runs start the dt chain at the same
+dt0, so each launch shape is timed on the same waves; after the trials the
+state is restored once more, so tuning does not advance the simulation.
+Results are cached per device and mesh signature in
+~/.cache/samoa_amd/autotune.json.
 """
 from __future__ import annotations
 
@@ -90,6 +93,9 @@ def tune(batch, dt0, steps: int = 20, repeats: int = 3, use_cache: bool = True,
     timings = []
     try:
         for chunk, grid in candidates(props.multi_processor_count):
+            # every candidate starts from the saved state
+            for s, st in enumerate(saved):
+                batch.upload(s, *st)
             batch.set_tuning(chunk, grid)
             batch.run(2, dt0, forward=False)  # capture + warm
             reps = []

@@ -36,6 +36,7 @@
 #include <chrono>
 
 #include "swe_internal.hpp"
+#include "swe_sweep_stamp.hpp"
 
 namespace swe {
 namespace {
@@ -133,12 +134,12 @@ __device__ __forceinline__ uint32_t integrity_word(uint32_t v0, uint32_t v1, uin
 // state than any other tile may have written; it writes every word of its
 // tile.
 //
-// Sweep numbers carry the call's tag in their high 16 bits.  stamp[T] is the
-// last sweep in which a cell next to tile T but outside it changed (or T
-// itself did not settle).  From the second sweep on, a tile whose stamp is
-// older than the previous sweep (a stamp of an earlier call included) has
-// read every neighbour word since it last changed and is at its fixpoint: it
-// is skipped.  *changed gets the sweep if a word changed.
+// Sweep numbers carry the call's tag in their high 16 bits (sweep_number).
+// stamp[T] is the last sweep in which a cell next to tile T but outside it
+// changed (or T itself did not settle).  From the second sweep on, a tile
+// whose stamp is older than the previous sweep (a stamp of an earlier call
+// included, compared modulo 2^32: stamp_is_old) has read every neighbour word
+// since it last changed and is at its fixpoint: it is skipped.  *changed gets the sweep if a word changed.
 __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t *depth,
                                                         const int32_t *nbr, uint32_t *slots,
                                                         uint32_t *stamp, uint32_t *changed,
@@ -148,7 +149,7 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
     __shared__ uint32_t s_skip;
     const bool first = (sweep & 0xffffu) == 1u;
     if (!first) {
-        if (threadIdx.x == 0) s_skip = ld_relaxed(&stamp[blockIdx.x]) + 1u < sweep ? 1u : 0u;
+        if (threadIdx.x == 0) s_skip = stamp_is_old(ld_relaxed(&stamp[blockIdx.x]), sweep) ? 1u : 0u;
         __syncthreads();
         if (s_skip) return;  // uniform across the workgroup
     }
@@ -858,13 +859,13 @@ hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool o
     // read: the count pass also tells whether the words are at the fixpoint,
     // so no sweep is spent on confirming it.
     plan->sweeps = -1;
-    const uint32_t tag = (epoch & 0xffffu) << 16;
+    const uint32_t tag = sweep_number(epoch, 0);
     if (max_sweeps > 0xffff) max_sweeps = 0xffff;
     int batch = first_batch < 1 ? 1 : first_batch, done = 0;
     while (done < max_sweeps) {
         for (int j = 1; j <= batch; ++j)
             conf_sweep_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.nbr, a.slots, a.stamp,
-                                                a.status + 1, tag | (uint32_t)(done + j),
+                                                a.status + 1, sweep_number(epoch, (uint32_t)(done + j)),
                                                 a.flags, a.dmin, a.dmax);
         done += batch;
         if (tiles > 0)

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "swe_internal.hpp"
+#include "swe_sweep_stamp.hpp"
 #include "swe_mi355x.h"
 #include "swe_riemann.hpp"
 
@@ -122,7 +123,10 @@ int ensure_tables(int device) {
 
 // The captured pair of steps bakes in which buffer holds the state, the
 // starting direction and the launch shape; a pair returns the state to the
-// buffer it started from, so the graph stays valid across run calls.
+// buffer it started from, so the graph stays valid across run calls.  A
+// handle keeps one per (state buffer, starting direction): a run of an odd
+// number of steps leaves the next run in the other buffer or direction, which
+// would otherwise capture again.
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
     int cur = -1, forward = -1, chunk = 0, keep = -1, jumps = -1;
@@ -211,7 +215,7 @@ struct Batch {
     int8_t *plotter = nullptr, *depth = nullptr, *refine = nullptr;
     int cur = 0;
     std::vector<char> uploaded;
-    GraphCache graph;
+    GraphCache graph[2][2];  // [state buffer][starting direction]
     // remeshing (single mesh): the state and topology buffers hold `cap`
     // cells; a spare topology and bathymetry set receives a new mesh that
     // fits, the spare state buffer eta/hu/hv[1 - cur] its state
@@ -258,7 +262,9 @@ namespace {
 
 void release(Batch &B) {
     // teardown: errors cannot be reported from here, the buffers go either way
-    if (B.graph.exec) (void)hipGraphExecDestroy(B.graph.exec);
+    for (auto &row : B.graph)
+        for (GraphCache &gc : row)
+            if (gc.exec) (void)hipGraphExecDestroy(gc.exec);
     for (int i = 0; i < 2; ++i) {
         (void)hipFree(B.eta[i]);
         (void)hipFree(B.hu[i]);
@@ -340,10 +346,12 @@ int rebuild_chunks(Batch &B, bool upload = true) {
     if (upload)
         HIP_TRY(hipMemcpyAsync(B.d_scen, B.scen.data(), sizeof(swe::Scen) * B.scen.size(),
                                hipMemcpyHostToDevice, B.stream));
-    if (B.graph.exec) {
-        (void)hipGraphExecDestroy(B.graph.exec);
-        B.graph = GraphCache{};
-    }
+    for (auto &row : B.graph)
+        for (GraphCache &gc : row)
+            if (gc.exec) {
+                (void)hipGraphExecDestroy(gc.exec);
+                gc = GraphCache{};
+            }
     return SWE_OK;
 }
 
@@ -568,7 +576,7 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
     int fwd = forward ? 1 : 0;
     const int64_t pairs = n_steps / 2;
     if (pairs > 0) {
-        GraphCache &gc = B.graph;
+        GraphCache &gc = B.graph[B.cur][fwd];
         if (!gc.exec || gc.cur != B.cur || gc.forward != fwd || gc.chunk != B.chunk ||
             gc.grid != B.grid || gc.keep != B.keep_flags || gc.jumps != B.jump_flags ||
             gc.jump_refine != B.jump_refine || gc.jump_coarsen != B.jump_coarsen) {
@@ -736,9 +744,17 @@ int batch_adapt(Batch &B, const int8_t *refine_host, int64_t *n_out, int *sweeps
                                             std::log(2.0));
         a.raster_cap = (int)std::min<long>(sc.dmax + 3, data_depth + 3);
     }
+    const uint32_t epoch = ++B.ad_epoch;
+    if (swe::stamp_tag_wraps(epoch)) {
+        // the sweeps' 16-bit call tag starts again at 0: the stamps and the
+        // last-change word of the calls 65536 back would read as this call's
+        HIP_TRY(hipMemsetAsync(B.ad_status + 1, 0, sizeof(uint32_t), B.stream));
+        HIP_TRY(hipMemsetAsync(a.stamp, 0, sizeof(uint32_t) * (size_t)(B.ad_cap / 256 + 1),
+                               B.stream));
+    }
     swe::AdaptPlan plan{};
     HIP_TRY(swe::adapt_plan(a, B.stream, B.ad_last_sweeps, B.ad_off_valid, B.ad_last_changed,
-                            1 << 16, ++B.ad_epoch, B.ad_host, B.ad_host_dev, &B.ad_seq, &plan));
+                            1 << 16, epoch, B.ad_host, B.ad_host_dev, &B.ad_seq, &plan));
     B.ad_last_changed = (plan.status & 2u) != 0;
     const int sweeps = plan.sweeps;
     if (sweeps < 0) return fail(SWE_ERR_STATE, "adapt: the conformity sweeps did not converge");

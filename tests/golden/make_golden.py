@@ -10,6 +10,7 @@ test time.
     python tests/golden/make_golden.py vtu    (the VTK output files only)
     python tests/golden/make_golden.py adapt  (the adaptive-mesh runs only)
     python tests/golden/make_golden.py remesh (the runs that remesh every step)
+    python tests/golden/make_golden.py radial (the radial dam break in a closed basin)
 
 Fixtures:
   kat_geoclaw.npz      inputs (n,10) + reference outputs of c_bind_geoclaw_solver
@@ -22,6 +23,9 @@ Fixtures:
   vtu/                 the reference's own VTK output files of two small runs
   adapt_*.npz          runs on the reference's adaptively refined (then frozen)
                        meshes: cell corners, depths, states, times, flags
+  run_radial_*.npz     the radial dam break in a closed basin (unit square, walls)
+  radial_d14_n100.json the same at depth 14 over 100 steps: digests of the
+                       initial and final states (the 32,768-cell run itself)
   remesh_*.npz         runs that remesh before every step (-nadapt 1) with the
                        dam-break variant of the reference (10 m | 5 m): per
                        output the depth sequence of the mesh (which fixes it),
@@ -155,9 +159,11 @@ def run_reference(binary: str, depth: int, steps: int, nout: int) -> list[dict]:
 
 
 def write_run(name: str, binary: str, depth: int, steps: int, nout: int, bc: int,
-              solver: int = 0) -> None:
+              solver: int = 0, scaling: float = 10.0, offset: float = -5.0) -> None:
+    """scaling, offset: the domain of the binary's scenario (SWE.f90:169-170;
+    [-5,5]^2 unless a variant patches it)."""
     outs = run_reference(binary, depth, steps, nout)
-    S, off = 10.0, -5.0  # SWE.f90:169-170 (non-ASAGI domain [-5,5]^2)
+    S, off = scaling, offset
     n = 1 << (depth + 1)
     pts0 = outs[0]["Point"].reshape(n, 3, 3)[:, :, :2]
     # map reference output order -> forward SFC index via the oracle mesh
@@ -304,6 +310,9 @@ def main() -> None:
     if sys.argv[1:] == ["remesh"]:
         write_remesh_fixtures()
         return
+    if sys.argv[1:] == ["radial"]:
+        write_radial_fixtures()
+        return
     kin = kat_inputs()
     aug = run_kat("geoclaw", kin, "3", nout=7)
     fwave = run_kat("geoclaw", kin, "1", nout=7)
@@ -332,6 +341,50 @@ def main() -> None:
     write_run("bath_d9_n7", "samoa_swe_bath", 9, 35, 7, 1)
     write_vtu_fixtures()
     write_adaptive_fixtures()
+    write_radial_fixtures()
+
+
+def write_radial_fixtures() -> None:
+    """The radial dam break in a closed basin (the reference's VARIANT=radial
+    build: unit square, eta 10 m inside r < 0.1 around the centre, 5 m
+    outside, flat bed, reflecting walls), 100 steps at depths 8 and 10, and
+    the same run at depth 14 (32,768 cells) whose final state is stored as
+    SHA-256 digests of its bytes beside a few sums."""
+    write_run("radial_d8_n10", "samoa_swe_radial", 8, 100, 10, 1, scaling=1.0, offset=0.0)
+    write_run("radial_d10_n50", "samoa_swe_radial", 10, 100, 50, 1, scaling=1.0, offset=0.0)
+    import hashlib
+
+    d, steps = 14, 100
+    outs = run_reference("samoa_swe_radial", d, steps, steps)
+    n = 1 << (d + 1)
+    _, co, _ = po.mesh_uniform(d)
+    index = {tuple(np.round(c, 9)): i for i, c in enumerate(co.mean(axis=1))}
+    res = {}
+    for tag, v in (("initial", outs[0]), ("final", outs[-1])):
+        pts = v["Point"].reshape(n, 3, 3)[:, :, :2]
+        perm = np.array([index[tuple(np.round(c, 9))] for c in pts.mean(axis=1)])
+        st = np.zeros((4, n))
+        st[0, perm] = v["water height"]
+        m = v["momentum"].reshape(n, 3)
+        st[1, perm] = m[:, 0]
+        st[2, perm] = m[:, 1]
+        st[3, perm] = v["bathymetry"]
+        rf = np.zeros(n, np.int8)
+        rf[perm] = v["refinement flag"]
+        res[tag] = dict(
+            time=float(v["time"][0]),
+            sha256={k: hashlib.sha256(np.ascontiguousarray(a, "<f8").tobytes()).hexdigest()
+                    for k, a in zip(("eta", "hu", "hv", "b"), st)},
+            refine_sha256=hashlib.sha256(rf.tobytes()).hexdigest(),
+            sums={k: float(a.sum()) for k, a in zip(("eta", "hu", "hv"), st[:3])},
+            forward=bool(np.array_equal(perm, np.arange(n))))
+    meta = dict(binary="samoa_swe_radial", depth=d, steps=steps, nout=steps, bc=1, scaling=1.0,
+                offset=0.0, courant=F32_COURANT, drytol=F32_DRYTOL, solver=0,
+                cmd="-threads 1 -sections 1 -dstart 14 -dmin 14 -dmax 14 -nadapt -1 -tmax -1 "
+                    f"-nmax {steps} -xmloutput .true. -nout {steps}")
+    with open(os.path.join(HERE, "radial_d14_n100.json"), "w") as fh:
+        json.dump(dict(meta=meta, **res), fh, indent=1, sort_keys=True)
+    print(f"radial_d14_n100: final time {res['final']['time']!r}")
 
 
 def write_remesh_run(name: str, binary: str, dstart: int, dmin: int, dmax: int, steps: int,

@@ -36,14 +36,34 @@ def test_mesh_large_is_consistent():
 @pytest.mark.parametrize("name", [r[0] for r in golden_io.runs()])
 def test_dam_break_and_initial_dt_match_reference(name):
     _, z, meta = {r[0]: r for r in golden_io.runs()}[name]
-    # the plain dam break: the reference unpatched, with any flux solver
-    if meta["binary"] not in ("samoa_swe", "samoa_swe_llfbath"):
-        pytest.skip("scenario-patched reference variant")
+    # the plain dam break: the reference unpatched, with any flux solver; the
+    # radial dam break: its VARIANT=radial build
     m = host.mesh_uniform(meta["depth"])
-    st = host.reference_dam_break(m)
+    if meta["binary"] == "samoa_swe_radial":
+        st = host.radial_dam_break(m)
+    elif meta["binary"] in ("samoa_swe", "samoa_swe_llfbath"):
+        st = host.reference_dam_break(m)
+    else:
+        pytest.skip("scenario-patched reference variant")
     assert np.array_equal(np.stack(st), z["states"][0])
     dt0 = host.initial_dt(m, st, meta["courant"])
     assert dt0 == z["times"][1] if meta["nout"] == 1 else dt0 == po.initial_dt(m.depth, st, meta["courant"])
+
+
+def test_radial_dam_break_closed_form():
+    """eta = 10 m where the centroid lies within 0.1 of (0.5, 0.5), else 5 m;
+    flat bed, still water (cells whose centroid distance is within 1e-12 of
+    the radius are left to the rounding of norm2)."""
+    for d in (8, 14):
+        m = host.mesh_uniform(d)
+        eta, hu, hv, b = host.radial_dam_break(m)
+        r = np.hypot(*(m.centroids() - 0.5).T)
+        assert (eta[r < 0.1 - 1e-12] == 10.0).all() and (eta[r > 0.1 + 1e-12] == 5.0).all()
+        assert set(np.unique(eta)) == {5.0, 10.0}
+        assert not hu.any() and not hv.any() and not b.any()
+        # the disc's area, to the resolution of the cells
+        inside = (eta == 10.0).sum() * 0.5 ** (d + 1)
+        assert abs(inside - np.pi * 0.01) < 3.0 * 2.0 ** (-(d // 2)) * 0.1
 
 
 def test_initial_dt_matches_oracle_on_wet_dry_state():

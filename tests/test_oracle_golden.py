@@ -2,6 +2,8 @@
 
 Fixtures in tests/golden/ were written by tests/golden/make_golden.py from the
 reference compiled with amdflang (oracle/_ref/)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -145,3 +147,36 @@ def test_reversed_plotter_normals_swap_legs():
         a, b = po.normals(p), po.normals(-p)
         assert np.array_equal(a[0], b[2]) and np.array_equal(a[2], b[0])
         assert np.array_equal(a[1], b[1])
+
+
+def test_oracle_radial_d14_matches_reference_digests():
+    """The oracle over the reference's own depth-14 radial dam-break run
+    (32,768 cells, reflecting walls, 100 steps): the final state's bytes and
+    time equal the reference's."""
+    import hashlib
+    import json
+
+    fx = json.load(open(os.path.join(golden_io.GOLDEN, "radial_d14_n100.json")))
+    meta = fx["meta"]
+    d = meta["depth"]
+    plotter, co, nbr = po.mesh_uniform(d)
+    depth = np.full(plotter.shape[0], d, np.int8)
+    r = np.hypot(*(co.mean(axis=1) - 0.5).T)
+    st = np.zeros((4, depth.shape[0]))
+    st[0] = np.where(r < 0.1, 10.0, 5.0)
+    dig = lambda a: hashlib.sha256(np.ascontiguousarray(a, "<f8").tobytes()).hexdigest()  # noqa: E731
+    assert dig(st[0]) == fx["initial"]["sha256"]["eta"]
+    dt = po.initial_dt(depth, st, meta["courant"])
+    t = 0.0
+    sched = golden_io.direction_schedule(meta["steps"], meta["nout"])
+    for k in range(meta["steps"]):
+        e, u, v, rf, cand, _ = po.step(depth, plotter, nbr, st, dt, scaling=meta["scaling"],
+                                       drytol=meta["drytol"], bc=meta["bc"], forward=sched[k],
+                                       threads=True)
+        st = np.stack([e, u, v, st[3]])
+        t += dt
+        dt = meta["courant"] * cand
+    assert t == fx["final"]["time"]
+    for name, a in zip(("eta", "hu", "hv", "b"), st):
+        assert dig(a) == fx["final"]["sha256"][name], name
+    assert hashlib.sha256(rf.tobytes()).hexdigest() == fx["final"]["refine_sha256"]
