@@ -130,6 +130,46 @@ int swe_step_device(swe_handle *h, const double *eta, const double *hu, const do
                     double dt, int forward, double *dt_cfl_dev, int64_t *n_refine_dev,
                     int8_t *refine_dev, void *stream);
 
+/* ---- The step without device allocations -----------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): swe_step_device without a handle.  The caller passes the
+ * mesh, the state and the outputs as device addresses; the step's scratch
+ * (the CFL minima and refinement counts of its 256-cell chunks and the
+ * finalize kernel's result record) lives in a fixed device array of the
+ * library, 768.25 KiB per device, held until the process ends.  Limit:
+ * n_cells <= 16,777,216 (65,536 chunks); larger meshes return SWE_ERR_ARG.
+ * The call allocates nothing (no hipMalloc, no host allocation) and is
+ * asynchronous on `stream` (a hipStream_t; NULL = the null stream).  The
+ * first call on a device also looks up the scratch's address and uploads the
+ * geometry tables there (synchronous copies): make it outside graph capture.
+ * Every call on a device shares that device's scratch: calls that overlap on
+ * one device (other streams, other threads) race; calls in order on one
+ * stream do not.  Nothing in the scratch must start at zero.  The mesh is not
+ * validated (check it once with swe_create): nbr holds swe_mesh_desc's plain
+ * neighbour cells, and an entry outside [0, n_cells) reads as a domain
+ * boundary; depth and plotter must be in their documented ranges.  The
+ * chunk size is the library's default (swe_chunk_cells), one workgroup per
+ * chunk; results equal swe_step_device's, which does not depend on its
+ * launch shape. */
+typedef struct swe_device_mesh {
+    int64_t n_cells;
+    const int8_t *depth;    /* device [n_cells] */
+    const int8_t *plotter;  /* device [n_cells] */
+    const int32_t *nbr;     /* device [3*n_cells], neighbour cell or -1 */
+    double scaling;         /* as swe_mesh_desc */
+    double g;
+    double dry_tolerance;
+    int32_t min_depth, max_depth; /* for the refinement flags */
+    int32_t solver;         /* SWE_SOLVER_* */
+    int32_t boundary;       /* SWE_BC_* */
+} swe_device_mesh;
+
+int swe_step_device_noalloc(const swe_device_mesh *mesh, const double *eta, const double *hu,
+                            const double *hv, const double *b, double *eta_out, double *hu_out,
+                            double *hv_out, double dt, int forward, double *dt_cfl_dev,
+                            int64_t *n_refine_dev, int8_t *refine_dev, void *stream);
+
 /* ---- Discrete adjoint ------------------------------------------------------
  * No counterpart in the reference (the reference has no adjoint).  For the
  * traversal Q^{n+1} = F(Q^n, b; dt_n) with dt_n and the wet/dry limiter

@@ -43,6 +43,24 @@ class MeshDesc(C.Structure):
     ]
 
 
+class DeviceMesh(C.Structure):
+    """swe_device_mesh: a mesh in device memory (swe_step_device_noalloc)."""
+
+    _fields_ = [
+        ("n_cells", C.c_int64),
+        ("depth", C.c_void_p),
+        ("plotter", C.c_void_p),
+        ("nbr", C.c_void_p),
+        ("scaling", C.c_double),
+        ("g", C.c_double),
+        ("dry_tolerance", C.c_double),
+        ("min_depth", C.c_int32),
+        ("max_depth", C.c_int32),
+        ("solver", C.c_int32),
+        ("boundary", C.c_int32),
+    ]
+
+
 # entry point -> (restype, argtypes); every one declared in include/swe_mi355x.h
 _P = C.c_void_p
 _D = C.c_double
@@ -55,6 +73,8 @@ SIGNATURES = {
     "swe_step": (_I, [_P, _D, _I, C.POINTER(_D), C.POINTER(_I64), _P]),
     "swe_run": (_I, [_P, _I64, _D, _I, _D, C.POINTER(_D), C.POINTER(_D)]),
     "swe_step_device": (_I, [_P, _P, _P, _P, _P, _P, _P, _P, _D, _I, _P, _P, _P, _P]),
+    "swe_step_device_noalloc": (_I, [C.POINTER(DeviceMesh), _P, _P, _P, _P, _P, _P, _P, _D, _I,
+                                     _P, _P, _P, _P]),
     "swe_set_stream": (_I, [_P, _P]),
     "swe_adapt": (_I, [_P, _P, C.POINTER(_I64), C.POINTER(_I)]),
     "swe_keep_refinement_flags": (_I, [_P, _I]),

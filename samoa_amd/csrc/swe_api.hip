@@ -105,19 +105,23 @@ void leg_table(double out[64]) {
     }
 }
 
+// Devices the library serves (ordinals 0 .. MAX_DEVICES - 1); per-device
+// set-up is recorded in fixed tables, so that it allocates nothing.
+constexpr int MAX_DEVICES = 64;
 std::mutex g_table_mutex;
-std::vector<int> g_tables_on_device;
+bool g_tables_on_device[MAX_DEVICES] = {};
 
 int ensure_tables(int device) {
+    if (device < 0 || device >= MAX_DEVICES)
+        return fail(SWE_ERR_ARG, "device %d beyond the %d the library serves", device, MAX_DEVICES);
     std::lock_guard<std::mutex> lock(g_table_mutex);
-    for (int d : g_tables_on_device)
-        if (d == device) return SWE_OK;
+    if (g_tables_on_device[device]) return SWE_OK;
     double normal[17 * 3 * 2], leg[64];
     normal_table(normal);
     leg_table(leg);
     HIP_TRY(swe::upload_tables(normal, leg));
     HIP_TRY(swe::upload_tables_adjoint(normal, leg));
-    g_tables_on_device.push_back(device);
+    g_tables_on_device[device] = true;
     return SWE_OK;
 }
 
@@ -1247,6 +1251,124 @@ int swe_adjoint_step_device(swe_handle *h, const double *eta, const double *hu, 
     if (!h) return fail(SWE_ERR_ARG, "swe_adjoint_step_device: null handle");
     return batch_adjoint_device(h->B, eta, hu, hv, b, masks, leta, lhu, lhv, leta_out, lhu_out,
                                 lhv_out, gb, dt_dev, forward, stream);
+}
+
+}  // extern "C"
+
+// ---- the step without device allocations -----------------------------------
+//
+// swe_step_device_noalloc keeps the step's scratch (the dt/result slots of the
+// finalize kernel and the per-chunk CFL minima and refinement counts) in one
+// file-scope device array, `scratch`, of a fixed size; every call on a device
+// shares that device's copy.  Host code never passes the array's name to a
+// kernel (on the host it names a stand-in, not device memory): the device
+// address comes from hipGetSymbolAddress, once per device, and is kept in
+// g_scratch.
+namespace {
+constexpr int64_t NOALLOC_MAX_CHUNKS = 65536;
+constexpr int64_t NOALLOC_MAX_CELLS = NOALLOC_MAX_CHUNKS * swe::DEFAULT_CHUNK;  // 16,777,216
+constexpr size_t NOALLOC_SLOT_BYTES = 256;  // one Slots record, padded
+constexpr size_t NOALLOC_BYTES =
+    NOALLOC_SLOT_BYTES + (size_t)NOALLOC_MAX_CHUNKS * (sizeof(double) + sizeof(int32_t));
+static_assert(sizeof(swe::Slots) <= NOALLOC_SLOT_BYTES, "slots record");
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+namespace {
+std::mutex g_scratch_mutex;
+unsigned char *g_scratch[MAX_DEVICES] = {};  // device address of `scratch` per device
+
+// The device address of `scratch` on the current device (looked up on the
+// first call there).
+int device_scratch(int device, unsigned char **out) {
+    if (device < 0 || device >= MAX_DEVICES)
+        return fail(SWE_ERR_ARG, "device %d beyond the %d the library serves", device, MAX_DEVICES);
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[device]) {
+        void *p = nullptr;
+        HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(SWE_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch");
+        g_scratch[device] = (unsigned char *)p;
+    }
+    *out = g_scratch[device];
+    return SWE_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int swe_step_device_noalloc(const swe_device_mesh *m, const double *eta, const double *hu,
+                            const double *hv, const double *b, double *eta_out, double *hu_out,
+                            double *hv_out, double dt, int forward, double *dt_cfl_dev,
+                            int64_t *n_refine_dev, int8_t *refine_dev, void *stream) {
+    if (!m || !eta || !hu || !hv || !b || !eta_out || !hu_out || !hv_out || !dt_cfl_dev)
+        return fail(SWE_ERR_ARG, "swe_step_device_noalloc: null argument");
+    if (!m->depth || !m->plotter || !m->nbr)
+        return fail(SWE_ERR_ARG, "swe_step_device_noalloc: null mesh array");
+    if (eta_out == eta || hu_out == hu || hv_out == hv)
+        return fail(SWE_ERR_ARG, "swe_step_device_noalloc: outputs must not alias inputs");
+    const int64_t n = m->n_cells;
+    if (n <= 0 || n > NOALLOC_MAX_CELLS)
+        return fail(SWE_ERR_ARG,
+                    "swe_step_device_noalloc: n_cells %lld out of range (1..%lld, the size of "
+                    "its fixed scratch)", (long long)n, (long long)NOALLOC_MAX_CELLS);
+    if (!(m->scaling > 0.0) || !(m->g > 0.0) || !(m->dry_tolerance >= swe::MIN_DRY_TOLERANCE))
+        return fail(SWE_ERR_ARG,
+                    "swe_step_device_noalloc: scaling and g must be > 0, dry_tolerance >= %g",
+                    swe::MIN_DRY_TOLERANCE);
+    if (m->solver < SWE_SOLVER_AUG_RIEMANN || m->solver > SWE_SOLVER_LLF_BATH)
+        return fail(SWE_ERR_ARG, "swe_step_device_noalloc: unknown solver %d", m->solver);
+    if (m->boundary != SWE_BC_OUTFLOW && m->boundary != SWE_BC_WALL)
+        return fail(SWE_ERR_ARG, "swe_step_device_noalloc: unknown boundary %d", m->boundary);
+    if (m->max_depth < 0 || m->max_depth > 60 || m->min_depth > m->max_depth)
+        return fail(SWE_ERR_ARG, "swe_step_device_noalloc: depths %d..%d out of range",
+                    m->min_depth, m->max_depth);
+    int device = 0;
+    HIP_TRY(hipGetDevice(&device));
+    SWE_TRY(ensure_tables(device));
+    unsigned char *scr = nullptr;
+    SWE_TRY(device_scratch(device, &scr));
+
+    double leg[64];
+    leg_table(leg);
+    swe::StepArgs a{};
+    a.eta = eta;
+    a.hu = hu;
+    a.hv = hv;
+    a.b = b;
+    a.eta_out = eta_out;
+    a.hu_out = hu_out;
+    a.hv_out = hv_out;
+    a.nbr = m->nbr;
+    a.plotter = m->plotter;
+    a.depth = m->depth;
+    a.refine = refine_dev;
+    a.slots = (swe::Slots *)scr;
+    a.part_min = (double *)(scr + NOALLOC_SLOT_BYTES);
+    a.part_nref = (int32_t *)(scr + NOALLOC_SLOT_BYTES + sizeof(double) * NOALLOC_MAX_CHUNKS);
+    a.out_dtcfl = dt_cfl_dev;
+    a.out_nref = n_refine_dev;
+    a.n_scen = 1;
+    a.nchunks = (n + swe::DEFAULT_CHUNK - 1) / swe::DEFAULT_CHUNK;
+    a.dt = dt;
+    a.g = m->g;
+    a.inv16g = 1.0 / (16.0 * m->g);
+    a.bc = m->boundary;
+    a.forward = forward ? 1 : 0;
+    a.raw_nbr = 1;
+    swe::Scen &sc = a.scen0;
+    sc.n = n;
+    sc.nchunks = a.nchunks;
+    sc.scaling = m->scaling;
+    sc.drytol = m->dry_tolerance;
+    sc.refine_thr = (5.0 * m->scaling) * leg[m->max_depth + 1];  // volume_op :339
+    sc.dt_clamp = -1.0;
+    sc.depth = -1;
+    sc.dmin = m->min_depth;
+    sc.dmax = m->max_depth;
+    HIP_TRY(swe::launch_step_raw(a, m->solver, (hipStream_t)stream));
+    return SWE_OK;
 }
 
 // ---- batch of meshes -------------------------------------------------------

@@ -45,7 +45,7 @@ struct StepArgs {
     int8_t *refine;                        // state-indexed [n] or null
     int8_t *masks;                         // state-indexed [n] limiter masks (bit 0 wetting,
                                            // bit 1 drying fix taken) or null
-    const Scen *scen;                      // [n_scen]
+    const Scen *scen;                      // [n_scen], or null: one scenario, scen0
     const int32_t *chunk_scen;             // [nchunks] scenario of each chunk, or null (one scenario)
     Slots *slots;                          // [n_scen]
     const double *dt_host;                 // [n_scen] dt per scenario when device_dt == 0, or null
@@ -65,6 +65,12 @@ struct StepArgs {
     // jump_coarsen) instead of volume_op's |dQ_h| criterion
     int flag_jumps;
     double jump_refine, jump_coarsen;
+    // nbr holds plain neighbour cells (-1 on the boundary; swe_mesh_desc's
+    // layout) instead of the packed (o << 2) | back edge: the back edge of an
+    // in-chunk neighbour is looked up in the chunk's staged topology, and a
+    // neighbour outside [0, n) reads as a domain boundary
+    int raw_nbr;
+    Scen scen0;                            // the scenario when scen is null
 };
 
 // Discrete adjoint of one traversal (swe_adjoint_step.hip).
@@ -174,5 +180,9 @@ hipError_t launch_adjoint(const AdjArgs &a, int solver, int chunk, int64_t grid_
 // grid_blocks: number of workgroups (multiple of 8) or 0 for one per chunk.
 hipError_t launch_step(const StepArgs &a, int solver, int chunk, int64_t grid_blocks,
                        hipStream_t s);
+// The step of a single mesh whose nbr holds plain neighbour cells
+// (StepArgs::raw_nbr), chunk DEFAULT_CHUNK, one workgroup per chunk, flags
+// by volume_op's criterion; then the finalize kernel.
+hipError_t launch_step_raw(const StepArgs &a, int solver, hipStream_t s);
 
 }  // namespace swe

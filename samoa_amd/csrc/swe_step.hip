@@ -58,8 +58,9 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 // unspilled ones: the LDS of four 256-cell workgroups fits a CU.
 // JUMPS: the refinement flags follow the elevation-jump criterion
 // (StepArgs::flag_jumps); a template parameter so that the plain kernel's
-// code and registers do not pay for it.
-template <int SOLVER, int CH, bool JUMPS>
+// code and registers do not pay for it.  RAW: one mesh (StepArgs::scen0)
+// whose nbr holds plain neighbour cells (StepArgs::raw_nbr).
+template <int SOLVER, int CH, bool JUMPS, bool RAW = false>
 __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void euler_step_kernel(
     StepArgs a) {
     constexpr int NW = CH / 64;
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
         if (i >= q + (xcd < r ? 1 : 0)) continue;  // uniform across the workgroup
         const int64_t chunk = xcd * q + (xcd < r ? xcd : r) + i;
         const int s = a.chunk_scen ? a.chunk_scen[chunk] : 0;
-        const Scen sc = a.scen[s];
+        const Scen sc = RAW ? a.scen0 : a.scen[s];
         const int64_t lbase = (chunk - sc.chunk_off) * CH;  // first local cell of the chunk
         const int cnt = (int)((sc.n - lbase) < CH ? (sc.n - lbase) : CH);
         const bool active = t < cnt;
@@ -114,6 +115,11 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
             nb[1] = a.nbr[3 * (gt + lc_t) + 1];
             nb[2] = a.nbr[3 * (gt + lc_t) + 2];
             plot = a.plotter[gt + lc_t];
+            if (RAW) {  // a neighbour outside the mesh reads as the boundary
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    if ((uint32_t)nb[k] >= (uint64_t)sc.n) nb[k] = -1;
+            }
         }
         s_nbr[0][t] = nb[0];
         s_nbr[1][t] = nb[1];
@@ -127,7 +133,7 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int32_t p = nb[k];
-                const int64_t o = (int64_t)(p >> 2);
+                const int64_t o = (int64_t)(RAW ? p : p >> 2);
                 const bool inside = p >= 0 && o >= lbase && o < lbase + cnt;
                 if (!inside || o > lc_t) own |= 1 << k;
             }
@@ -160,9 +166,16 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
             const double nx = s_normal[(pi * 3 + k) * 2], ny = s_normal[(pi * 3 + k) * 2 + 1];
             const double qe = s_q[0][lc], qu = s_q[1][lc], qv = s_q[2][lc], qb = s_q[3][lc];
             const bool bnd = p < 0;
-            const int64_t o = bnd ? cc : (int64_t)(p >> 2);
-            const int ko = p & 3;
+            const int64_t o = bnd ? cc : (int64_t)(RAW ? p : p >> 2);
             const bool inside = !bnd && o >= lbase && o < lbase + cnt;
+            // the neighbour's edge back to this cell: packed with it, or for
+            // plain neighbours found among the in-chunk neighbour's own
+            // (two cells of a conforming mesh share at most one edge)
+            int ko = p & 3;
+            if (RAW && inside) {
+                const int lo = (int)(o - lbase);
+                ko = s_nbr[0][lo] == (int32_t)cc ? 0 : (s_nbr[1][lo] == (int32_t)cc ? 1 : 2);
+            }
             double oe, ou, ov, ob;
             if (bnd) {  // ghost state of bnd_skeleton_scalar_op (outflow copy or mirror)
                 oe = qe;
@@ -249,7 +262,7 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                     for (int k = 0; k < 3; ++k) {
                         const int32_t p = s_nbr[k][t];
                         if (p < 0) continue;
-                        const int64_t o = (int64_t)(p >> 2);
+                        const int64_t o = (int64_t)(RAW ? p : p >> 2);
                         double oe, ob;
                         if (o >= lbase && o < lbase + cnt) {
                             oe = s_q[0][o - lbase];
@@ -346,7 +359,7 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(StepArgs a) {
     __shared__ long long s_ref[FIN_THREADS / 64];
     const int s = blockIdx.x;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const Scen sc = a.scen[s];
+    const Scen sc = a.scen ? a.scen[s] : a.scen0;
     const double huge = 1.7976931348623157e308;  // huge(1.0_SR)
     double m0 = huge, m1 = huge, m2 = huge, m3 = huge;
     long long r = 0;
@@ -430,8 +443,34 @@ static hipError_t launch_chunk(const StepArgs &a, int chunk, int64_t grid_blocks
     return hipGetLastError();
 }
 
+template <int SOLVER>
+static void launch_raw_t(const StepArgs &a, hipStream_t s) {
+    constexpr int CH = DEFAULT_CHUNK;
+    const int64_t nvirt = 8 * ((a.nchunks + 7) / 8);
+    hipLaunchKernelGGL((euler_step_kernel<SOLVER, CH, false, true>), dim3((unsigned)nvirt),
+                       dim3(CH), 0, s, a);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(FIN_THREADS), 0, s, a);
+}
+
+hipError_t launch_step_raw(const StepArgs &a, int solver, hipStream_t s) {
+    if (a.scen || a.chunk_scen || a.n_scen != 1 || !a.raw_nbr || a.flag_jumps ||
+        a.nchunks != (a.scen0.n + DEFAULT_CHUNK - 1) / DEFAULT_CHUNK)
+        return hipErrorInvalidValue;
+    switch (solver) {
+        case 0: launch_raw_t<0>(a, s); break;
+        case 1: launch_raw_t<1>(a, s); break;
+        case SOLVER_LF: launch_raw_t<SOLVER_LF>(a, s); break;
+        case SOLVER_LLF: launch_raw_t<SOLVER_LLF>(a, s); break;
+        case SOLVER_LF_BATH: launch_raw_t<SOLVER_LF_BATH>(a, s); break;
+        case SOLVER_LLF_BATH: launch_raw_t<SOLVER_LLF_BATH>(a, s); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_step(const StepArgs &a, int solver, int chunk, int64_t grid_blocks,
                        hipStream_t s) {
+    if (a.raw_nbr || !a.scen) return hipErrorInvalidValue;
     if (grid_blocks < 0 || grid_blocks % 8) return hipErrorInvalidValue;
     switch (solver) {
         case 0: return launch_chunk<0>(a, chunk, grid_blocks, s);
