@@ -68,7 +68,7 @@ typedef struct swe_mesh_desc {
     const int8_t *plotter;  /* [n_cells] */
     const int32_t *nbr;     /* [3*n_cells] */
     double scaling;         /* cfg%scaling: edge length of the unit square in metres */
-    double g;               /* gravitational constant (SWE_data_types.f90:34: 9.80665) */
+    double g;               /* gravitational constant (SWE_data_types.f90:34: 9.80665), in [1e-10, 1e10] */
     double dry_tolerance;   /* cfg%dry_tolerance, >= 1e-100 */
     double courant;         /* cfg%courant_number (applied by swe_run only) */
     int32_t min_depth;      /* cfg%i_min_depth, for the refinement flags */
@@ -302,6 +302,24 @@ int swe_set_stream(swe_handle *h, void *stream);
 int swe_set_tuning(swe_handle *h, int chunk_cells, int64_t grid_blocks);
 int swe_get_tuning(swe_handle *h, int *chunk_cells, int64_t *grid_blocks);
 
+/* Arithmetic of the step kernels (no counterpart in the reference).
+ * SWE_ARITH_EXACT (the default): every operation rounds as the reference's
+ * build with -ffp-contract=off rounds it; results are bit-identical to it.
+ * SWE_ARITH_TOLERANCED: the GeoClaw solvers (SWE_SOLVER_AUG_RIEMANN,
+ * SWE_SOLVER_FWAVE) share reciprocals of repeated denominators, take
+ * hardware reciprocal and square-root estimates refined by Newton steps,
+ * contract products into fused multiply-adds and solve each edge from
+ * per-cell velocities; one step stays within a few ulps of the terms it
+ * sums (tested against the exact path at 1e-12 of the largest field value
+ * per step and 1e-9 after 1000 steps), and refinement flags can differ where
+ * a criterion sits within rounding of its threshold.  The Lax-Friedrichs
+ * solvers and the device-pointer sweeps of the adjoint (swe_forward_device,
+ * swe_batch_forward_device) always round exactly.  Applies to swe_step,
+ * swe_run and swe_step_device (and their batch forms) from the next call. */
+#define SWE_ARITH_EXACT 0
+#define SWE_ARITH_TOLERANCED 1
+int swe_set_arithmetic(swe_handle *h, int mode);
+
 /* Releases the handle and its device memory. NULL is accepted. */
 int swe_destroy(swe_handle *h);
 
@@ -344,6 +362,8 @@ int swe_batch_run(swe_batch *b, int64_t n_steps, const double *dt0, int forward,
 int swe_batch_set_tuning(swe_batch *b, int chunk_cells, int64_t grid_blocks);
 int swe_batch_get_tuning(swe_batch *b, int *chunk_cells, int64_t *grid_blocks);
 int swe_batch_set_stream(swe_batch *b, void *stream);
+/* As swe_set_arithmetic, for every scenario of the batch. */
+int swe_batch_set_arithmetic(swe_batch *b, int mode);
 /* Total cells of all scenarios, or -1 for a NULL batch. */
 int64_t swe_batch_cells(swe_batch *b);
 /* First cell of a scenario in the concatenated state arrays, or -1. */

@@ -76,6 +76,8 @@ SIGNATURES = {
     "swe_step_device_noalloc": (_I, [C.POINTER(DeviceMesh), _P, _P, _P, _P, _P, _P, _P, _D, _I,
                                      _P, _P, _P, _P]),
     "swe_set_stream": (_I, [_P, _P]),
+    "swe_set_arithmetic": (_I, [_P, _I]),
+    "swe_batch_set_arithmetic": (_I, [_P, _I]),
     "swe_adapt": (_I, [_P, _P, C.POINTER(_I64), C.POINTER(_I)]),
     "swe_keep_refinement_flags": (_I, [_P, _I]),
     "swe_keep_eta_jump_flags": (_I, [_P, _D, _D]),

@@ -26,6 +26,11 @@ K-th state (K = ceil(sqrt(n_steps)), so about 2 sqrt(n_steps) states are
 alive at once) and the backward pass recomputes one segment of K states at a
 time, recording the limiter masks, then runs the adjoint through it
 backwards.  The recompute costs one extra forward sweep.
+
+`PooledGaugeInversion` runs independent instances (one mesh each) with a
+checkpoint pool per instance (plan_pools): store-all or sqrt(N) chosen per
+instance against the HBM budget, and admission rounds when even the
+checkpointed tapes of all instances exceed it.
 """
 from __future__ import annotations
 
@@ -231,6 +236,190 @@ class GaugeInversion:
 
     def close(self):
         self.op.close()
+
+
+TAPE_STATE_BYTES = 24   # eta, hu, hv per cell
+TAPE_MASK_BYTES = 1     # limiter masks per cell
+
+
+def tape_bytes(n_cells: int, n_steps: int, store_all: bool, K: int = 1) -> float:
+    """HBM a tape takes: every state and mask (store_all), or the
+    checkpoints plus one recomputed segment of K states and masks."""
+    per = (TAPE_STATE_BYTES + TAPE_MASK_BYTES) * float(n_cells)
+    if store_all:
+        return n_steps * per
+    return (math.ceil(n_steps / K) + K + 2) * per
+
+
+def plan_pools(cells, n_steps: int, hbm_budget: float, checkpoint_every="auto"):
+    """Per-instance checkpoint pools for independent instances of `cells[s]`
+    cells each, admitted against an HBM budget for their tapes.
+
+    Returns a list of rounds; a round is a list of groups (store_all, K,
+    [instances]) whose tapes are alive together; rounds run one after the
+    other, each freeing its tapes before the next is admitted.
+
+    "auto": every instance keeps all its states if the sum fits; otherwise
+    the instances that save the most (the largest) switch to sqrt(N)
+    checkpoints (K = ceil(sqrt(n_steps))) until it fits; if even all of them
+    checkpointed exceed the budget, instances are admitted in rounds (first
+    fit, largest first), and inside a round the smallest instances keep all
+    their states while that fits.  An instance whose checkpointed tape alone
+    exceeds the budget raises MemoryError.  "all" or an int K: one round with
+    that mode for every instance."""
+    cells = [int(c) for c in cells]
+    idx = list(range(len(cells)))
+    K = max(1, math.ceil(math.sqrt(n_steps)))
+    if checkpoint_every == "all":
+        return [[(True, 1, idx)]]
+    if checkpoint_every != "auto":
+        Kc = int(checkpoint_every)
+        if Kc < 1:
+            raise ValueError("checkpoint_every must be >= 1")
+        return [[(False, Kc, idx)]]
+    full = {s: tape_bytes(cells[s], n_steps, True) for s in idx}
+    ckpt = {s: tape_bytes(cells[s], n_steps, False, K) for s in idx}
+    for s in idx:
+        if ckpt[s] > hbm_budget:
+            raise MemoryError(f"instance {s}: {cells[s]} cells x {n_steps} steps do not fit a "
+                              f"{hbm_budget / 2**30:.2f} GiB checkpoint budget")
+
+    def modes(members):
+        """store-all for as many as fit, smallest first; the rest checkpointed."""
+        keep = set()
+        total = sum(ckpt[s] for s in members)
+        for s in sorted(members, key=lambda s: (full[s] - ckpt[s], s)):
+            extra = full[s] - ckpt[s]
+            if total + extra <= hbm_budget:
+                keep.add(s)
+                total += extra
+        groups = []
+        a = [s for s in members if s in keep]
+        c = [s for s in members if s not in keep]
+        if a:
+            groups.append((True, 1, a))
+        if c:
+            groups.append((False, K, c))
+        return groups
+
+    if sum(ckpt.values()) <= hbm_budget:
+        return [modes(idx)]
+    rounds, load = [], []
+    for s in sorted(idx, key=lambda s: (-ckpt[s], s)):
+        for r, members in enumerate(rounds):
+            if load[r] + ckpt[s] <= hbm_budget:
+                members.append(s)
+                load[r] += ckpt[s]
+                break
+        else:
+            rounds.append([s])
+            load.append(ckpt[s])
+    return [modes(sorted(m)) for m in rounds]
+
+
+class PooledGaugeInversion:
+    """Independent instances (one mesh each), each with its own checkpoint
+    pool (plan_pools): the instances of one tape mode are advanced together
+    by one launch per step (a GaugeInversion over a Batch per group), and the
+    groups of one admission round have their tapes alive together.
+
+    Gauges are cell indices per instance; eta0 and b are concatenated over
+    the instances in instance order, as the series' columns are over their
+    gauges.  `forward_backward(eta0, b, loss_fn)` runs round after round and
+    returns per-instance results (the loss of each instance only depends on
+    its own gauges)."""
+
+    def __init__(self, meshes, gauges, n_steps: int, params, *, checkpoint_every="auto",
+                 hbm_budget: float | None = None, device="cuda:0"):
+        self.device = torch.device(device)
+        self.n_inst = len(meshes)
+        self.n_steps = int(n_steps)
+        cells = [m.n for m in meshes]
+        self.offsets = np.cumsum([0] + cells)
+        if hbm_budget is None:
+            free, _ = torch.cuda.mem_get_info(self.device)
+            hbm_budget = 0.5 * free
+        self.hbm_budget = float(hbm_budget)
+        self.rounds = plan_pools(cells, self.n_steps, self.hbm_budget, checkpoint_every)
+        g_counts = [len(g) for g in gauges]
+        g_offs = np.cumsum([0] + g_counts)
+        self.gauge_count = int(g_offs[-1])
+        self.groups = []
+        for r, groups in enumerate(self.rounds):
+            for store_all, K, members in groups:
+                cell_idx = np.concatenate([np.arange(self.offsets[s], self.offsets[s + 1])
+                                           for s in members])
+                local_off = np.cumsum([0] + [cells[s] for s in members])
+                g_local = np.concatenate([np.asarray(gauges[s], np.int64) + local_off[i]
+                                          for i, s in enumerate(members)])
+                cols = np.concatenate([np.arange(g_offs[s], g_offs[s + 1]) for s in members])
+                prob = GaugeInversion([meshes[s] for s in members], g_local, n_steps,
+                                      [params[s] for s in members],
+                                      checkpoint_every="all" if store_all else K,
+                                      device=device)
+                self.groups.append(dict(
+                    round=r, store_all=store_all, K=prob.K, members=list(members), prob=prob,
+                    cells=torch.as_tensor(cell_idx, device=self.device),
+                    cols=torch.as_tensor(cols, device=self.device),
+                    tape_bytes=sum(tape_bytes(cells[s], n_steps, store_all, prob.K)
+                                   for s in members)))
+
+    def dt_of(self, instance: int) -> torch.Tensor:
+        """The frozen dt sequence of one instance, [n_steps]."""
+        for g in self.groups:
+            if instance in g["members"]:
+                return g["prob"].dt[: self.n_steps, g["members"].index(instance)]
+        raise IndexError(instance)
+
+    def plan(self) -> dict:
+        """The admission plan: per round, its groups' modes and instances."""
+        return {"hbm_budget": self.hbm_budget, "rounds": [
+            [{"store_all": g["store_all"], "K": g["K"], "instances": g["members"],
+              "tape_bytes": g["tape_bytes"]} for g in self.groups if g["round"] == r]
+            for r in range(len(self.rounds))]}
+
+    def record_dt(self, eta0: torch.Tensor, b: torch.Tensor, dt0) -> None:
+        dt0 = np.broadcast_to(np.asarray(dt0, np.float64), (self.n_inst,))
+        for g in self.groups:
+            g["prob"].record_dt(eta0.detach()[g["cells"]].contiguous(),
+                                b.detach()[g["cells"]].contiguous(), dt0[g["members"]])
+
+    def gauge_series(self, eta0: torch.Tensor, b: torch.Tensor) -> torch.Tensor:
+        """The series of every instance (columns in instance and gauge order)
+        without keeping any tape (no gradients)."""
+        out = torch.empty(self.n_steps, self.gauge_count, dtype=torch.float64, device=self.device)
+        with torch.no_grad():
+            for g in self.groups:
+                out[:, g["cols"]] = g["prob"].gauge_series(eta0[g["cells"]].contiguous(),
+                                                           b[g["cells"]].contiguous())
+        return out
+
+    def forward_backward(self, make_inputs, loss_fn):
+        """make_inputs() -> (eta0, b, leaves): fresh differentiable inputs,
+        rebuilt for every round; loss_fn(series, cols, members) -> per-member
+        losses.  Runs each round's forward sweeps, then its backward sweep
+        (freeing its tapes), and returns ([per-instance loss], grads of
+        `leaves` summed over rounds)."""
+        losses = torch.zeros(self.n_inst, dtype=torch.float64, device=self.device)
+        grads = None
+        for r in range(len(self.rounds)):
+            eta0, b, leaves = make_inputs()
+            total = None
+            for g in self.groups:
+                if g["round"] != r:
+                    continue
+                G = g["prob"].gauge_series(eta0[g["cells"]].contiguous(), b[g["cells"]].contiguous())
+                per = loss_fn(G, g["cols"], g["members"])
+                losses[g["members"]] = per.detach()
+                total = per.sum() if total is None else total + per.sum()
+            gr = torch.autograd.grad(total, leaves, allow_unused=True)
+            gr = [torch.zeros_like(l) if x is None else x for x, l in zip(gr, leaves)]
+            grads = gr if grads is None else [a + c for a, c in zip(grads, gr)]
+        return losses, grads
+
+    def close(self):
+        for g in self.groups:
+            g["prob"].close()
 
 
 class _GaugeSeries(torch.autograd.Function):

@@ -133,7 +133,7 @@ int ensure_tables(int device) {
 // would otherwise capture again.
 struct GraphCache {
     hipGraphExec_t exec = nullptr;
-    int cur = -1, forward = -1, chunk = 0, keep = -1, jumps = -1;
+    int cur = -1, forward = -1, chunk = 0, keep = -1, jumps = -1, tol = -1;
     double jump_refine = 0.0, jump_coarsen = 0.0;
     int64_t grid = -1;
 };
@@ -199,6 +199,7 @@ struct Batch {
     hipStream_t stream = nullptr;      // where work is launched (caller may replace it)
     hipStream_t own_stream = nullptr;  // created by the library; used for graph capture
     int solver = 0, bc = 0;
+    int tol = 0;  // SWE_ARITH_TOLERANCED: the toleranced step kernels
     double g = 9.80665;
     int32_t nscen = 0;
     std::vector<swe::Scen> scen;
@@ -382,9 +383,11 @@ int batch_init(Batch &B, int32_t nscen, const swe_mesh_desc *descs) {
                         (long long)n);
         if (!d.depth || !d.plotter || !d.nbr)
             return fail(SWE_ERR_ARG, "scenario %d: null mesh array", s);
-        if (!(d.scaling > 0.0) || !(d.g > 0.0) || !(d.dry_tolerance >= swe::MIN_DRY_TOLERANCE))
-            return fail(SWE_ERR_ARG, "scenario %d: scaling and g must be > 0, dry_tolerance >= %g",
-                        s, swe::MIN_DRY_TOLERANCE);
+        if (!(d.scaling > 0.0) || !(d.g >= swe::MIN_G && d.g <= swe::MAX_G) ||
+            !(d.dry_tolerance >= swe::MIN_DRY_TOLERANCE))
+            return fail(SWE_ERR_ARG,
+                        "scenario %d: scaling must be > 0, g in [%g, %g], dry_tolerance >= %g", s,
+                        swe::MIN_G, swe::MAX_G, swe::MIN_DRY_TOLERANCE);
         if (d.solver < SWE_SOLVER_AUG_RIEMANN || d.solver > SWE_SOLVER_LLF_BATH)
             return fail(SWE_ERR_ARG, "scenario %d: unknown solver %d", s, d.solver);
         if (d.boundary != SWE_BC_OUTFLOW && d.boundary != SWE_BC_WALL)
@@ -484,6 +487,7 @@ swe::StepArgs make_args(const Batch &B, int from, int to, int forward) {
     a.flag_jumps = B.jump_flags;
     a.jump_refine = B.jump_refine;
     a.jump_coarsen = B.jump_coarsen;
+    a.tol = B.tol;
     return a;
 }
 
@@ -583,6 +587,7 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
         GraphCache &gc = B.graph[B.cur][fwd];
         if (!gc.exec || gc.cur != B.cur || gc.forward != fwd || gc.chunk != B.chunk ||
             gc.grid != B.grid || gc.keep != B.keep_flags || gc.jumps != B.jump_flags ||
+            gc.tol != B.tol ||
             gc.jump_refine != B.jump_refine || gc.jump_coarsen != B.jump_coarsen) {
             if (gc.exec) {
                 (void)hipGraphExecDestroy(gc.exec);
@@ -607,6 +612,7 @@ int batch_run(Batch &B, int64_t n_steps, const double *dt0, int forward, double 
             gc.grid = B.grid;
             gc.keep = B.keep_flags;
             gc.jumps = B.jump_flags;
+            gc.tol = B.tol;
             gc.jump_refine = B.jump_refine;
             gc.jump_coarsen = B.jump_coarsen;
         }
@@ -935,6 +941,7 @@ int batch_forward_device(Batch &B, const double *eta, const double *hu, const do
     a.masks = masks;
     a.out_dtcfl = dt_cfl_dev;
     a.skip_finalize = dt_cfl_dev ? 0 : 1;  // dt comes from the caller: no result is read
+    a.tol = 0;  // the adjoint's sweeps: the rounding its kernels differentiate
     HIP_TRY(swe::launch_step(a, B.solver, B.chunk, B.grid, st));
     return SWE_OK;
 }
@@ -984,6 +991,15 @@ int batch_adjoint_device(Batch &B, const double *eta, const double *hu, const do
 int batch_set_stream(Batch &B, void *stream) {
     HIP_TRY(hipStreamSynchronize(B.stream));
     B.stream = stream ? (hipStream_t)stream : B.own_stream;
+    return SWE_OK;
+}
+
+int batch_set_arithmetic(Batch &B, int mode) {
+    if (mode != SWE_ARITH_EXACT && mode != SWE_ARITH_TOLERANCED)
+        return fail(SWE_ERR_ARG, "arithmetic mode %d (SWE_ARITH_EXACT or SWE_ARITH_TOLERANCED)",
+                    mode);
+    HIP_TRY(hipStreamSynchronize(B.stream));
+    B.tol = mode == SWE_ARITH_TOLERANCED ? 1 : 0;
     return SWE_OK;
 }
 
@@ -1200,6 +1216,11 @@ int swe_set_tuning(swe_handle *h, int chunk_cells, int64_t grid_blocks) {
     return batch_set_tuning(h->B, chunk_cells, grid_blocks);
 }
 
+int swe_set_arithmetic(swe_handle *h, int mode) {
+    if (!h) return fail(SWE_ERR_ARG, "swe_set_arithmetic: null handle");
+    return batch_set_arithmetic(h->B, mode);
+}
+
 int swe_get_tuning(swe_handle *h, int *chunk_cells, int64_t *grid_blocks) {
     if (!h || !chunk_cells || !grid_blocks)
         return fail(SWE_ERR_ARG, "swe_get_tuning: null argument");
@@ -1313,10 +1334,11 @@ int swe_step_device_noalloc(const swe_device_mesh *m, const double *eta, const d
         return fail(SWE_ERR_ARG,
                     "swe_step_device_noalloc: n_cells %lld out of range (1..%lld, the size of "
                     "its fixed scratch)", (long long)n, (long long)NOALLOC_MAX_CELLS);
-    if (!(m->scaling > 0.0) || !(m->g > 0.0) || !(m->dry_tolerance >= swe::MIN_DRY_TOLERANCE))
+    if (!(m->scaling > 0.0) || !(m->g >= swe::MIN_G && m->g <= swe::MAX_G) ||
+        !(m->dry_tolerance >= swe::MIN_DRY_TOLERANCE))
         return fail(SWE_ERR_ARG,
-                    "swe_step_device_noalloc: scaling and g must be > 0, dry_tolerance >= %g",
-                    swe::MIN_DRY_TOLERANCE);
+                    "swe_step_device_noalloc: scaling must be > 0, g in [%g, %g], "
+                    "dry_tolerance >= %g", swe::MIN_G, swe::MAX_G, swe::MIN_DRY_TOLERANCE);
     if (m->solver < SWE_SOLVER_AUG_RIEMANN || m->solver > SWE_SOLVER_LLF_BATH)
         return fail(SWE_ERR_ARG, "swe_step_device_noalloc: unknown solver %d", m->solver);
     if (m->boundary != SWE_BC_OUTFLOW && m->boundary != SWE_BC_WALL)
@@ -1423,6 +1445,11 @@ int swe_batch_run(swe_batch *b, int64_t n_steps, const double *dt0, int forward,
 int swe_batch_set_tuning(swe_batch *b, int chunk_cells, int64_t grid_blocks) {
     if (!b) return fail(SWE_ERR_ARG, "swe_batch_set_tuning: null batch");
     return batch_set_tuning(b->B, chunk_cells, grid_blocks);
+}
+
+int swe_batch_set_arithmetic(swe_batch *b, int mode) {
+    if (!b) return fail(SWE_ERR_ARG, "swe_batch_set_arithmetic: null batch");
+    return batch_set_arithmetic(b->B, mode);
 }
 
 int swe_batch_get_tuning(swe_batch *b, int *chunk_cells, int64_t *grid_blocks) {

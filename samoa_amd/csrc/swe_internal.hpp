@@ -71,6 +71,8 @@ struct StepArgs {
     // neighbour outside [0, n) reads as a domain boundary
     int raw_nbr;
     Scen scen0;                            // the scenario when scen is null
+    // 1: toleranced arithmetic (swe_riemann_tol.hpp) for the GeoClaw solvers
+    int tol;
 };
 
 // Discrete adjoint of one traversal (swe_adjoint_step.hip).

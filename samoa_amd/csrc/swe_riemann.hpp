@@ -34,6 +34,34 @@ SWE_HD double dmax2(double a, double b) { return a > b ? a : b; }
 // divides by is a normal number whose square and reciprocal are finite, which
 // the exact-zero arguments of wall_hstar and riemanntype rely on.
 constexpr double MIN_DRY_TOLERANCE = 1.0e-100;
+// Range of the gravitational constant the library accepts: with it and the
+// dry tolerance above, g h of a wet side is at least 1e-110.
+constexpr double MIN_G = 1.0e-10, MAX_G = 1.0e10;
+
+// sqrt(x) for x == 0, x == +inf or x >= 2^-767: the compiler's correctly
+// rounded FP64 square root without its range scaling.  The compiler scales
+// an argument below 2^-767 by 2^256 before its refinement (and the root back
+// by 2^-128), and returns zeros and infinities as they are; for the
+// arguments allowed here the scaling is by 2^0, so the refinement below --
+// the same operations in the same order -- gives the same bits.  Used for
+// sqrt(g h) of a side whose depth is 0 or above the dry tolerance.
+SWE_HD double sqrt_wet(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double g0 = __builtin_amdgcn_rsq(x);
+    const double s0 = x * g0;
+    const double h0 = g0 * 0.5;
+    const double r0 = __builtin_fma(-h0, s0, 0.5);
+    const double s1 = __builtin_fma(s0, r0, s0);
+    const double h1 = __builtin_fma(h0, r0, h0);
+    const double d0 = __builtin_fma(-s1, s1, x);
+    const double s2 = __builtin_fma(d0, h1, s1);
+    const double d1 = __builtin_fma(-s2, s2, x);
+    const double s3 = __builtin_fma(d1, h1, s2);
+    return __builtin_amdgcn_class(x, 0x260) ? x : s3;  // -0, +0, +inf
+#else
+    return sqrt(x);
+#endif
+}
 
 // riemannsolvers.f:507-618 (maxiter = 1); returns hm, s1m, s2m.
 // sqL = sqrt(g*hL) and sqR = sqrt(g*hR) are passed in: the reference
@@ -200,12 +228,12 @@ SWE_HD void geoclaw(double hL, double hR, double huL, double huR,
     const double delphi = (((huR - huL) * (uL + uR) - (uL * uR) * (hR - hL)) +
                            ((0.5 * g) * (((bR + hR) - bL) - hL)) * (hR + hL)) -
                           ((0.5 * g) * (hR + hL)) * (bR - bL);
-    // :123-132
-    const double sqL = sqrt(g * hL), sqR = sqrt(g * hR);
+    // :123-132 (a side's depth is 0 or above the dry tolerance here)
+    const double sqL = sqrt_wet(g * hL), sqR = sqrt_wet(g * hR);
     const double sL = uL - sqL;
     const double sR = uR + sqR;
     const double uhat = (sqL * uL + sqR * uR) / (sqR + sqL);
-    const double chat = sqrt((g * 0.5) * (hR + hL));
+    const double chat = sqrt_wet((g * 0.5) * (hR + hL));
     double sE1 = dmin2(sL, uhat - chat);
     double sE2 = dmax2(sR, uhat + chat);
 
@@ -299,19 +327,21 @@ SWE_HD void geoclaw(double hL, double hR, double huL, double huR,
         f21 = (((hR * uR) * vR - (hL * uL) * vL) - f20) - f22;  // :254
     }
 
-    // :148-153 wall masking
-    sw0 *= wall0;
-    sw1 *= wall1;
-    sw2 *= wall2;
-    f00 *= wall0;
-    f10 *= wall0;
-    f20 *= wall0;
-    f01 *= wall1;
-    f11 *= wall1;
-    f21 *= wall1;
-    f02 *= wall2;
-    f12 *= wall2;
-    f22 *= wall2;
+    // :148-153 wall masking (a factor 1.0 leaves every value as it is)
+    if (wall0 != 1.0 || wall1 != 1.0 || wall2 != 1.0) {
+        sw0 *= wall0;
+        sw1 *= wall1;
+        sw2 *= wall2;
+        f00 *= wall0;
+        f10 *= wall0;
+        f20 *= wall0;
+        f01 *= wall1;
+        f11 *= wall1;
+        f21 *= wall1;
+        f02 *= wall2;
+        f12 *= wall2;
+        f22 *= wall2;
+    }
 
     // :156-167 net updates by wave-speed sign, 50/50 at exactly zero
 #define SWE_SPLIT(sw, fa, fb, fc)          \

@@ -36,6 +36,7 @@
 
 #include "swe_internal.hpp"
 #include "swe_riemann.hpp"
+#include "swe_riemann_tol.hpp"
 
 namespace swe {
 
@@ -59,12 +60,21 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 // JUMPS: the refinement flags follow the elevation-jump criterion
 // (StepArgs::flag_jumps); a template parameter so that the plain kernel's
 // code and registers do not pay for it.  RAW: one mesh (StepArgs::scen0)
-// whose nbr holds plain neighbour cells (StepArgs::raw_nbr).
-template <int SOLVER, int CH, bool JUMPS, bool RAW = false>
+// whose nbr holds plain neighbour cells (StepArgs::raw_nbr).  TOL: the
+// toleranced arithmetic of swe_riemann_tol.hpp (StepArgs::tol; the GeoClaw
+// solvers only): the chunk stages each cell's depth, world velocity,
+// bathymetry, sqrt(g h) and 1/h instead of its state, keeps its own state in
+// registers, and solves every edge with the cell of the list entry on the
+// left (the Riemann problem seen from either side is the same, up to
+// rounding), so no operands are swapped.
+template <int SOLVER, int CH, bool JUMPS, bool RAW = false, bool TOL = false>
 __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void euler_step_kernel(
     StepArgs a) {
+    static_assert(!(RAW && TOL), "the toleranced kernel reads packed neighbours");
+    static_assert(!TOL || SOLVER == 0 || SOLVER == 1, "toleranced GeoClaw solvers only");
     constexpr int NW = CH / 64;
-    __shared__ double s_q[4][CH];        // eta, hu, hv, b of the chunk
+    __shared__ double s_q[4][CH];        // eta, hu, hv, b of the chunk (TOL: h, wu, wv, b)
+    __shared__ double s_x[TOL ? CH : 1];  // TOL: sqrt(g h) (0 when dry)
     __shared__ double s_f[4][3][CH];     // flux (h, hu, hv, s) per (edge, cell)
     __shared__ int32_t s_nbr[3][CH];     // packed neighbour (o << 2 | back edge)
     __shared__ int8_t s_plot[CH];
@@ -73,16 +83,28 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
     __shared__ double s_wmin[NW];
     __shared__ int s_wref[NW];
     // geometry tables in LDS: the per-lane indexed reads would otherwise be
-    // global loads that every edge waits for
-    __shared__ double s_normal[17 * 3 * 2];
-    __shared__ double s_leg[64];
+    // global loads that every edge waits for (the toleranced kernel reads
+    // them from constant memory: its LDS holds four workgroups per CU only
+    // without them)
+    __shared__ double s_normal[TOL ? 1 : 17 * 3 * 2];
+    __shared__ double s_leg[TOL ? 1 : 64];
+    auto normal_at = [&](int i) -> double {
+        if constexpr (TOL) return c_normal[i];
+        else return s_normal[i];
+    };
+    auto leg_at = [&](int i) -> double {
+        if constexpr (TOL) return c_leg[i];
+        else return s_leg[i];
+    };
 
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const double g = a.g, inv16g = a.inv16g;
-    for (int i = t; i < 17 * 3 * 2; i += CH) s_normal[i] = c_normal[i];
-    for (int i = t; i < 64; i += CH) s_leg[i] = c_leg[i];
-    __syncthreads();
+    if constexpr (!TOL) {
+        for (int i = t; i < 17 * 3 * 2; i += CH) s_normal[i] = c_normal[i];
+        for (int i = t; i < 64; i += CH) s_leg[i] = c_leg[i];
+        __syncthreads();
+    }
 
     // XCD-aware order: virtual block v runs on XCD v % 8 (the dispatcher deals
     // workgroups round robin and the grid is a multiple of 8); XCD x owns a
@@ -96,25 +118,49 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
         const int64_t chunk = xcd * q + (xcd < r ? xcd : r) + i;
         const int s = a.chunk_scen ? a.chunk_scen[chunk] : 0;
         const Scen sc = RAW ? a.scen0 : a.scen[s];
-        const int64_t lbase = (chunk - sc.chunk_off) * CH;  // first local cell of the chunk
+        // Indices below are 32-bit: a scenario holds at most 2^29 cells, and
+        // the state and topology arrays are addressed from per-scenario
+        // (gs_*) and per-chunk (c_*) base pointers that are uniform across
+        // the workgroup.
+        const int32_t lbase = (int32_t)((chunk - sc.chunk_off) * CH);  // first cell of the chunk
         const int cnt = (int)((sc.n - lbase) < CH ? (sc.n - lbase) : CH);
         const bool active = t < cnt;
-        const int64_t lc_t = lbase + t;                     // local cell of this thread
         const int64_t gs = sc.cell_off, gt = sc.topo_off;  // state / topology offsets
+        const double *const gs_eta = a.eta + gs, *const gs_hu = a.hu + gs, *const gs_hv = a.hv + gs;
+        const double *const gs_b = a.b + gs;
+        const double *const c_eta = gs_eta + lbase, *const c_hu = gs_hu + lbase;
+        const double *const c_hv = gs_hv + lbase, *const c_b = gs_b + lbase;
+        const int32_t *const c_nbr = a.nbr + 3 * (gt + lbase);
+        const int8_t *const c_plot = a.plotter + gt + lbase;
+        const int8_t *const c_depth = a.depth + gt + lbase;
+        const int64_t c_out = gs + lbase;  // first output cell of the chunk
         const double dt = a.device_dt ? a.slots[s].dt : (a.dt_host ? a.dt_host[s] : a.dt);
         const double drytol = sc.drytol;
 
         int32_t nb[3] = {-1, -1, -1};
         int8_t plot = 1;
+        double my_e = 0.0, my_u = 0.0, my_v = 0.0;  // TOL: this thread's cell state
         if (active) {
-            s_q[0][t] = a.eta[gs + lc_t];
-            s_q[1][t] = a.hu[gs + lc_t];
-            s_q[2][t] = a.hv[gs + lc_t];
-            s_q[3][t] = a.b[gs + lc_t];
-            nb[0] = a.nbr[3 * (gt + lc_t)];
-            nb[1] = a.nbr[3 * (gt + lc_t) + 1];
-            nb[2] = a.nbr[3 * (gt + lc_t) + 2];
-            plot = a.plotter[gt + lc_t];
+            if constexpr (TOL) {
+                my_e = c_eta[t];
+                my_u = c_hu[t];
+                my_v = c_hv[t];
+                const tol::CellQ cq = tol::cell_q(my_e, my_u, my_v, c_b[t], drytol, g);
+                s_q[0][t] = cq.h;
+                s_q[1][t] = cq.wu;
+                s_q[2][t] = cq.wv;
+                s_q[3][t] = cq.b;
+                s_x[t] = cq.sq;
+            } else {
+                s_q[0][t] = c_eta[t];
+                s_q[1][t] = c_hu[t];
+                s_q[2][t] = c_hv[t];
+                s_q[3][t] = c_b[t];
+            }
+            nb[0] = c_nbr[3 * t];
+            nb[1] = c_nbr[3 * t + 1];
+            nb[2] = c_nbr[3 * t + 2];
+            plot = c_plot[t];
             if (RAW) {  // a neighbour outside the mesh reads as the boundary
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
@@ -133,9 +179,9 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const int32_t p = nb[k];
-                const int64_t o = (int64_t)(RAW ? p : p >> 2);
-                const bool inside = p >= 0 && o >= lbase && o < lbase + cnt;
-                if (!inside || o > lc_t) own |= 1 << k;
+                const int lo = (RAW ? p : p >> 2) - lbase;  // chunk-local neighbour
+                const bool inside = p >= 0 && (uint32_t)lo < (uint32_t)cnt;
+                if (!inside || lo > t) own |= 1 << k;
             }
         }
         const int nown = __popc(own);
@@ -158,24 +204,77 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
         __syncthreads();
 
         for (int e = t; e < total; e += CH) {
+            if constexpr (TOL) {
+                const int ent = s_list[e];
+                const int lc = ent >> 2, k = ent & 3;
+                const int32_t p = s_nbr[k][lc];
+                const int pi = s_plot[lc] + 8;
+                const double nx = normal_at((pi * 3 + k) * 2), ny = normal_at((pi * 3 + k) * 2 + 1);
+                const double hC = s_q[0][lc], uC = s_q[1][lc], vC = s_q[2][lc], bC = s_q[3][lc];
+                const double sqC = s_x[lc];
+                const bool bnd = p < 0;
+                const int32_t o = RAW ? p : p >> 2;  // the neighbour (not read when bnd)
+                const int lo = o - lbase;
+                const bool inside = !bnd && (uint32_t)lo < (uint32_t)cnt;
+                double hO, uO, vO, bO, sqO;
+                if (bnd) {  // ghost: the cell itself, velocity mirrored at a wall
+                    hO = hC;
+                    bO = bC;
+                    sqO = sqC;
+                    uO = uC;
+                    vO = vC;
+                    if (a.bc == 1) {
+                        const double dp = uC * nx + vC * ny;
+                        uO = uC - (2.0 * dp) * nx;
+                        vO = vC - (2.0 * dp) * ny;
+                    }
+                } else if (inside) {
+                    hO = s_q[0][lo];
+                    uO = s_q[1][lo];
+                    vO = s_q[2][lo];
+                    bO = s_q[3][lo];
+                    sqO = s_x[lo];
+                } else {
+                    const tol::CellQ cq =
+                        tol::cell_q(gs_eta[o], gs_hu[o], gs_hv[o], gs_b[o], drytol, g);
+                    hO = cq.h;
+                    uO = cq.wu;
+                    vO = cq.wv;
+                    bO = cq.b;
+                    sqO = cq.sq;
+                }
+                Flux fc, fo;
+                tol::edge_flux<SOLVER>(nx, ny, hC, uC, vC, bC, sqC, hO, uO, vO, bO, sqO, drytol, g,
+                                       inv16g, fc, fo);
+                s_f[0][k][lc] = fc.h;
+                s_f[1][k][lc] = fc.hu;
+                s_f[2][k][lc] = fc.hv;
+                s_f[3][k][lc] = fc.s;
+                if (inside) {
+                    const int ko = p & 3;
+                    s_f[0][ko][lo] = fo.h;
+                    s_f[1][ko][lo] = fo.hu;
+                    s_f[2][ko][lo] = fo.hv;
+                    s_f[3][ko][lo] = fo.s;
+                }
+                continue;
+            }
             const int ent = s_list[e];
             const int lc = ent >> 2, k = ent & 3;
-            const int64_t cc = lbase + lc;
+            const int32_t cc = lbase + lc;
             const int32_t p = s_nbr[k][lc];
             const int pi = s_plot[lc] + 8;
-            const double nx = s_normal[(pi * 3 + k) * 2], ny = s_normal[(pi * 3 + k) * 2 + 1];
+            const double nx = normal_at((pi * 3 + k) * 2), ny = normal_at((pi * 3 + k) * 2 + 1);
             const double qe = s_q[0][lc], qu = s_q[1][lc], qv = s_q[2][lc], qb = s_q[3][lc];
             const bool bnd = p < 0;
-            const int64_t o = bnd ? cc : (int64_t)(RAW ? p : p >> 2);
-            const bool inside = !bnd && o >= lbase && o < lbase + cnt;
+            const int32_t o = bnd ? cc : (RAW ? p : p >> 2);
+            const int lo = o - lbase;  // chunk-local neighbour
+            const bool inside = !bnd && (uint32_t)lo < (uint32_t)cnt;
             // the neighbour's edge back to this cell: packed with it, or for
             // plain neighbours found among the in-chunk neighbour's own
             // (two cells of a conforming mesh share at most one edge)
             int ko = p & 3;
-            if (RAW && inside) {
-                const int lo = (int)(o - lbase);
-                ko = s_nbr[0][lo] == (int32_t)cc ? 0 : (s_nbr[1][lo] == (int32_t)cc ? 1 : 2);
-            }
+            if (RAW && inside) ko = s_nbr[0][lo] == cc ? 0 : (s_nbr[1][lo] == cc ? 1 : 2);
             double oe, ou, ov, ob;
             if (bnd) {  // ghost state of bnd_skeleton_scalar_op (outflow copy or mirror)
                 oe = qe;
@@ -188,20 +287,19 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                     ov = qv - (2.0 * dp) * ny;
                 }
             } else if (inside) {
-                const int lo = (int)(o - lbase);
                 oe = s_q[0][lo];
                 ou = s_q[1][lo];
                 ov = s_q[2][lo];
                 ob = s_q[3][lo];
             } else {
-                oe = a.eta[gs + o];
-                ou = a.hu[gs + o];
-                ov = a.hv[gs + o];
-                ob = a.b[gs + o];
+                oe = gs_eta[o];
+                ou = gs_hu[o];
+                ov = gs_hv[o];
+                ob = gs_b[o];
             }
             // one solver call per entry: the left state is this cell on the
             // boundary and when it comes first in the traversal direction
-            const bool c_left = bnd || (a.forward ? (cc < o) : (cc > o));
+            const bool c_left = bnd || (a.forward ? (lc < lo) : (lc > lo));
             const double sx = c_left ? nx : -nx, sy = c_left ? ny : -ny;
             Flux fL, fR;
             edge_flux<SOLVER>(sx, sy, c_left ? qe : oe, c_left ? qu : ou, c_left ? qv : ov,
@@ -214,7 +312,6 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
             s_f[3][k][lc] = fc.s;
             if (inside) {
                 const Flux fo = c_left ? fR : fL;
-                const int lo = (int)(o - lbase);
                 s_f[0][ko][lo] = fo.h;
                 s_f[1][ko][lo] = fo.hu;
                 s_f[2][ko][lo] = fo.hv;
@@ -227,10 +324,10 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
         double cand = 1.7976931348623157e308;  // huge(1.0_SR), pre_traversal_op :139
         int refined = 0;
         if (active) {
-            const int d = sc.depth >= 0 ? sc.depth : (int)a.depth[gt + lc_t];
+            const int d = sc.depth >= 0 ? sc.depth : (int)c_depth[t];
             const double S = sc.scaling;
             const double volume = (S * S) * ldexp(1.0, -(d + 1));
-            const double lleg = S * s_leg[d + 1], lhyp = S * s_leg[d];
+            const double lleg = S * leg_at(d + 1), lhyp = S * leg_at(d);
             const int k0 = a.forward ? 0 : 2, k2 = a.forward ? 2 : 0;
             double dh = 0.0, d1 = 0.0, d2 = 0.0, sv = 0.0;
             dh = dh + lleg * s_f[0][k0][t];
@@ -247,7 +344,38 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
             sv = sv + lleg * s_f[3][k2][t];
 
             int8_t rf = 0;
-            if (JUMPS) {
+            if constexpr (JUMPS && TOL) {
+                // as below, from the staged depths: eta = h + b
+                int jb = 0;
+                const double qe = my_e;
+                if (s_q[0][t] > drytol) {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const int32_t p = s_nbr[k][t];
+                        if (p < 0) continue;
+                        const int32_t o = p >> 2;
+                        const int lo = o - lbase;
+                        double oe, oh;
+                        if ((uint32_t)lo < (uint32_t)cnt) {
+                            oh = s_q[0][lo];
+                            oe = oh + s_q[3][lo];
+                        } else {
+                            oe = gs_eta[o];
+                            oh = oe - gs_b[o];
+                        }
+                        if (oh > drytol) {
+                            const double jd = fabs(qe - oe);
+                            jb |= (jd > a.jump_refine ? 1 : 0) | (jd >= a.jump_coarsen ? 2 : 0);
+                        }
+                    }
+                }
+                if (d < sc.dmax && (jb & 1)) {
+                    rf = 1;
+                    refined = 1;
+                } else if (d > sc.dmin && !(jb & 2) && a.jump_coarsen > 0.0) {
+                    rf = -1;
+                }
+            } else if (JUMPS) {
                 // elevation jumps to the wet neighbours of a wet cell, in
                 // the state the step reads: bit 0 if one is above the
                 // refinement tolerance, bit 1 if one is not below the
@@ -262,14 +390,15 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                     for (int k = 0; k < 3; ++k) {
                         const int32_t p = s_nbr[k][t];
                         if (p < 0) continue;
-                        const int64_t o = (int64_t)(RAW ? p : p >> 2);
+                        const int32_t o = RAW ? p : p >> 2;
+                        const int lo = o - lbase;
                         double oe, ob;
-                        if (o >= lbase && o < lbase + cnt) {
-                            oe = s_q[0][o - lbase];
-                            ob = s_q[3][o - lbase];
+                        if ((uint32_t)lo < (uint32_t)cnt) {
+                            oe = s_q[0][lo];
+                            ob = s_q[3][lo];
                         } else {
-                            oe = a.eta[gs + o];
-                            ob = a.b[gs + o];
+                            oe = gs_eta[o];
+                            ob = gs_b[o];
                         }
                         if (oe - ob > drytol) {
                             const double jd = fabs(qe - oe);
@@ -292,14 +421,15 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                     rf = -1;
                 }
             }
-            if (a.refine) a.refine[gs + lc_t] = rf;
+            if (a.refine) a.refine[c_out + t] = rf;
             cand = volume / sv;
             const double scale = -dt / volume;
             dh = scale * dh;
             d1 = scale * d1;
             d2 = scale * d2;
 
-            double qh = s_q[0][t], q1 = s_q[1][t], q2 = s_q[2][t];
+            double qh = TOL ? my_e : s_q[0][t], q1 = TOL ? my_u : s_q[1][t];
+            double q2 = TOL ? my_v : s_q[2][t];
             const double qb = s_q[3][t];
             int mk = 0;
             if (qh < qb + drytol && dh > 0.0) {  // wetting, :271-275
@@ -317,10 +447,10 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                 q2 = 0.0;
                 mk |= 2;
             }
-            if (a.masks) a.masks[gs + lc_t] = (int8_t)mk;
-            a.eta_out[gs + lc_t] = qh;
-            a.hu_out[gs + lc_t] = q1;
-            a.hv_out[gs + lc_t] = q2;
+            if (a.masks) a.masks[c_out + t] = (int8_t)mk;
+            a.eta_out[c_out + t] = qh;
+            a.hu_out[c_out + t] = q1;
+            a.hv_out[c_out + t] = q2;
         }
 
         // CFL minimum and refinement count of the chunk (post_traversal_grid_op
@@ -422,7 +552,15 @@ template <int SOLVER, int CH>
 static void launch_t(const StepArgs &a, int64_t grid_blocks, hipStream_t s) {
     const int64_t nvirt = 8 * ((a.nchunks + 7) / 8);
     const int64_t blocks = grid_blocks > 0 && grid_blocks < nvirt ? grid_blocks : nvirt;
-    if (a.flag_jumps)
+    constexpr bool GEOCLAW = SOLVER == 0 || SOLVER == 1;  // the toleranced solvers
+    if (GEOCLAW && a.tol) {
+        if (a.flag_jumps)
+            hipLaunchKernelGGL((euler_step_kernel<SOLVER, CH, true, false, GEOCLAW>),
+                               dim3((unsigned)blocks), dim3(CH), 0, s, a);
+        else
+            hipLaunchKernelGGL((euler_step_kernel<SOLVER, CH, false, false, GEOCLAW>),
+                               dim3((unsigned)blocks), dim3(CH), 0, s, a);
+    } else if (a.flag_jumps)
         hipLaunchKernelGGL((euler_step_kernel<SOLVER, CH, true>), dim3((unsigned)blocks), dim3(CH),
                            0, s, a);
     else

@@ -9,21 +9,23 @@ tsunami_state):
     eta0 = amplitude * exp(-((x - x0)/a)^2 - ((y - y0)/(a/ratio))^2)  (wet cells)
     eta0 = min(b, 0)                                                    (dry cells)
 
-The gauge series of all instances come from one batched forward sweep
-(GaugeInversion over a Batch: one launch per step for every instance) and
-their gradients from one batched adjoint sweep; the loss of instance s is the
-sum over steps and its gauges of (eta - eta_target)^2.  Because instances
-are independent, the gradient of the summed loss with respect to theta_s is
-the gradient of instance s's own loss, and `lbfgs` runs one L-BFGS
-(two-loop recursion, Armijo backtracking) per instance, in lockstep: every
-function evaluation advances all instances together.
+The gauge series of the instances come from batched forward sweeps and
+their gradients from batched adjoint sweeps (PooledGaugeInversion: each
+instance has its own checkpoint pool, store-all or sqrt(N) checkpoints,
+admitted against an HBM budget; the instances of one mode and admission
+round share one launch per step); the loss of instance s is the sum over
+steps and its gauges of (eta - eta_target)^2.  Because instances are
+independent, the gradient of the summed loss with respect to theta_s is the
+gradient of instance s's own loss, and `lbfgs` runs one L-BFGS (two-loop
+recursion, Armijo backtracking) per instance, in lockstep: every function
+evaluation advances all instances together.
 """
 from __future__ import annotations
 
 import numpy as np
 import torch
 
-from .adjoint import GaugeInversion
+from .adjoint import PooledGaugeInversion
 
 SEMI_MAJOR = 1.0e5                     # metres
 SCALE = (1.0, 1.0, 1.0e5, 1.0e5)      # optimisation units of (amp, ratio, x0, y0)
@@ -34,7 +36,7 @@ class SourceInversion:
     `b[s]` (numpy), gauges (cell indices per instance), n_steps, dt0[s]."""
 
     def __init__(self, meshes, b, gauges, n_steps, dt0, *, scaling=1.0e6, dry_tolerance=1e-3,
-                 courant=0.45, checkpoint_every="auto", device="cuda:0"):
+                 courant=0.45, checkpoint_every="auto", hbm_budget=None, device="cuda:0"):
         self.device = torch.device(device)
         self.n_inst = len(meshes)
         self.scaling = float(scaling)
@@ -44,9 +46,9 @@ class SourceInversion:
         params = [dict(scaling=scaling, dry_tolerance=dry_tolerance, courant=courant)
                   for _ in meshes]
         self.gauge_counts = [len(g) for g in gauges]
-        all_g = np.concatenate([np.asarray(g, np.int64) + o for g, o in zip(gauges, offs[:-1])])
-        self.problem = GaugeInversion(list(meshes), all_g, n_steps, params,
-                                      checkpoint_every=checkpoint_every, device=device)
+        self.problem = PooledGaugeInversion(list(meshes), gauges, n_steps, params,
+                                            checkpoint_every=checkpoint_every,
+                                            hbm_budget=hbm_budget, device=device)
         dev = self.device
         self.b = torch.tensor(np.concatenate(b), device=dev)
         cen = np.concatenate([m.centroids() for m in meshes]) * scaling
@@ -79,14 +81,20 @@ class SourceInversion:
 
     def loss_and_grad(self, z: torch.Tensor):
         """Per-instance loss [n_inst] and gradient [n_inst, 4] in optimisation
-        units (theta = z * SCALE)."""
-        z = z.detach().requires_grad_()
-        G = self.problem.gauge_series(self.eta0(z * self.scale), self.b)
-        r2 = ((G - self.target) ** 2).sum(0)                 # per gauge
-        per = torch.zeros(self.n_inst, dtype=torch.float64, device=self.device)
-        per = per.index_add(0, self.gauge_inst, r2)
-        per.sum().backward()
-        return per.detach(), z.grad.detach()
+        units (theta = z * SCALE), admission round after admission round."""
+
+        def make_inputs():
+            zz = z.detach().requires_grad_()
+            return self.eta0(zz * self.scale), self.b, [zz]
+
+        def loss_fn(G, cols, members):
+            r2 = ((G - self.target[:, cols]) ** 2).sum(0)    # per gauge
+            per = torch.zeros(self.n_inst, dtype=torch.float64, device=self.device)
+            per = per.index_add(0, self.gauge_inst[cols], r2)
+            return per[members]
+
+        losses, (gz,) = self.problem.forward_backward(make_inputs, loss_fn)
+        return losses, gz.detach()
 
     def close(self):
         self.problem.close()

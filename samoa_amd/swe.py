@@ -28,6 +28,8 @@ SOLVER_LF_BATH = 4
 SOLVER_LLF_BATH = 5
 BC_OUTFLOW = 0
 BC_WALL = 1
+ARITH_EXACT = 0        # every operation rounds as the reference's
+ARITH_TOLERANCED = 1   # shared reciprocals, refined estimates, fused multiply-adds
 G = 9.80665  # SWE_data_types.f90:34
 
 
@@ -215,6 +217,10 @@ class EulerTimestep:
         """Launch on a caller's HIP stream (0 = the library's own)."""
         _native.check("swe_set_stream", self._lib.swe_set_stream(self._h, stream or None))
 
+    def set_arithmetic(self, mode: int) -> None:
+        """ARITH_EXACT (bit-identical to the reference) or ARITH_TOLERANCED."""
+        _native.check("swe_set_arithmetic", self._lib.swe_set_arithmetic(self._h, int(mode)))
+
     def set_tuning(self, chunk: int, grid: int = 0) -> None:
         _native.check("swe_set_tuning", self._lib.swe_set_tuning(self._h, chunk, grid))
 
@@ -328,6 +334,11 @@ class Batch:
         _native.check("swe_batch_adjoint_device", self._lib.swe_batch_adjoint_device(
             self._h, eta, hu, hv, b, masks, leta, lhu, lhv, leta_out, lhu_out, lhv_out, gb,
             dt_dev, int(bool(forward)), stream or None))
+
+    def set_arithmetic(self, mode: int) -> None:
+        """ARITH_EXACT or ARITH_TOLERANCED for every scenario."""
+        _native.check("swe_batch_set_arithmetic",
+                      self._lib.swe_batch_set_arithmetic(self._h, int(mode)))
 
     def set_tuning(self, chunk: int, grid: int = 0) -> None:
         _native.check("swe_batch_set_tuning", self._lib.swe_batch_set_tuning(self._h, chunk, grid))
