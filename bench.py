@@ -57,7 +57,8 @@ def parse():
     p.add_argument("--steps", type=int, default=1000)
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--workload", default="all",
-                   choices=["all", "tsunami", "frozen", "ensemble", "adjoint", "inversion"])
+                   choices=["all", "tsunami", "frozen", "ensemble", "adjoint", "inversion",
+                            "config1"])
     p.add_argument("--ens-steps", type=int, default=500)
     p.add_argument("--ens-scenarios", type=int, default=64)
     p.add_argument("--depth", type=int, default=21)
@@ -67,8 +68,12 @@ def parse():
                    help="surface-elevation variation (m) across a cell that asks for refinement")
     p.add_argument("--adj-depth", type=int, default=20)
     p.add_argument("--adj-steps", type=int, default=200)
+    p.add_argument("--fd-dirs", type=int, default=20,
+                   help="finite-difference directions checked against the adjoint gradient")
     p.add_argument("--inv-instances", type=int, default=16)
     p.add_argument("--inv-iters", type=int, default=20)
+    p.add_argument("--inv-hbm-gib", type=float, default=24.0,
+                   help="HBM budget (GiB) the inversions' checkpoint pools are admitted against")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--tuning", default=None,
                    help="CHUNK,GRID launch shape instead of autotuning (e.g. for profiling)")
@@ -113,7 +118,7 @@ def cpu_baseline(mesh, state, dt, seconds_target=15.0):
                       f"(oracle/swe_oracle.c swo_step_omp, {t_used:.1f} s)"}
 
 
-def tsunami_adaptive(args, torch):
+def tsunami_adaptive(args, torch, arith=0):
     """The adaptive single tsunami: the tsunami scenario (1000 km square,
     b = -4000 m + Gaussian random field + coastal shelf, as a bilinear raster,
     elliptical source of seed 0) on a mesh refined from depth 16 to depth 26
@@ -136,6 +141,7 @@ def tsunami_adaptive(args, torch):
     op = swe.EulerTimestep(mesh0, scaling=S, dry_tolerance=1e-3, courant=courant,
                            min_depth=dmin, max_depth=dmax, solver=swe.SOLVER_AUG_RIEMANN,
                            boundary=swe.BC_OUTFLOW)
+    op.set_arithmetic(arith)
     z = np.zeros(mesh0.n)
     op.upload(z, z, z, z)
     # the Fortran host's set-up: the initial phase of SWE.f90:230-262 with a
@@ -271,6 +277,7 @@ def tsunami_frozen(args, torch):
     eta, hu, hv, b = op.download()
     if not (np.isfinite(eta).all() and np.isfinite(hu).all()):
         raise RuntimeError("non-finite state after the timed steps")
+    exact_end = (eta.copy(), hu.copy(), hv.copy())
 
     achieved = BYTES_PER_CELL * cells / (ms_per_step * 1e-3)
     traffic, valu = None, None
@@ -298,6 +305,37 @@ def tsunami_frozen(args, torch):
         "ms_per_step": 1e3 * s_host / r_host,
         "per_step": "swe_upload of eta, hu, hv, b from pageable host memory, swe_step, "
                     "swe_download of the four arrays",
+    }
+
+    # the toleranced arithmetic on the same mesh, from the same state, with
+    # its own launch shape: throughput, and its distance from the exact run
+    # after the same steps
+    op.set_arithmetic(swe.ARITH_TOLERANCED)
+    op.upload(*state)
+    tuned_tol = fixed_tuning(args, op) or autotune.tune(op, dt0, use_cache=False)
+    op.set_stream(stream.cuda_stream)
+    op.upload(*state)
+    tw_tol, dt_tol = op.run(args.warmup, dt0, forward=False)
+    start2, end2 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start2.record(stream)
+    t_tol, _ = op.run(args.steps, dt_tol, forward=fwd, time=tw_tol)
+    end2.record(stream)
+    torch.cuda.synchronize()
+    ms_tol = start2.elapsed_time(end2)
+    e2, u2, v2, _ = op.download()
+    op.set_arithmetic(swe.ARITH_EXACT)
+    ach_tol = BYTES_PER_CELL * cells / (ms_tol / args.steps * 1e-3)
+    toleranced = {
+        "value": cells * args.steps / (ms_tol * 1e-3), "unit": "cell-updates/s",
+        "ms_per_step": ms_tol / args.steps,
+        "roofline": {"bound": "hbm", "achieved": ach_tol / 1e9, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": ach_tol / HBM_PEAK},
+        "tuning": {"chunk": tuned_tol["chunk"], "grid": tuned_tol["grid"]},
+        "max_rel_diff_vs_exact": {
+            k: float(np.abs(x - y).max() / np.abs(y).max())
+            for k, x, y in zip(("eta", "hu", "hv"), (e2, u2, v2), exact_end)},
+        "simulated_time_rel_diff": abs(t_tol - t_end) / t_end,
     }
 
     copy_rate = None
@@ -345,6 +383,7 @@ def tsunami_frozen(args, torch):
         },
         "tuning": {"chunk": tuned["chunk"], "grid": tuned["grid"], "trials": tuned["trials"]},
         "host_buffers": host_buffers,
+        "toleranced": toleranced,
     }
     if copy_rate is not None:
         out["copy_1gib_bytes_per_s"] = copy_rate
@@ -354,7 +393,7 @@ def tsunami_frozen(args, torch):
     return out, mesh, state, dt0
 
 
-def ensemble(args, torch):
+def ensemble(args, torch, arith=0):
     """Batched ensemble: seeds 0..N-1, uneven uniform depths from 14-20."""
     from samoa_amd import autotune, host, swe
 
@@ -364,6 +403,7 @@ def ensemble(args, torch):
     ms = [meshes[p["depth"]] for p in draws]
     params = [dict(scaling=1.0e6, dry_tolerance=1e-3, courant=0.45) for _ in draws]
     b = swe.Batch(ms, params)
+    b.set_arithmetic(arith)
     dt0 = []
     for s, m in enumerate(ms):
         st = host.tsunami(m, seed=s, drytol=1e-3)
@@ -396,6 +436,86 @@ def ensemble(args, torch):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK},
         "workload": "tsunami_ensemble_64_uniform_d14-20",
     }
+
+
+def config1(args, torch):
+    """The radial dam break in a closed basin at its full size: the unit
+    square, uniform depth 14 (32,768 cells), flat bed, eta 10 m within 0.1 of
+    the centre and 5 m outside, reflecting walls, the augmented Riemann
+    solver, courant 0.45 and dry tolerance 0.01 as the reference parses them
+    (float32), 100 steps from a captured HIP graph with the device dt chain,
+    in both arithmetic modes.  Its cpu_baseline is the reference itself
+    (oracle/_ref/samoa_swe_radial_O3: the reference's sources with this
+    scenario's one-line patches, built with -O3 in the development container)
+    on this host's cores, read from its "Cell update throughput" line
+    (SWE.f90:474)."""
+    from samoa_amd import host, swe
+
+    mesh = host.mesh_uniform(14)
+    st = host.radial_dam_break(mesh)
+    courant, drytol = float(np.float32(0.45)), float(np.float32(0.01))
+    dt0 = host.initial_dt(mesh, st, courant)
+    out = {"workload": "radial_dam_break_walls_uniform_d14_100steps", "cells": mesh.n,
+           "steps": 100}
+    stream = torch.cuda.Stream()
+    for name, mode in (("exact", swe.ARITH_EXACT), ("toleranced", swe.ARITH_TOLERANCED)):
+        op = swe.EulerTimestep(mesh, scaling=1.0, dry_tolerance=drytol, courant=courant,
+                               min_depth=14, max_depth=14, boundary=swe.BC_WALL)
+        op.set_arithmetic(mode)
+        op.set_stream(stream.cuda_stream)
+        op.upload(*st)
+        op.run(100, dt0, forward=True, time=0.0)  # warm: graph capture
+        op.upload(*st)
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        t_end, _ = op.run(100, dt0, forward=True, time=0.0)
+        b.record(stream)
+        torch.cuda.synchronize()
+        ms = a.elapsed_time(b)
+        res = op.download()
+        op.close()
+        out[name] = {"value": mesh.n * 100 / (ms * 1e-3), "unit": "cell-updates/s",
+                     "ms_per_step": ms / 100, "simulated_time_s": t_end}
+        if name == "exact":
+            ref_state = res
+        else:
+            out[name]["max_rel_diff_vs_exact"] = {
+                k: float(np.abs(x - y).max() / np.abs(y).max())
+                for k, x, y in zip(("eta", "hu", "hv"), res[:3], ref_state[:3])}
+    if not args.no_cpu:
+        out["cpu_baseline"] = reference_cpu(14, 100)
+    return out
+
+
+def reference_cpu(depth, steps):
+    """The reference binary (radial dam break, -O3) on this host's cores."""
+    import re
+    import subprocess
+    import tempfile
+
+    binary = os.path.join(ROOT, "oracle", "_ref", "samoa_swe_radial_O3")
+    if not os.path.exists(binary):
+        return {"skipped": "oracle/_ref/samoa_swe_radial_O3 is not built (make -C oracle ref-timing)"}
+    cores = len(os.sched_getaffinity(0))
+    best = None
+    for threads in sorted({1, cores}):
+        with tempfile.TemporaryDirectory() as od:
+            cmd = [binary, "-threads", str(threads), "-sections", "1", "-dstart", str(depth),
+                   "-dmin", str(depth), "-dmax", str(depth), "-nadapt", "-1", "-tmax", "-1",
+                   "-nmax", str(steps), "-xmloutput", ".false.", "-nout", str(steps + 1),
+                   "-output_dir", od]
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=300,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(threads)))
+        if r.returncode != 0:
+            return {"skipped": f"the reference exited with {r.returncode}"}
+        rate = float(re.findall(r"Cell update throughput:\s+([0-9.]+)\s+M/s", r.stdout)[-1]) * 1e6
+        if best is None or rate > best["value"]:
+            best = {"value": rate, "unit": "cell-updates/s", "cores": threads, "kind": "reference",
+                    "sample": f"{steps} steps of the same {2 ** (depth + 1)}-cell mesh and state; "
+                              "the faster of 1 and all host threads; the reference's own "
+                              "'Cell update throughput' (SWE.f90:474)"}
+    return best
 
 
 def adjoint(args, torch):
@@ -450,6 +570,8 @@ def adjoint(args, torch):
                         "gradients_finite": finite}
         if mode == "sqrt":
             g_sqrt = (e0g.grad.clone(), bg.grad.clone())
+            fd_check = finite_differences(prob, e0, bb, eta, b, target, g_sqrt, mesh, src,
+                                          n_dirs=args.fd_dirs)
         else:
             sweeps["gradients_equal_across_modes"] = bool(
                 torch.equal(g_sqrt[0], e0g.grad) and torch.equal(g_sqrt[1], bg.grad))
@@ -493,12 +615,55 @@ def adjoint(args, torch):
         "cells": mesh.n, "steps": N,
         "ratio": sweeps["sqrt"]["ratio"],
         "sweeps": sweeps,
+        "finite_differences": fd_check,
         "ms_forward_launch": times["fwd"], "ms_adjoint_launch": times["adj"],
         "kernel_ratio": times["adj"] / times["fwd"],
         "roofline_adjoint": {"bound": "hbm", "achieved": adj_bytes / 1e9, "peak": HBM_PEAK / 1e9,
                              "unit": "GB/s", "frac": adj_bytes / HBM_PEAK,
                              "bytes_per_cell_step": 96},
     }
+
+
+def finite_differences(prob, e0, bb, eta, b, target, grads, mesh, src, n_dirs=20):
+    """Central finite differences of the gauge misfit along seeded random
+    directions (alternately in eta0 and b) against the adjoint gradient.
+    Directions: standard normal values on the cells more than 50 m deep,
+    windowed by a 300 km Gaussian around the source and scaled to a largest
+    entry of 1; steps of 1 mm (eta0) and 0.5 m (b).  rel_terms divides the
+    discrepancy by sum |grad_i dir_i|, the size of the terms the directional
+    derivative sums (a random direction's derivative can cancel far below
+    them); rel_value divides by |fd|."""
+    import torch
+
+    cen = mesh.centroids() * 1.0e6
+    win = np.exp(-((cen[:, 0] - src["x0"]) ** 2 + (cen[:, 1] - src["y0"]) ** 2) / 3.0e5 ** 2)
+    mask = torch.tensor(((eta - b) > 50.0) * win, device=e0.device)
+    rng = np.random.default_rng(20261203)
+
+    def loss(e, bt):
+        with torch.no_grad():
+            return float(((prob.gauge_series(e, bt) - target) ** 2).sum())
+
+    rows = []
+    for k in range(n_dirs):
+        field = k % 2
+        dirn = torch.tensor(rng.standard_normal(mesh.n), device=e0.device) * mask
+        dirn = dirn / dirn.abs().max()
+        eps = 1e-3 if field == 0 else 0.5
+        base = [e0.clone(), bb.clone()]
+        p_, m_ = [x.clone() for x in base], [x.clone() for x in base]
+        p_[field] += eps * dirn
+        m_[field] -= eps * dirn
+        fd = (loss(*p_) - loss(*m_)) / (2 * eps)
+        ad = float((grads[field] * dirn).sum())
+        terms = float((grads[field] * dirn).abs().sum())
+        rows.append({"field": "eta0" if field == 0 else "b", "adjoint": ad, "fd": fd,
+                     "rel_terms": abs(ad - fd) / terms, "rel_value": abs(ad - fd) / abs(fd)})
+    return {"directions": n_dirs, "steps": {"eta0_m": 1e-3, "b_m": 0.5},
+            "max_rel_terms": max(r["rel_terms"] for r in rows),
+            "max_rel_value": max(r["rel_value"] for r in rows),
+            "median_rel_value": float(np.median([r["rel_value"] for r in rows])),
+            "per_direction": rows}
 
 
 def inversion(args, torch):
@@ -523,7 +688,7 @@ def inversion(args, torch):
         b.append(bb)
         gauges.append(gauges_near(m, p["x0"], p["y0"], 1.5e5, 32, seed=s))
         dt0.append(host.initial_dt(m, (eta, hu, hv, bb), 0.45))
-    inv = SourceInversion(ms, b, gauges, N, dt0)
+    inv = SourceInversion(ms, b, gauges, N, dt0, hbm_budget=args.inv_hbm_gib * 2.0 ** 30)
     theta_true = torch.tensor(truth, dtype=torch.float64, device=dev)
     inv.set_target(theta_true)
     scale = torch.tensor(SCALE, dtype=torch.float64, device=dev)
@@ -545,14 +710,19 @@ def inversion(args, torch):
     with torch.no_grad():
         e_rec, e_true = inv.eta0(theta), inv.eta0(theta_true)
         src_err = float((e_rec - e_true).norm() / e_true.norm())
-    cells = int(inv.problem.n)
-    store_all = inv.problem.store_all
+    cells = int(inv.problem.offsets[-1])
+    plan = inv.problem.plan()
     inv.close()
     return {
         "workload": f"tsunami_source_inversions_{n_inst}_uniform_d16-20_{N}steps_32gauges",
         "instances": n_inst, "cells": cells, "steps": N, "lbfgs_iterations": args.inv_iters,
         "depths": {str(d): [p["depth"] for p in draws].count(d) for d in sorted(meshes)},
-        "function_evaluations": evals[0], "seconds": wall, "store_all": store_all,
+        "function_evaluations": evals[0], "seconds": wall,
+        "checkpoint_pools": {
+            "hbm_budget_gib": args.inv_hbm_gib,
+            "rounds": [[{"store_all": g["store_all"], "K": g["K"], "instances": g["instances"],
+                         "tape_gib": g["tape_bytes"] / 2.0 ** 30} for g in r]
+                       for r in plan["rounds"]]},
         "loss_initial_max": float(hist[0].max()), "loss_final_max": float(f.max()),
         "param_rel_error_max": float(rel.max()), "param_rel_error_median": float(np.median(rel)),
         "source_rel_l2_error": src_err,
@@ -583,6 +753,9 @@ def main():
         return
     if args.workload == "inversion":
         print(json.dumps({"metric": METRIC, "inversion": inversion(args, torch)}), flush=True)
+        return
+    if args.workload == "config1":
+        print(json.dumps({"metric": METRIC, "config1": config1(args, torch)}), flush=True)
         return
 
     if args.workload == "frozen":
@@ -635,6 +808,20 @@ def main():
         frozen, mesh, state, dt0 = tsunami_frozen(args, torch)
         out["frozen_uniform"] = frozen
         out["ensemble"] = ensemble(args, torch)
+        ad_tol = tsunami_adaptive(args, torch, swe.ARITH_TOLERANCED)
+        for k in ("depth0", "state0", "dt0"):
+            ad_tol.pop(k)
+        ens_tol = ensemble(args, torch, swe.ARITH_TOLERANCED)
+        out["toleranced"] = {
+            "note": "SWE_ARITH_TOLERANCED: within 1e-12 of the exact path per step "
+                    "(tests/test_gpu_toleranced.py); the exact path is `value`",
+            "adaptive": {k: ad_tol[k] for k in ("value", "ms_per_step", "step_kernel_bytes_per_s",
+                                                "remesh_and_host_ms_per_step", "cells_min",
+                                                "cells_max")},
+            "frozen_uniform": frozen["toleranced"],
+            "ensemble": {k: ens_tol[k] for k in ("value", "ms_per_step", "roofline", "tuning")},
+        }
+        out["config1"] = config1(args, torch)
         out["adjoint"] = adjoint(args, torch)
         out["inversion"] = inversion(args, torch)
     if not args.no_cpu:

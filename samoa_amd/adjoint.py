@@ -22,10 +22,11 @@ gb += (dF/db)^T lam^{n+1}.  The forward sweep is taped in one of two ways
 needs 50 MB per step, so 200 steps take 10 GB of the 288 GB), all of them
 are kept and the backward sweep runs the adjoint kernels only; otherwise
 memory is bounded by two-level checkpointing: the forward pass keeps every
-K-th state (K = ceil(sqrt(n_steps)), so about 2 sqrt(n_steps) states are
+K-th state (K = ceil(sqrt(n_steps)), so about 3 sqrt(n_steps) states are
 alive at once) and the backward pass recomputes one segment of K states at a
 time, recording the limiter masks, then runs the adjoint through it
-backwards.  The recompute costs one extra forward sweep.
+backwards.  The recompute costs one extra forward sweep; it runs on a second
+stream, one segment ahead of the adjoint, so the two share the GPU.
 
 `PooledGaugeInversion` runs independent instances (one mesh each) with a
 checkpoint pool per instance (plan_pools): store-all or sqrt(N) chosen per
@@ -90,7 +91,7 @@ class GaugeInversion:
         step (1 B/cell) are kept, so the backward sweep runs the adjoint
         kernels only.  Otherwise every K-th state is kept and the backward
         sweep recomputes one segment of K states at a time (two-level
-        checkpointing, peak about (n_steps / K + K) states).
+        checkpointing, peak about (n_steps / K + 2 K) states).
 
         checkpoint_every: "all", an int K, or "auto": store everything when
         it fits in hbm_budget bytes (default: half of the free device memory),
@@ -109,7 +110,7 @@ class GaugeInversion:
         state = 24.0 * n_cells
         if n_steps * (state + n_cells) <= hbm_budget:
             return True, 1
-        if (math.ceil(n_steps / K_sqrt) + K_sqrt + 2) * (state + n_cells) > hbm_budget:
+        if tape_bytes(n_cells, n_steps, False, K_sqrt) > hbm_budget:
             raise MemoryError(f"{n_cells} cells x {n_steps} steps do not fit a "
                               f"{hbm_budget / 2**30:.1f} GiB checkpoint budget")
         return False, K_sqrt
@@ -134,12 +135,12 @@ class GaugeInversion:
         if t.dtype != torch.float64 or t.device != self.device or t.numel() != self.n:
             raise ValueError(f"{name} must be a float64 tensor of {self.n} cells on {self.device}")
 
-    def _fwd(self, q, out, b, step, masks=None, cfl=None):
+    def _fwd(self, q, out, b, step, masks=None, cfl=None, stream=None):
         self.op.forward_device(_p(q[0]), _p(q[1]), _p(q[2]), _p(b), _p(out[0]), _p(out[1]),
                                _p(out[2]), self._dt(step), self.direction(step),
                                dt_cfl=_p(cfl[step]) if cfl is not None else 0,
                                masks=_p(masks) if masks is not None else 0,
-                               stream=self._stream())
+                               stream=stream.cuda_stream if stream is not None else self._stream())
 
     def record_dt(self, eta0: torch.Tensor, b: torch.Tensor, dt0) -> torch.Tensor:
         """Runs the scheme once from (eta0, p = 0, b) with the reference's dt
@@ -212,26 +213,64 @@ class GaugeInversion:
                 if n >= 1:
                     lam[0].index_add_(0, self.gauges, g_series[n - 1])
             return lam[0].clone(), gb
+        # Two-level checkpointing, pipelined: the recompute of segment j - 1
+        # runs on a side stream while the adjoint sweeps segment j on the
+        # current stream, in the other of two segment buffers.  The two
+        # kernels then share the GPU (the adjoint kernel alone holds two
+        # waves per SIMD).  Events order every buffer's reuse.
         checkpoints = tape["checkpoints"]
         starts = sorted(checkpoints)
-        masks = [torch.empty(self.n, dtype=torch.int8, device=self.device) for _ in range(self.K)]
-        states = [[self._empty() for _ in range(3)] for _ in range(self.K)]
-        spare = [self._empty() for _ in range(3)]
-        for j in reversed(range(len(starts))):
+        main = torch.cuda.current_stream(self.device)
+        side = torch.cuda.Stream(device=self.device)
+        side.wait_stream(main)  # the tape and the seeds are written on main
+        bufs = []
+        for _ in range(2):
+            states = [[self._empty() for _ in range(3)] for _ in range(self.K)]
+            masks = [torch.empty(self.n, dtype=torch.int8, device=self.device)
+                     for _ in range(self.K)]
+            spare = [self._empty() for _ in range(3)]
+            for t in [x for st in states for x in st] + masks + spare:
+                t.record_stream(side)
+            bufs.append((states, masks, spare))
+
+        def segment(j):
             s0 = starts[j]
-            s1 = starts[j + 1] if j + 1 < len(starts) else self.n_steps
-            # recompute the segment's states Q^{s0..s1-1} and the masks of its steps
-            for x, y in zip(states[0], checkpoints[s0]):
-                x.copy_(y)
-            for n in range(s0, s1):
-                i = n - s0
-                dst = states[i + 1] if i + 1 < s1 - s0 else spare
-                self._fwd(states[i], dst, b, n, masks=masks[i])
+            return s0, (starts[j + 1] if j + 1 < len(starts) else self.n_steps)
+
+        def recompute(j):
+            """Q^{s0..s1-1} of segment j and the masks of its steps, on side."""
+            states, masks, spare = bufs[j % 2]
+            s0, s1 = segment(j)
+            with torch.cuda.stream(side):
+                for x, y in zip(states[0], checkpoints[s0]):
+                    x.copy_(y)
+                for n in range(s0, s1):
+                    i = n - s0
+                    dst = states[i + 1] if i + 1 < s1 - s0 else spare
+                    self._fwd(states[i], dst, b, n, masks=masks[i], stream=side)
+            ev = torch.cuda.Event()
+            ev.record(side)
+            return ev
+
+        last = len(starts) - 1
+        ready = {last: recompute(last)}
+        freed = [None, None]  # the adjoint that last read each buffer
+        for j in reversed(range(len(starts))):
+            if j >= 1:
+                if freed[(j - 1) % 2] is not None:
+                    side.wait_event(freed[(j - 1) % 2])
+                ready[j - 1] = recompute(j - 1)
+            main.wait_event(ready.pop(j))
+            states, masks, _ = bufs[j % 2]
+            s0, s1 = segment(j)
             for n in reversed(range(s0, s1)):
                 self._adjoint(states[n - s0], masks[n - s0], b, lam, lam_out, gb, n)
                 lam, lam_out = lam_out, lam
                 if n >= 1:
                     lam[0].index_add_(0, self.gauges, g_series[n - 1])
+            freed[j % 2] = torch.cuda.Event()
+            freed[j % 2].record(main)
+        main.wait_stream(side)
         return lam[0].clone(), gb
 
     def close(self):
@@ -244,11 +283,13 @@ TAPE_MASK_BYTES = 1     # limiter masks per cell
 
 def tape_bytes(n_cells: int, n_steps: int, store_all: bool, K: int = 1) -> float:
     """HBM a tape takes: every state and mask (store_all), or the
-    checkpoints plus one recomputed segment of K states and masks."""
+    checkpoints plus two recomputed segments of K states and masks."""
     per = (TAPE_STATE_BYTES + TAPE_MASK_BYTES) * float(n_cells)
     if store_all:
         return n_steps * per
-    return (math.ceil(n_steps / K) + K + 2) * per
+    # the checkpoints, two segment buffers (the backward sweep recomputes one
+    # while it runs the adjoint through the other) and their spares
+    return (math.ceil(n_steps / K) + 2 * K + 4) * per
 
 
 def plan_pools(cells, n_steps: int, hbm_budget: float, checkpoint_every="auto"):

@@ -15,7 +15,7 @@ def test_store_everything_when_it_fits():
 
 def test_sqrt_checkpoints_when_tight():
     n, N = 2 ** 21, 200
-    store_all, K = plan(n, N, "auto", hbm_budget=40 * 25 * n)
+    store_all, K = plan(n, N, "auto", hbm_budget=60 * 25 * n)
     assert not store_all and K == math.ceil(math.sqrt(N)) == 15
 
 
