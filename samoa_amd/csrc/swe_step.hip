@@ -63,10 +63,8 @@ __device__ __forceinline__ int wave_sum_i(int v) {
 // whose nbr holds plain neighbour cells (StepArgs::raw_nbr).  TOL: the
 // toleranced arithmetic of swe_riemann_tol.hpp (StepArgs::tol; the GeoClaw
 // solvers only): the chunk stages each cell's depth, world velocity,
-// bathymetry, sqrt(g h) and 1/h instead of its state, keeps its own state in
-// registers, and solves every edge with the cell of the list entry on the
-// left (the Riemann problem seen from either side is the same, up to
-// rounding), so no operands are swapped.
+// bathymetry and sqrt(g h) instead of its state and keeps its own state in
+// registers.
 template <int SOLVER, int CH, bool JUMPS, bool RAW = false, bool TOL = false>
 __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void euler_step_kernel(
     StepArgs a) {
@@ -243,9 +241,20 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                     bO = cq.b;
                     sqO = cq.sq;
                 }
-                Flux fc, fo;
-                tol::edge_flux<SOLVER>(nx, ny, hC, uC, vC, bC, sqC, hO, uO, vO, bO, sqO, drytol, g,
-                                       inv16g, fc, fo);
+                // the left state as in the exact path: the operands of the
+                // well-balancing sums (b + h of the right cell first) round
+                // with the side they are on, and in deep water those
+                // roundings are the largest part of a momentum update
+                const bool c_left = bnd || (a.forward ? (lc < lo) : (lc > lo));
+                const double sx = c_left ? nx : -nx, sy = c_left ? ny : -ny;
+                Flux fL, fR;
+                tol::edge_flux<SOLVER>(sx, sy, c_left ? hC : hO, c_left ? uC : uO,
+                                       c_left ? vC : vO, c_left ? bC : bO, c_left ? sqC : sqO,
+                                       c_left ? hO : hC, c_left ? uO : uC, c_left ? vO : vC,
+                                       c_left ? bO : bC, c_left ? sqO : sqC, drytol, g, inv16g,
+                                       fL, fR);
+                const Flux fc = c_left ? fL : fR;
+                const Flux fo = c_left ? fR : fL;
                 s_f[0][k][lc] = fc.h;
                 s_f[1][k][lc] = fc.hu;
                 s_f[2][k][lc] = fc.hv;
