@@ -37,6 +37,7 @@ from __future__ import annotations
 
 import argparse
 import gc
+import hashlib
 import json
 import os
 import sys
@@ -137,88 +138,103 @@ def tsunami_adaptive(args, torch, arith=0):
 
     S, courant, tol = 1.0e6, 0.45, args.adapt_tol
     dmin, dmax = 16, 26
-    mesh0 = host.mesh_uniform(dmin)
-    op = swe.EulerTimestep(mesh0, scaling=S, dry_tolerance=1e-3, courant=courant,
-                           min_depth=dmin, max_depth=dmax, solver=swe.SOLVER_AUG_RIEMANN,
-                           boundary=swe.BC_OUTFLOW)
-    op.set_arithmetic(arith)
-    z = np.zeros(mesh0.n)
-    op.upload(z, z, z, z)
-    # the Fortran host's set-up: the initial phase of SWE.f90:230-262 with a
-    # tolerance on the source's variation across a cell, the scenario state,
-    # the bathymetry raster for later splits (swe_driver.f90)
-    t0 = time.perf_counter()
-    n0, dt0 = host.tsunami_adaptive_setup(op, 0, tol, dmax, 1e-3, courant)
-    t_init = time.perf_counter() - t0
-    depth = op.download_mesh()[0]
-    depths0 = np.bincount(depth, minlength=dmax + 1)[dmin:].tolist()
-    dt = dt0
     stream = torch.cuda.Stream(device=torch.device("cuda:0"))
-    op.set_stream(stream.cuda_stream)
 
-    state0 = op.download()
-    t, fwd = 0.0, False
-    step_ev = []
-    # every step leaves flags from the elevation jumps of the state it reads
-    # (a jump to a neighbour's centroid spans about half a corner range)
-    op.keep_eta_jump_flags(tol / 2.0, tol / 16.0)
+    def phase(per_step_events):
+        """Set-up, warm-up and the timed steps from the same initial state.
+        The throughput pass has nothing in the loop but the steps and the
+        remeshing; the second pass records events around every step launch
+        to time the step kernels (each event is a packet in the queue that
+        adds a few microseconds between the kernels)."""
+        mesh0 = host.mesh_uniform(dmin)
+        op = swe.EulerTimestep(mesh0, scaling=S, dry_tolerance=1e-3, courant=courant,
+                               min_depth=dmin, max_depth=dmax, solver=swe.SOLVER_AUG_RIEMANN,
+                               boundary=swe.BC_OUTFLOW)
+        op.set_arithmetic(arith)
+        z = np.zeros(mesh0.n)
+        op.upload(z, z, z, z)
+        # the Fortran host's set-up: the initial phase of SWE.f90:230-262 with
+        # a tolerance on the source's variation across a cell, the scenario
+        # state, the bathymetry raster for later splits (swe_driver.f90)
+        t0 = time.perf_counter()
+        n0, dt0 = host.tsunami_adaptive_setup(op, 0, tol, dmax, 1e-3, courant)
+        t_init = time.perf_counter() - t0
+        depth = op.download_mesh()[0]
+        op.set_stream(stream.cuda_stream)
+        state0 = op.download()
+        # every step leaves flags from the elevation jumps of the state it
+        # reads (a jump to a neighbour's centroid spans about half a corner
+        # range)
+        op.keep_eta_jump_flags(tol / 2.0, tol / 16.0)
+        # the per-step events exist before the timed loop: creating them
+        # there would put the harness's own allocations on the host's
+        # critical path
+        events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+                  for _ in range(args.steps if per_step_events else 0)]
+        step_ev = []
+        t, dt, fwd = 0.0, dt0, False
 
-    # the per-step events exist before the timed loop: creating them there
-    # would put the harness's own allocations on the host's critical path
-    events = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-              for _ in range(args.steps)]
+        def one_step(k, timed):
+            nonlocal t, dt, fwd
+            if (k + 1) % args.nadapt == 0:  # time step k + 1 of SWE.f90:385-390
+                if k > 0:  # before the first step the flags are all 0: nothing changes
+                    op.adapt()  # the flags the last step left
+                fwd = not fwd  # the adaption traversal flips the direction
+            n = op.n
+            if timed and per_step_events:
+                ev = events[len(step_ev)]
+                ev[0].record(stream)
+            # after the first step the dt chain stays on the device and
+            # nothing waits for a step
+            if k == 0:
+                t, dt = op.run(1, dt, fwd, time=t)
+            else:
+                op.run(1, None, fwd, wait=False)
+            if timed and per_step_events:
+                ev[1].record(stream)
+                step_ev.append((ev, n))
+            fwd = not fwd
+            return n
 
-    def one_step(k, timed):
-        nonlocal t, dt, fwd
-        if (k + 1) % args.nadapt == 0:  # time step k + 1 of SWE.f90:385-390
-            if k > 0:  # before the first step the flags are all 0: nothing changes
-                op.adapt()  # the flags the last step left
-            fwd = not fwd  # the adaption traversal flips the direction
-        n = op.n
-        if timed:
-            ev = events[len(step_ev)]
-            ev[0].record(stream)
-        # after the first step the dt chain stays on the device and nothing
-        # waits for a step
-        if k == 0:
-            t, dt = op.run(1, dt, fwd, time=t)
-        else:
-            op.run(1, None, fwd, wait=False)
-        if timed:
-            ev[1].record(stream)
-            step_ev.append((ev, n))
-        fwd = not fwd
-        return n
+        for k in range(args.warmup):
+            one_step(k, False)
+        torch.cuda.synchronize()
+        cells = []
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        # Python's cycle collector would pause the host loop at random steps;
+        # the Fortran host that drives production runs has no such pauses
+        gc_was = gc.isenabled()
+        gc.disable()
+        w0 = time.perf_counter()
+        start.record(stream)
+        for k in range(args.warmup, args.warmup + args.steps):
+            cells.append(one_step(k, True))
+        end.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - w0
+        if gc_was:
+            gc.enable()
+        t, dt = op.run(0, None, fwd)  # read the device chain
+        ms = start.elapsed_time(end)
+        eta, hu, hv, b = op.download()
+        if not (np.isfinite(eta).all() and np.isfinite(hu).all()):
+            raise RuntimeError("non-finite state after the adaptive steps")
+        d_end = op.download_mesh()[0]
+        op.close()
+        return dict(n0=n0, dt0=dt0, t_init=t_init, depth=depth, state0=state0, cells=cells,
+                    ms=ms, wall=wall, t=t, d_end=d_end, step_ev=step_ev,
+                    digest=hashlib.sha256(eta.tobytes() + hu.tobytes()).hexdigest()[:16])
 
-    for k in range(args.warmup):
-        one_step(k, False)
-    torch.cuda.synchronize()
-    cells = []
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    # Python's cycle collector would pause the host loop at random steps;
-    # the Fortran host that drives production runs has no such pauses
-    gc_was = gc.isenabled()
-    gc.disable()
-    w0 = time.perf_counter()
-    start.record(stream)
-    for k in range(args.warmup, args.warmup + args.steps):
-        cells.append(one_step(k, True))
-    end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - w0
-    if gc_was:
-        gc.enable()
-    t, dt = op.run(0, None, fwd)  # read the device chain
-    ms = start.elapsed_time(end)
-    step_ms = np.array([a.elapsed_time(b) for (a, b), _ in step_ev])
-    step_cells = np.array([n for _, n in step_ev], dtype=np.float64)
-    eta, hu, hv, b = op.download()
-    if not (np.isfinite(eta).all() and np.isfinite(hu).all()):
-        raise RuntimeError("non-finite state after the adaptive steps")
-    d_end = op.download_mesh()[0]
-    op.close()
+    run = phase(False)
+    prof = phase(True)
+    if prof["cells"] != run["cells"] or prof["digest"] != run["digest"]:
+        raise RuntimeError("the two passes of the adaptive run differ")
+    step_ms = np.array([a.elapsed_time(b) for (a, b), _ in prof["step_ev"]])
+    step_cells = np.array([n for _, n in prof["step_ev"]], dtype=np.float64)
+    cells, d_end = run["cells"], run["d_end"]
     updates = float(np.sum(cells))
-    elapsed = max(ms * 1e-3, wall)
+    elapsed = max(run["ms"] * 1e-3, run["wall"])
+    elapsed_prof = max(prof["ms"] * 1e-3, prof["wall"])
     return {
         "value": updates / elapsed,
         "ms_per_step": 1e3 * elapsed / args.steps,
@@ -226,14 +242,19 @@ def tsunami_adaptive(args, torch, arith=0):
         "step_kernel_ms_total": float(step_ms.sum()),
         "step_kernel_bytes_per_s": float(BYTES_PER_CELL * step_cells.sum() / (step_ms.sum() * 1e-3)),
         "remesh_and_host_ms_per_step": float((elapsed * 1e3 - step_ms.sum()) / args.steps),
-        "elapsed_gpu_s": ms * 1e-3, "elapsed_wall_s": wall,
-        "cells_initial": n0, "depths_initial_16_to_26": depths0,
+        "elapsed_gpu_s": run["ms"] * 1e-3, "elapsed_wall_s": run["wall"],
+        "ms_per_step_with_step_events": 1e3 * elapsed_prof / args.steps,
+        "timing": "throughput from a pass with nothing but the steps and the remeshing in the "
+                  "loop; step kernel times from a second pass of the same steps with events "
+                  "around every step launch (same cells and final state)",
+        "cells_initial": run["n0"], "depths_initial_16_to_26":
+            np.bincount(run["depth"], minlength=dmax + 1)[dmin:].tolist(),
         "cells_final": int(len(d_end)),
         "depths_final_16_to_26": np.bincount(d_end, minlength=dmax + 1)[dmin:].tolist(),
         "cells_min": int(min(cells)), "cells_max": int(max(cells)),
-        "initial_refinement_s": t_init,
-        "nadapt": args.nadapt, "adapt_tol_m": tol, "simulated_time_s": t,
-        "depth0": depth, "state0": state0, "dt0": dt0,
+        "initial_refinement_s": run["t_init"],
+        "nadapt": args.nadapt, "adapt_tol_m": tol, "simulated_time_s": run["t"],
+        "depth0": run["depth"], "state0": run["state0"], "dt0": run["dt0"],
     }
 
 
@@ -800,7 +821,10 @@ def main():
             "kernel": "swe::euler_step_kernel<0, 256, true> + swe::finalize_kernel (one step)",
             "valu_issue_frac_profiled": valu,
             "bytes_per_cell_update": BYTES_PER_CELL,
-            "timing": "HIP events around every step launch of the adaptive run",
+            "timing": "HIP events around every step launch of a second pass of the same steps",
+            "traffic_source": "HBM bytes per cell of the rocprofv3 FETCH_SIZE and WRITE_SIZE "
+                              "passes in profiles/traffic.json (not counted in this run) "
+                              "times this run's mean cells per step",
         },
         "adaptive": ad,
     }
