@@ -753,6 +753,14 @@ def inversion(args, torch):
 import numpy as np  # noqa: E402
 
 
+_T0 = time.perf_counter()
+
+
+def progress(what):
+    """One line on stderr per finished block, so that a long run shows progress."""
+    print(f"bench.py: {what} done at {time.perf_counter() - _T0:.0f} s", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -787,6 +795,7 @@ def main():
         return
 
     ad = tsunami_adaptive(args, torch)
+    progress("adaptive tsunami")
     achieved = ad["step_kernel_bytes_per_s"]
     traffic, valu = None, None
     if os.path.exists(PROFILE_TRAFFIC):
@@ -831,11 +840,15 @@ def main():
     if args.workload == "all":
         frozen, mesh, state, dt0 = tsunami_frozen(args, torch)
         out["frozen_uniform"] = frozen
+        progress("frozen uniform tsunami")
         out["ensemble"] = ensemble(args, torch)
+        progress("ensemble")
         ad_tol = tsunami_adaptive(args, torch, swe.ARITH_TOLERANCED)
+        progress("adaptive tsunami, toleranced")
         for k in ("depth0", "state0", "dt0"):
             ad_tol.pop(k)
         ens_tol = ensemble(args, torch, swe.ARITH_TOLERANCED)
+        progress("ensemble, toleranced")
         out["toleranced"] = {
             "note": "SWE_ARITH_TOLERANCED: within 1e-12 of the exact path per step "
                     "(tests/test_gpu_toleranced.py); the exact path is `value`",
@@ -846,8 +859,11 @@ def main():
             "ensemble": {k: ens_tol[k] for k in ("value", "ms_per_step", "roofline", "tuning")},
         }
         out["config1"] = config1(args, torch)
+        progress("config 1")
         out["adjoint"] = adjoint(args, torch)
+        progress("adjoint")
         out["inversion"] = inversion(args, torch)
+        progress("inversions")
     if not args.no_cpu:
         from samoa_amd import host
 
