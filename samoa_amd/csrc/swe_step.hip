@@ -77,6 +77,9 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
     __shared__ int32_t s_nbr[3][CH];     // packed neighbour (o << 2 | back edge)
     __shared__ int8_t s_plot[CH];
     __shared__ uint16_t s_list[3 * CH];  // (local cell << 2) | edge
+    // JUMPS (exact arithmetic): elevation-jump bits per (edge, cell), from the
+    // edge loop, which holds both states
+    __shared__ uint8_t s_jb[JUMPS && !TOL ? 3 : 1][JUMPS && !TOL ? CH : 1];
     __shared__ int s_wsum[NW];
     __shared__ double s_wmin[NW];
     __shared__ int s_wref[NW];
@@ -319,12 +322,25 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
             s_f[1][k][lc] = fc.hu;
             s_f[2][k][lc] = fc.hv;
             s_f[3][k][lc] = fc.s;
+            uint8_t jb = 0;
+            if constexpr (JUMPS && !TOL) {
+                // the elevation jump across an edge between two wet cells:
+                // bit 0 if above the refinement tolerance, bit 1 if not below
+                // the coarsening tolerance (a NaN jump sets neither); the
+                // same for both cells, as |a - b| = |b - a|
+                if (!bnd && qe - qb > drytol && oe - ob > drytol) {
+                    const double jd = fabs(qe - oe);
+                    jb = (uint8_t)((jd > a.jump_refine ? 1 : 0) | (jd >= a.jump_coarsen ? 2 : 0));
+                }
+                s_jb[k][lc] = jb;
+            }
             if (inside) {
                 const Flux fo = c_left ? fR : fL;
                 s_f[0][ko][lo] = fo.h;
                 s_f[1][ko][lo] = fo.hu;
                 s_f[2][ko][lo] = fo.hv;
                 s_f[3][ko][lo] = fo.s;
+                if constexpr (JUMPS && !TOL) s_jb[ko][lo] = jb;
             }
         }
         __syncthreads();
@@ -386,35 +402,12 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                 }
             } else if (JUMPS) {
                 // elevation jumps to the wet neighbours of a wet cell, in
-                // the state the step reads: bit 0 if one is above the
-                // refinement tolerance, bit 1 if one is not below the
-                // coarsening tolerance (a NaN jump sets neither, as fmax
-                // ignores it).  j = the largest jump (0 without a wet
-                // neighbour): j > tol if some edge's is, j < tol if every
-                // edge's is and tol > 0.
-                int jb = 0;
-                const double qe = s_q[0][t];
-                if (qe - s_q[3][t] > drytol) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const int32_t p = s_nbr[k][t];
-                        if (p < 0) continue;
-                        const int32_t o = RAW ? p : p >> 2;
-                        const int lo = o - lbase;
-                        double oe, ob;
-                        if ((uint32_t)lo < (uint32_t)cnt) {
-                            oe = s_q[0][lo];
-                            ob = s_q[3][lo];
-                        } else {
-                            oe = gs_eta[o];
-                            ob = gs_b[o];
-                        }
-                        if (oe - ob > drytol) {
-                            const double jd = fabs(qe - oe);
-                            jb |= (jd > a.jump_refine ? 1 : 0) | (jd >= a.jump_coarsen ? 2 : 0);
-                        }
-                    }
-                }
+                // the state the step reads, from the edge loop's bits: bit 0
+                // if one is above the refinement tolerance, bit 1 if one is
+                // not below the coarsening tolerance.  j = the largest jump
+                // (0 without a wet neighbour): j > tol if some edge's is,
+                // j < tol if every edge's is and tol > 0.
+                const int jb = s_jb[0][t] | s_jb[1][t] | s_jb[2][t];
                 if (d < sc.dmax && (jb & 1)) {
                     rf = 1;
                     refined = 1;
