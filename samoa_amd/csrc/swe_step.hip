@@ -77,9 +77,9 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
     __shared__ int32_t s_nbr[3][CH];     // packed neighbour (o << 2 | back edge)
     __shared__ int8_t s_plot[CH];
     __shared__ uint16_t s_list[3 * CH];  // (local cell << 2) | edge
-    // JUMPS (exact arithmetic): elevation-jump bits per (edge, cell), from the
-    // edge loop, which holds both states
-    __shared__ uint8_t s_jb[JUMPS && !TOL ? 3 : 1][JUMPS && !TOL ? CH : 1];
+    // JUMPS: elevation-jump bits per (edge, cell), from the edge loop, which
+    // holds both states
+    __shared__ uint8_t s_jb[JUMPS ? 3 : 1][JUMPS ? CH : 1];
     __shared__ int s_wsum[NW];
     __shared__ double s_wmin[NW];
     __shared__ int s_wref[NW];
@@ -262,12 +262,23 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                 s_f[1][k][lc] = fc.hu;
                 s_f[2][k][lc] = fc.hv;
                 s_f[3][k][lc] = fc.s;
+                uint8_t jb = 0;
+                if constexpr (JUMPS) {
+                    // as in the exact path below, with eta = h + b
+                    if (!bnd && hC > drytol && hO > drytol) {
+                        const double jd = fabs((hC + bC) - (hO + bO));
+                        jb = (uint8_t)((jd > a.jump_refine ? 1 : 0) |
+                                       (jd >= a.jump_coarsen ? 2 : 0));
+                    }
+                    s_jb[k][lc] = jb;
+                }
                 if (inside) {
                     const int ko = p & 3;
                     s_f[0][ko][lo] = fo.h;
                     s_f[1][ko][lo] = fo.hu;
                     s_f[2][ko][lo] = fo.hv;
                     s_f[3][ko][lo] = fo.s;
+                    if constexpr (JUMPS) s_jb[ko][lo] = jb;
                 }
                 continue;
             }
@@ -323,7 +334,7 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
             s_f[2][k][lc] = fc.hv;
             s_f[3][k][lc] = fc.s;
             uint8_t jb = 0;
-            if constexpr (JUMPS && !TOL) {
+            if constexpr (JUMPS) {
                 // the elevation jump across an edge between two wet cells:
                 // bit 0 if above the refinement tolerance, bit 1 if not below
                 // the coarsening tolerance (a NaN jump sets neither); the
@@ -340,7 +351,7 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                 s_f[1][ko][lo] = fo.hu;
                 s_f[2][ko][lo] = fo.hv;
                 s_f[3][ko][lo] = fo.s;
-                if constexpr (JUMPS && !TOL) s_jb[ko][lo] = jb;
+                if constexpr (JUMPS) s_jb[ko][lo] = jb;
             }
         }
         __syncthreads();
@@ -369,38 +380,7 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
             sv = sv + lleg * s_f[3][k2][t];
 
             int8_t rf = 0;
-            if constexpr (JUMPS && TOL) {
-                // as below, from the staged depths: eta = h + b
-                int jb = 0;
-                const double qe = my_e;
-                if (s_q[0][t] > drytol) {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const int32_t p = s_nbr[k][t];
-                        if (p < 0) continue;
-                        const int32_t o = p >> 2;
-                        const int lo = o - lbase;
-                        double oe, oh;
-                        if ((uint32_t)lo < (uint32_t)cnt) {
-                            oh = s_q[0][lo];
-                            oe = oh + s_q[3][lo];
-                        } else {
-                            oe = gs_eta[o];
-                            oh = oe - gs_b[o];
-                        }
-                        if (oh > drytol) {
-                            const double jd = fabs(qe - oe);
-                            jb |= (jd > a.jump_refine ? 1 : 0) | (jd >= a.jump_coarsen ? 2 : 0);
-                        }
-                    }
-                }
-                if (d < sc.dmax && (jb & 1)) {
-                    rf = 1;
-                    refined = 1;
-                } else if (d > sc.dmin && !(jb & 2) && a.jump_coarsen > 0.0) {
-                    rf = -1;
-                }
-            } else if (JUMPS) {
+            if constexpr (JUMPS) {
                 // elevation jumps to the wet neighbours of a wet cell, in
                 // the state the step reads, from the edge loop's bits: bit 0
                 // if one is above the refinement tolerance, bit 1 if one is

@@ -164,3 +164,35 @@ def test_batch_modes_on_the_same_meshes():
     for x, y in zip(out[swe.ARITH_TOLERANCED][0], out[swe.ARITH_EXACT][0]):
         errs = field_err(x, y)
         assert max(errs) <= 1e-12, errs
+
+
+def test_elevation_jump_flags_both_modes():
+    """The elevation-jump refinement flags a step leaves (swe_keep_eta_jump_
+    flags), toleranced against exact on the same state: the toleranced
+    kernel takes each wet edge's jump as |(h + b) - (h' + b')|, so only
+    cells whose largest jump sits within rounding of a tolerance may differ;
+    the exact kernel's flags equal the numpy restatement of the criterion."""
+    from test_gpu_remesh import eta_jump_flags
+    d = 12
+    mesh = host.mesh_uniform(d)
+    st = np.stack(host.tsunami(mesh, seed=5, drytol=1e-3))
+    st[0] *= 50.0
+    jumps = np.abs(st[0][:, None] - st[0][np.maximum(mesh.nbr, 0)])
+    rtol = float(np.quantile(jumps[jumps > 0], 0.9))
+    dt = host.initial_dt(mesh, st, 0.45) * 1.0e6
+    flags = {}
+    for mode in (swe.ARITH_EXACT, swe.ARITH_TOLERANCED):
+        op = swe.EulerTimestep(mesh, scaling=1.0e6, dry_tolerance=1e-3, min_depth=d - 2,
+                               max_depth=d + 2)
+        op.set_arithmetic(mode)
+        op.upload(*st)
+        op.keep_eta_jump_flags(rtol, rtol / 20)
+        flags[mode] = op.traverse(dt, True, flags=True).refine
+        op.close()
+    want = eta_jump_flags(mesh, st, 1e-3, rtol, rtol / 20, d - 2, d + 2)
+    flips = int((flags[swe.ARITH_TOLERANCED] != flags[swe.ARITH_EXACT]).sum())
+    print(f"case: jump flags d={d}: refine {int((want == 1).sum())} coarsen "
+          f"{int((want == -1).sum())} toleranced mismatches {flips}", flush=True)
+    assert np.array_equal(flags[swe.ARITH_EXACT], want)
+    assert (want == 1).sum() > 0 and (want == -1).sum() > 0
+    assert flips <= mesh.n // 1000
