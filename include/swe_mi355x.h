@@ -307,12 +307,13 @@ int swe_get_tuning(swe_handle *h, int *chunk_cells, int64_t *grid_blocks);
  * build with -ffp-contract=off rounds it; results are bit-identical to it.
  * SWE_ARITH_TOLERANCED: the GeoClaw solvers (SWE_SOLVER_AUG_RIEMANN,
  * SWE_SOLVER_FWAVE) share reciprocals of repeated denominators, take
- * hardware reciprocal and square-root estimates refined by Newton steps,
- * contract products into fused multiply-adds and solve each edge from
- * per-cell velocities; one step stays within a few ulps of the terms it
- * sums (tested against the exact path at 1e-12 of the largest field value
- * per step and 1e-9 after 1000 steps), and refinement flags can differ where
- * a criterion sits within rounding of its threshold.  The Lax-Friedrichs
+ * hardware reciprocal and square-root estimates refined by Newton steps and
+ * solve each edge from per-cell depths and velocities, with the exact path's
+ * left and right sides and no fused multiply-adds; one step stays within a
+ * few ulps of the terms it sums (tested against the exact path at 1e-12 of
+ * the largest field value per step and 1e-9 after 1000 steps), and
+ * refinement flags can differ where a criterion sits within rounding of its
+ * threshold.  The Lax-Friedrichs
  * solvers and the device-pointer sweeps of the adjoint (swe_forward_device,
  * swe_batch_forward_device) always round exactly.  Applies to swe_step,
  * swe_run and swe_step_device (and their batch forms) from the next call. */
