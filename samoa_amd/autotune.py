@@ -8,8 +8,8 @@ few repetitions).  The state is saved before the trials and restored before
 every candidate, and every candidate's runs start the dt chain at the same
 dt0, so each launch shape is timed on the same waves; after the trials the
 state is restored once more, so tuning does not advance the simulation.
-Results are cached per device and mesh signature in
-~/.cache/samoa_amd/autotune.json.
+Results are cached per device, mesh sizes, solver and arithmetic in
+~/.cache/samoa_amd/autotune.json; the caller's stream is restored.
 """
 from __future__ import annotations
 
@@ -38,6 +38,18 @@ class _Single:
     def upload(self, s, *st):
         self.op.upload(*st)
 
+    @property
+    def stream(self):
+        return self.op.stream
+
+    @property
+    def solver(self):
+        return self.op.solver
+
+    @property
+    def arithmetic(self):
+        return self.op.arithmetic
+
     def set_stream(self, stream):
         self.op.set_stream(stream)
 
@@ -49,9 +61,12 @@ class _Single:
 
 
 def signature(batch) -> str:
+    """The meshes' sizes, the solver and the arithmetic: the best launch
+    shape depends on all three."""
     sizes = sorted(m.n for m in batch.meshes)
     h = hashlib.sha1(json.dumps(sizes).encode()).hexdigest()[:16]
-    return f"n{batch.n_scen}_c{batch.cells}_{h}"
+    return (f"n{batch.n_scen}_c{batch.cells}_{h}_s{getattr(batch, 'solver', 0)}"
+            f"_a{getattr(batch, 'arithmetic', 0)}")
 
 
 def candidates(n_cus: int):
@@ -88,6 +103,7 @@ def tune(batch, dt0, steps: int = 20, repeats: int = 3, use_cache: bool = True,
             return best
 
     saved = [batch.download(s) for s in range(batch.n_scen)]
+    caller_stream = getattr(batch, "stream", 0)
     stream = torch.cuda.Stream(device=dev)
     batch.set_stream(stream.cuda_stream)
     timings = []
@@ -115,7 +131,7 @@ def tune(batch, dt0, steps: int = 20, repeats: int = 3, use_cache: bool = True,
     finally:
         for s, st in enumerate(saved):
             batch.upload(s, *st)
-        batch.set_stream(0)
+        batch.set_stream(caller_stream)
     best = min(timings, key=lambda r: r["ms_per_step"])
     result = {"chunk": best["chunk"], "grid": best["grid"], "ms_per_step": best["ms_per_step"],
               "trials": timings}

@@ -29,7 +29,7 @@ SOLVER_LLF_BATH = 5
 BC_OUTFLOW = 0
 BC_WALL = 1
 ARITH_EXACT = 0        # every operation rounds as the reference's
-ARITH_TOLERANCED = 1   # shared reciprocals, refined estimates, fused multiply-adds
+ARITH_TOLERANCED = 1   # shared reciprocals, refined estimates, within 1e-12 per step
 G = 9.80665  # SWE_data_types.f90:34
 
 
@@ -89,6 +89,9 @@ class EulerTimestep:
         self._h = h
         self.n = mesh.n
         self.courant = float(courant)
+        self.solver = int(solver)
+        self.stream = 0  # the caller's HIP stream (0: the library's own)
+        self.arithmetic = ARITH_EXACT
 
     # -- state -----------------------------------------------------------
     def upload(self, eta, hu, hv, b) -> None:
@@ -216,10 +219,12 @@ class EulerTimestep:
     def set_stream(self, stream: int) -> None:
         """Launch on a caller's HIP stream (0 = the library's own)."""
         _native.check("swe_set_stream", self._lib.swe_set_stream(self._h, stream or None))
+        self.stream = stream or 0
 
     def set_arithmetic(self, mode: int) -> None:
         """ARITH_EXACT (bit-identical to the reference) or ARITH_TOLERANCED."""
         _native.check("swe_set_arithmetic", self._lib.swe_set_arithmetic(self._h, int(mode)))
+        self.arithmetic = int(mode)
 
     def set_tuning(self, chunk: int, grid: int = 0) -> None:
         _native.check("swe_set_tuning", self._lib.swe_set_tuning(self._h, chunk, grid))
@@ -285,6 +290,9 @@ class Batch:
         self.n_scen = len(meshes)
         self.courant = np.array([float(p.get("courant", 0.45)) for p in params])
         self.cells = int(self._lib.swe_batch_cells(h))
+        self.solver = int(params[0].get("solver", SOLVER_AUG_RIEMANN))
+        self.stream = 0  # the caller's HIP stream (0: the library's own)
+        self.arithmetic = ARITH_EXACT
 
     def upload(self, s: int, eta, hu, hv, b) -> None:
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (eta, hu, hv, b)]
@@ -339,6 +347,7 @@ class Batch:
         """ARITH_EXACT or ARITH_TOLERANCED for every scenario."""
         _native.check("swe_batch_set_arithmetic",
                       self._lib.swe_batch_set_arithmetic(self._h, int(mode)))
+        self.arithmetic = int(mode)
 
     def set_tuning(self, chunk: int, grid: int = 0) -> None:
         _native.check("swe_batch_set_tuning", self._lib.swe_batch_set_tuning(self._h, chunk, grid))
@@ -350,7 +359,9 @@ class Batch:
         return c.value, g.value
 
     def set_stream(self, stream: int) -> None:
+        """Launch on a caller's HIP stream (0 = the library's own)."""
         _native.check("swe_batch_set_stream", self._lib.swe_batch_set_stream(self._h, stream or None))
+        self.stream = stream or 0
 
     def close(self) -> None:
         if getattr(self, "_h", None):
