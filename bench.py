@@ -151,6 +151,7 @@ def tsunami_adaptive(args, torch, arith=0):
                                min_depth=dmin, max_depth=dmax, solver=swe.SOLVER_AUG_RIEMANN,
                                boundary=swe.BC_OUTFLOW)
         op.set_arithmetic(arith)
+        fixed_tuning(args, op)  # --tuning: a launch shape other than the default
         z = np.zeros(mesh0.n)
         op.upload(z, z, z, z)
         # the Fortran host's set-up: the initial phase of SWE.f90:230-262 with
