@@ -244,9 +244,13 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
     // stage A: wet/dry zeroing (:60-80)
     const bool wetR = hR > drytol, wetL = hL > drytol;
     const double hR1 = wetR ? hR : 0.0, huR1 = wetR ? huR : 0.0;
-    const double uR1 = wetR ? huR / hR : 0.0, vR1 = wetR ? hvR / hR : 0.0;
+    // u steers the wall test and the Riemann structure: the reference's
+    // division; v only scales f-waves: a refined reciprocal, shared with the
+    // reverse of stage A
+    const double ihR = wetR ? rcp(hR) : 0.0, ihL = wetL ? rcp(hL) : 0.0;
+    const double uR1 = wetR ? huR / hR : 0.0, vR1 = wetR ? hvR * ihR : 0.0;
     const double hL1 = wetL ? hL : 0.0, huL1 = wetL ? huL : 0.0;
-    const double uL1 = wetL ? huL / hL : 0.0, vL1 = wetL ? hvL / hL : 0.0;
+    const double uL1 = wetL ? huL / hL : 0.0, vL1 = wetL ? hvL * ihL : 0.0;
 
     // stage B: wall ghosting (:87-115)
     // (the decisions only: hstar carries no gradient; one code path for both
@@ -308,8 +312,9 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
         const double deldelphi = -(K * delb);
         const double dec = delphi - deldelphi;
         const double dd = sE2 - sE1;
-        const double beta1 = (sE2 * delhu - dec) / dd;
-        const double beta2 = (dec - sE1 * delhu) / dd;
+        const double idd = rcp(dd);  // the wave strengths steer nothing
+        const double beta1 = (sE2 * delhu - dec) * idd;
+        const double beta2 = (dec - sE1 * delhu) * idd;
         const double f20 = beta1 * vL2, f22 = beta2 * vR2;
         // f21 = ((huR vR - huL vL) - f20) - f22
         const double gA = gf[2][1];
@@ -326,7 +331,6 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
         gvL2 += gf20 * beta1;
         (void)f20;
         (void)f22;
-        const double idd = rcp(dd);
         const double gn2 = gb2 * idd, gn1 = gb1 * idd;
         const double gdd = -(gb1 * beta1 + gb2 * beta2) * idd;
         double gdec = gn2 - gn1;
@@ -412,10 +416,11 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
         const double deldelphi = sd_ ? t2 : lo;
         const double d0 = delh - deldelh, d1 = delhu, d2 = delphi - deldelphi;
         const double det = -l2 + l0;
-        const double beta0 = (-(d0 * l2) + d1) / det;
+        const double idet = rcp(det);  // the wave strengths steer nothing
+        const double beta0 = (-(d0 * l2) + d1) * idet;
         const double Aq = d1 * r22 - l2 * d2, Bq = l0 * r22 - l2 * r20, Cq = l0 * d2 - d1 * r20;
-        const double beta1 = ((Aq - d0 * Bq) + Cq) / det;
-        const double beta2 = (-d1 + d0 * l0) / det;
+        const double beta1 = ((Aq - d0 * Bq) + Cq) * idet;
+        const double beta2 = (-d1 + d0 * l0) * idet;
         const double f00 = beta0 * l0, f02 = beta2 * l2;
 
         sw[0] = l0 * wall0;
@@ -450,7 +455,6 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
         double gbeta0 = gf[1][0] * r20 + gf00 * l0;
         double gr20 = gf[1][0] * beta0;
         double gl0 = gf00 * beta0;
-        const double idet = rcp(det);
         const double gn0 = gbeta0 * idet, gn1 = gbeta1 * idet, gn2 = gbeta2 * idet;
         const double gdet = -(gbeta0 * beta0 + gbeta1 * beta1 + gbeta2 * beta2) * idet;
         double gd0 = 0, gd1 = 0, gd2 = 0;
@@ -696,13 +700,13 @@ SWE_HD void geoclaw_adj(double hL, double hR, double huL, double huR, double hvL
     }
     // stage A reverse
     if (wetR) {
-        const double ih = rcp(hR);
+        const double ih = ihR;
         gin[1] += ghR1 - guR1 * uR1 * ih - gvR1 * vR1 * ih;
         gin[3] += ghuR1 + guR1 * ih;
         gin[5] += gvR1 * ih;
     }
     if (wetL) {
-        const double ih = rcp(hL);
+        const double ih = ihL;
         gin[0] += ghL1 - guL1 * uL1 * ih - gvL1 * vL1 * ih;
         gin[2] += ghuL1 + guL1 * ih;
         gin[4] += gvL1 * ih;
