@@ -216,20 +216,12 @@ SWE_HD void riemanntype_adj(double hL, double hR, double uL, double uR, double s
     }
     guR += gdelu;
     guL -= gdelu;
-    if (r_min) {
-        ghR += ghmin;
-        gsqR += gsqmin;
-    } else {
-        ghL += ghmin;
-        gsqL += gsqmin;
-    }
-    if (r_max) {
-        ghR += ghmax;
-        gsqR += gsqmax;
-    } else {
-        ghL += ghmax;
-        gsqL += gsqmax;
-    }
+    // selects, not branches: accumulating into a variable chosen by a
+    // condition makes the compiler index a private array in scratch
+    ghR += (r_min ? ghmin : 0.0) + (r_max ? ghmax : 0.0);
+    gsqR += (r_min ? gsqmin : 0.0) + (r_max ? gsqmax : 0.0);
+    ghL += (r_min ? 0.0 : ghmin) + (r_max ? 0.0 : ghmax);
+    gsqL += (r_min ? 0.0 : gsqmin) + (r_max ? 0.0 : gsqmax);
 }
 
 // Reverse of geoclaw<SOLVER> (c_bind_riemannsolvers.f90:16-176).  Inputs as
