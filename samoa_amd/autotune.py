@@ -8,7 +8,9 @@ few repetitions).  The state is saved before the trials and restored before
 every candidate, and every candidate's runs start the dt chain at the same
 dt0, so each launch shape is timed on the same waves; after the trials the
 state is restored once more, so tuning does not advance the simulation.
-Results are cached per device, mesh sizes, solver and arithmetic in
+A persistent grid is chosen only if it beats one workgroup per chunk of the
+same size by more than 2 %.  Results are cached per device, mesh sizes,
+solver and arithmetic in
 ~/.cache/samoa_amd/autotune.json; the caller's stream is restored.
 """
 from __future__ import annotations
@@ -20,6 +22,8 @@ import os
 import numpy as np
 
 CHUNKS = (128, 256, 512)
+# relative margin by which a persistent grid must beat one workgroup per chunk
+PERSISTENT_MARGIN = 0.02
 CACHE = os.path.join(os.path.expanduser("~"), ".cache", "samoa_amd", "autotune.json")
 
 
@@ -133,6 +137,13 @@ def tune(batch, dt0, steps: int = 20, repeats: int = 3, use_cache: bool = True,
             batch.upload(s, *st)
         batch.set_stream(caller_stream)
     best = min(timings, key=lambda r: r["ms_per_step"])
+    if best["grid"] != 0:
+        # a persistent grid must beat one workgroup per chunk (the hardware's
+        # dispatch, which balances uneven chunks over the run) by more than
+        # the trials' noise
+        one = next(r for r in timings if r["chunk"] == best["chunk"] and r["grid"] == 0)
+        if one["ms_per_step"] <= best["ms_per_step"] * (1.0 + PERSISTENT_MARGIN):
+            best = one
     result = {"chunk": best["chunk"], "grid": best["grid"], "ms_per_step": best["ms_per_step"],
               "trials": timings}
     batch.set_tuning(result["chunk"], result["grid"])

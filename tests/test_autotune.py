@@ -53,3 +53,62 @@ def test_tune_restores_state_and_stream():
     assert op.stream == s.cuda_stream and op.arithmetic == swe.ARITH_TOLERANCED
     assert np.array_equal(np.stack(op.download()), st)
     op.close()
+
+
+def test_persistent_grid_needs_a_margin(monkeypatch):
+    """The selection rule on made-up timings: a persistent grid within 2 % of
+    one workgroup per chunk loses to it; one more than 2 % faster wins."""
+    import types
+
+    def run_with(times):
+        calls = {}
+
+        class Batch(FakeBatch):
+            def download(self, s):
+                return (np.zeros(1),) * 4
+
+            def upload(self, s, *st):
+                pass
+
+            def set_stream(self, stream):
+                pass
+
+            def set_tuning(self, chunk, grid):
+                calls["shape"] = (chunk, grid)
+
+            def run(self, n, dt0, forward):
+                pass
+
+        class Event:
+            def __init__(self, enable_timing=True):
+                pass
+
+            def record(self, stream=None):
+                pass
+
+            def elapsed_time(self, other):
+                return times[calls["shape"]] * 20
+
+        fake_torch = types.SimpleNamespace(
+            cuda=types.SimpleNamespace(
+                current_device=lambda: 0,
+                get_device_properties=lambda d: types.SimpleNamespace(name="x",
+                                                                        multi_processor_count=256),
+                Stream=lambda device=None: types.SimpleNamespace(cuda_stream=1),
+                Event=Event, synchronize=lambda: None))
+        monkeypatch.setitem(__import__("sys").modules, "torch", fake_torch)
+        b = Batch([10], 0, 0)
+        b.n_scen = 1
+        return autotune.tune(b, 1.0, steps=20, repeats=1, use_cache=False)
+
+    base = {shape: 1.0 for shape in autotune.candidates(256)}
+    near = dict(base)
+    near[(256, 0)] = 0.500
+    near[(256, 2048)] = 0.495
+    r = run_with(near)
+    assert (r["chunk"], r["grid"]) == (256, 0)
+    far = dict(base)
+    far[(256, 0)] = 0.500
+    far[(256, 2048)] = 0.480
+    r = run_with(far)
+    assert (r["chunk"], r["grid"]) == (256, 2048)
