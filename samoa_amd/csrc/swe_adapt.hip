@@ -601,18 +601,28 @@ __device__ __forceinline__ int32_t edge_slot(const AdaptArgs &a, int64_t c, int 
 }
 
 // The cells that keep their place: a shifted copy of depth, plotter type,
-// tree offset and state, and the neighbours that keep their place too.
+// tree offset and state, and their neighbours.  An edge to a cell that keeps
+// its place too keeps its pairing; an edge to a changing cell gets -1 here and
+// its pairing from link_kernel, which writes both sides of every edge of the
+// changing cells.  Writing all three entries makes the neighbour store one
+// 12-B store per lane, dense across the wave.
 __global__ __launch_bounds__(AT) void emit_keep_kernel(AdaptArgs a) {
     const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
     if (c >= a.n) return;
     if (c == 0 && a.scen_dev) *a.scen_dev = a.scen_new;  // no host upload of the table
-    // every load is issued before the test, so that they are in flight
-    // together (a changing cell, one in twenty, reads them for nothing)
+    // every load, the neighbours' new indices included, is issued before the
+    // test (a changing cell, one in twenty, reads them for nothing); the
+    // empty asm keeps the compiler from sinking them below it, which would
+    // cost a memory round trip each
     const int32_t kc = a.kidx[c];
     const int8_t dc = a.depth[c], pc = a.plotter[c];
     const int64_t oc = a.off[c];
     const double ec = a.eta[c], uc = a.hu[c], vc = a.hv[c], bc = a.b[c];
-    const int32_t q3[3] = {a.nbr[3 * c], a.nbr[3 * c + 1], a.nbr[3 * c + 2]};
+    const int32_t q0 = a.nbr[3 * c], q1 = a.nbr[3 * c + 1], q2 = a.nbr[3 * c + 2];
+    const int32_t k0 = a.kidx[q0 >= 0 ? q0 >> 2 : c], k1 = a.kidx[q1 >= 0 ? q1 >> 2 : c],
+                  k2 = a.kidx[q2 >= 0 ? q2 >> 2 : c];
+    asm volatile("" ::"v"((int)dc), "v"((int)pc), "v"(oc), "v"(ec), "v"(uc), "v"(vc), "v"(bc), "v"(k0),
+                 "v"(k1), "v"(k2));
     if (kc < 0) return;  // a changing cell: emit_change_kernel
     const int64_t b0 = kc;
     a.depth_out[b0] = dc;
@@ -622,17 +632,9 @@ __global__ __launch_bounds__(AT) void emit_keep_kernel(AdaptArgs a) {
     a.hu_out[b0] = uc;
     a.hv_out[b0] = vc;
     a.b_out[b0] = bc;
-    // an edge to a cell that keeps its place too keeps its pairing; the
-    // others are written by link_kernel from the changing side
-    for (int e = 0; e < 3; ++e) {
-        const int32_t q = q3[e];
-        if (q < 0) {
-            a.nbr_out[3 * b0 + e] = -1;
-        } else {
-            const int32_t ko = a.kidx[q >> 2];
-            if (ko >= 0) a.nbr_out[3 * b0 + e] = (ko << 2) | (q & 3);
-        }
-    }
+    a.nbr_out[3 * b0] = q0 >= 0 && k0 >= 0 ? (k0 << 2) | (q0 & 3) : -1;
+    a.nbr_out[3 * b0 + 1] = q1 >= 0 && k1 >= 0 ? (k1 << 2) | (q1 & 3) : -1;
+    a.nbr_out[3 * b0 + 2] = q2 >= 0 && k2 >= 0 ? (k2 << 2) | (q2 & 3) : -1;
 }
 
 // The cells that split or merge (list[i], in curve order): their new cells,
