@@ -56,28 +56,26 @@ __constant__ int8_t c_child[17] = {3, 2, 1, 8, 7, 6, 5, 4, 0, -6, -7, -8, -1, -2
 // Per new cell: depth increment, type rule (0: child(p), 1: -child(-p),
 // 2/3: the same rules applied to the first child's type, 4/5: to the second
 // child's) and per edge either 0x10 | (old edge << 2) | half (half 2: the
-// whole old edge) or 0x20 | (cell of the pattern << 2) | its edge.
-struct SplitCell {
-    int8_t dd, rule;
-    uint8_t e[3];
-};
-__constant__ SplitCell c_split[4][4] = {
-    {{1, 0, {0x20 | (1 << 2) | 2, 0x10 | (2 << 2) | 2, 0x10 | (1 << 2) | 1}},
-     {1, 1, {0x10 | (1 << 2) | 0, 0x10 | (0 << 2) | 2, 0x20 | (0 << 2) | 0}},
-     {0, 0, {0, 0, 0}},
-     {0, 0, {0, 0, 0}}},
-    {{2, 2, {0x20 | (1 << 2) | 2, 0x10 | (1 << 2) | 1, 0x10 | (2 << 2) | 0}},
-     {2, 3, {0x10 | (2 << 2) | 1, 0x20 | (2 << 2) | 2, 0x20 | (0 << 2) | 0}},
-     {1, 1, {0x10 | (1 << 2) | 0, 0x10 | (0 << 2) | 2, 0x20 | (1 << 2) | 1}},
-     {0, 0, {0, 0, 0}}},
-    {{1, 0, {0x20 | (1 << 2) | 1, 0x10 | (2 << 2) | 2, 0x10 | (1 << 2) | 1}},
-     {2, 4, {0x20 | (2 << 2) | 2, 0x20 | (0 << 2) | 0, 0x10 | (0 << 2) | 0}},
-     {2, 5, {0x10 | (0 << 2) | 1, 0x10 | (1 << 2) | 0, 0x20 | (1 << 2) | 0}},
-     {0, 0, {0, 0, 0}}},
-    {{2, 2, {0x20 | (1 << 2) | 2, 0x10 | (1 << 2) | 1, 0x10 | (2 << 2) | 0}},
-     {2, 3, {0x10 | (2 << 2) | 1, 0x20 | (2 << 2) | 1, 0x20 | (0 << 2) | 0}},
-     {2, 4, {0x20 | (3 << 2) | 2, 0x20 | (1 << 2) | 1, 0x10 | (0 << 2) | 0}},
-     {2, 5, {0x10 | (0 << 2) | 1, 0x10 | (1 << 2) | 0, 0x20 | (2 << 2) | 0}}},
+// whole old edge) or 0x20 | (cell of the pattern << 2) | its edge.  Each new
+// cell is one 64-bit word (byte 0 the increment, byte 1 the rule, bytes 2-4
+// the edges), so that a pattern is read with two 16-B loads.
+constexpr uint64_t split_cell(int dd, int rule, int e0, int e1, int e2) {
+    return (uint64_t)(uint8_t)dd | ((uint64_t)(uint8_t)rule << 8) | ((uint64_t)(uint8_t)e0 << 16) |
+           ((uint64_t)(uint8_t)e1 << 24) | ((uint64_t)(uint8_t)e2 << 32);
+}
+__constant__ __align__(16) uint64_t c_split[4][4] = {
+    {split_cell(1, 0, 0x20 | (1 << 2) | 2, 0x10 | (2 << 2) | 2, 0x10 | (1 << 2) | 1),
+     split_cell(1, 1, 0x10 | (1 << 2) | 0, 0x10 | (0 << 2) | 2, 0x20 | (0 << 2) | 0), 0, 0},
+    {split_cell(2, 2, 0x20 | (1 << 2) | 2, 0x10 | (1 << 2) | 1, 0x10 | (2 << 2) | 0),
+     split_cell(2, 3, 0x10 | (2 << 2) | 1, 0x20 | (2 << 2) | 2, 0x20 | (0 << 2) | 0),
+     split_cell(1, 1, 0x10 | (1 << 2) | 0, 0x10 | (0 << 2) | 2, 0x20 | (1 << 2) | 1), 0},
+    {split_cell(1, 0, 0x20 | (1 << 2) | 1, 0x10 | (2 << 2) | 2, 0x10 | (1 << 2) | 1),
+     split_cell(2, 4, 0x20 | (2 << 2) | 2, 0x20 | (0 << 2) | 0, 0x10 | (0 << 2) | 0),
+     split_cell(2, 5, 0x10 | (0 << 2) | 1, 0x10 | (1 << 2) | 0, 0x20 | (1 << 2) | 0), 0},
+    {split_cell(2, 2, 0x20 | (1 << 2) | 2, 0x10 | (1 << 2) | 1, 0x10 | (2 << 2) | 0),
+     split_cell(2, 3, 0x10 | (2 << 2) | 1, 0x20 | (2 << 2) | 1, 0x20 | (0 << 2) | 0),
+     split_cell(2, 4, 0x20 | (3 << 2) | 2, 0x20 | (1 << 2) | 1, 0x10 | (0 << 2) | 0),
+     split_cell(2, 5, 0x10 | (0 << 2) | 1, 0x10 | (1 << 2) | 0, 0x20 | (2 << 2) | 0)},
 };
 
 // codes of the count kernel
@@ -168,6 +166,30 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
     }
     const uint32_t start = own;
     w[threadIdx.x] = own;
+    // a neighbour outside the tile is read from memory: at its own index
+    // (oi), every lane at some index (its own cell, for a neighbour in the
+    // tile or none), so that the three loads are issued together.  In the
+    // first sweep those words are the requests, fixed for the whole sweep,
+    // and are read once.
+    bool in_tile[3];
+    int64_t oi[3];
+    uint32_t fw[3] = {0u, 0u, 0u};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int64_t o = q[k] >> 2;
+        in_tile[k] = q[k] < 0 || (o >= t0 && o < t0 + AT);
+        oi[k] = in_tile[k] ? (live ? c : 0) : o;
+    }
+    if (first) {
+        int8_t fo[3], dp[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            fo[k] = flags[oi[k]];
+            dp[k] = depth[oi[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) fw[k] = request_word(fo[k], dp[k], dmin, dmax);
+    }
     __syncthreads();
     // each round settles at least one more cell of a chain; a tile that has
     // not settled after 32 rounds stamps itself for the next sweep
@@ -175,20 +197,22 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
     for (int round = 0; round < 32; ++round) {
         uint32_t nw = own;
         if (live) {
+            uint32_t ow[3] = {fw[0], fw[1], fw[2]};
+            if (!first) {
+                const uint32_t s0 = ld_relaxed(&slots[oi[0]]), s1 = ld_relaxed(&slots[oi[1]]),
+                               s2 = ld_relaxed(&slots[oi[2]]);
+                ow[0] = s0;
+                ow[1] = s1;
+                ow[2] = s2;
+            }
             uint32_t v[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const uint32_t sk = (own >> (2 * k)) & 3u;
                 v[k] = sk;
                 if (q[k] >= 0) {
-                    const int64_t o = q[k] >> 2;
-                    uint32_t ow;
-                    if (o >= t0 && o < t0 + AT)
-                        ow = w[o - t0];
-                    else
-                        ow = first ? request_word(flags[o], depth[o], dmin, dmax)
-                                   : ld_relaxed(&slots[o]);
-                    const uint32_t tk = (ow >> (2 * (q[k] & 3))) & 3u;
+                    const uint32_t wk = in_tile[k] ? w[(q[k] >> 2) - t0] : ow[k];
+                    const uint32_t tk = (wk >> (2 * (q[k] & 3))) & 3u;
                     v[k] = ((sk | tk) & 1u) | (sk & tk & 2u);
                 }
             }
@@ -483,18 +507,6 @@ __global__ __launch_bounds__(SCAN_T) void scan_codes_kernel(int64_t n, const int
     }
 }
 
-__device__ __forceinline__ int split_type(int p, int rule) {
-    const int c1 = c_child[p + 8], c2 = -c_child[-p + 8];
-    switch (rule) {
-        case 0: return c1;
-        case 1: return c2;
-        case 2: return c_child[c1 + 8];
-        case 3: return -c_child[-c1 + 8];
-        case 4: return c_child[c2 + 8];
-        default: return -c_child[-c2 + 8];
-    }
-}
-
 // Corners (unit square; entry, right angle, exit) of the cell at tree offset
 // `off` and depth d, from the corners c of its ancestor at depth d0 (or of
 // its root, d0 = 0): one halving per level along the path the offset spells
@@ -639,22 +651,33 @@ __global__ __launch_bounds__(AT) void emit_keep_kernel(AdaptArgs a) {
 
 // The cells that split or merge (list[i], in curve order): their new cells,
 // and for every half of every old edge the new cell-edge that lies on it.
+// The loads run in four rounds, each issued at once: the cell (and, for a
+// merge, its sibling c + 1); its split pattern and the children's types;
+// the grandchildren's types.  The state stays in registers: as far as the
+// compiler knows, the new cells' stores could alias the old arrays, and it
+// would read them again after every store.
 __global__ __launch_bounds__(AT) void emit_change_kernel(AdaptArgs a) {
     const int64_t i = blockIdx.x * (int64_t)AT + threadIdx.x;
     if (i >= a.n_changing) return;
     const int64_t c = a.list[i];
+    const int64_t c1 = c + 1 < a.n ? c + 1 : c;
     const int d = a.depth[c];
     const int p = a.plotter[c];
-    const int8_t k = a.code[c];
+    const int k = a.code[c];
     const int64_t b0 = a.base[c] & 0xffffffffLL;  // the high word counts changing cells
+    const int64_t off0 = a.off[c];
+    const double ec = a.eta[c], uc = a.hu[c], vc = a.hv[c], bc = a.b[c];
+    const double e1 = a.eta[c1], u1 = a.hu[c1], v1 = a.hv[c1], b1 = a.b[c1];
+    asm volatile("" ::"v"(d), "v"(p), "v"(k), "v"(b0), "v"(off0), "v"(ec), "v"(uc), "v"(vc),
+                 "v"(bc), "v"(e1), "v"(u1), "v"(v1), "v"(b1));
     if (k == CODE_MERGE_FIRST) {
         a.depth_out[b0] = (int8_t)(d - 1);
         a.plotter_out[b0] = c_child[p + 8];  // the parent's type
-        a.off_out[b0] = a.off[c];
-        a.eta_out[b0] = 0.5 * (a.eta[c] + a.eta[c + 1]);
-        a.hu_out[b0] = 0.5 * (a.hu[c] + a.hu[c + 1]);
-        a.hv_out[b0] = 0.5 * (a.hv[c] + a.hv[c + 1]);
-        a.b_out[b0] = 0.5 * (a.b[c] + a.b[c + 1]);
+        a.off_out[b0] = off0;
+        a.eta_out[b0] = 0.5 * (ec + e1);
+        a.hu_out[b0] = 0.5 * (uc + u1);
+        a.hv_out[b0] = 0.5 * (vc + v1);
+        a.b_out[b0] = 0.5 * (bc + b1);
         // the first child's hypotenuse is the parent's leg 2, its outer leg
         // half of the parent's hypotenuse; its leg 0 disappears
         put_slot(a.eslot, c, 1, 2, (int32_t)((b0 << 2) | 2));
@@ -668,29 +691,42 @@ __global__ __launch_bounds__(AT) void emit_change_kernel(AdaptArgs a) {
     } else {
         const int pat = k - CODE_SPLIT;
         const int cells = 2 + (pat & 1) + (pat >> 1);
-        int64_t coff = a.off[c];  // tree offset of the next child
-        double pc[6];             // the parent's corners, for the raster
+        const uint64_t w0 = c_split[pat][0], w1 = c_split[pat][1], w2 = c_split[pat][2],
+                       w3 = c_split[pat][3];
+        // the types of the rules: child(p), -child(-p), then both applied to
+        // each of those two
+        const int t0 = c_child[p + 8], t1 = -c_child[-p + 8];
+        const int t2 = c_child[t0 + 8], t3 = -c_child[-t0 + 8], t4 = c_child[t1 + 8],
+                  t5 = -c_child[-t1 + 8];
+        int64_t coff = off0;  // tree offset of the next child
+        double pc[6];         // the parent's corners, for the raster
         if (a.raster) tree_corners(coff, 0, d, pc);
         for (int j = 0; j < cells; ++j) {
-            const SplitCell sc = c_split[pat][j];
+            const uint64_t w = j == 0 ? w0 : j == 1 ? w1 : j == 2 ? w2 : w3;
+            const int rule = (int)((w >> 8) & 0xff);
             const int64_t idx = b0 + j;
-            const int dc = d + sc.dd;
+            const int dc = d + (int)(w & 0xff);
             a.depth_out[idx] = (int8_t)dc;
-            a.plotter_out[idx] = (int8_t)split_type(p, sc.rule);
+            a.plotter_out[idx] = (int8_t)(rule == 0   ? t0
+                                          : rule == 1 ? t1
+                                          : rule == 2 ? t2
+                                          : rule == 3 ? t3
+                                          : rule == 4 ? t4
+                                                      : t5);
             a.off_out[idx] = coff;
-            a.eta_out[idx] = a.eta[c];
-            a.hu_out[idx] = a.hu[c];
-            a.hv_out[idx] = a.hv[c];
+            a.eta_out[idx] = ec;
+            a.hu_out[idx] = uc;
+            a.hv_out[idx] = vc;
             if (a.raster) {
                 double cc[6] = {pc[0], pc[1], pc[2], pc[3], pc[4], pc[5]};
                 tree_corners(coff, d, dc, cc);
                 a.b_out[idx] = raster_cell(a, cc, dc);
             } else {
-                a.b_out[idx] = a.b[c];
+                a.b_out[idx] = bc;
             }
             coff += (int64_t)1 << (60 - dc);
             for (int e = 0; e < 3; ++e) {
-                const int src = sc.e[e];
+                const int src = (int)((w >> (16 + 8 * e)) & 0xff);
                 if (src & 0x20) {
                     a.nbr_out[3 * idx + e] = (int32_t)(((b0 + ((src >> 2) & 3)) << 2) | (src & 3));
                 } else {
@@ -703,28 +739,62 @@ __global__ __launch_bounds__(AT) void emit_change_kernel(AdaptArgs a) {
 
 // Over the changing cells: each new edge on an old edge meets the new edge on
 // the same half across it, and both sides are written (the other side may
-// keep its place and then leaves the pairing to this kernel).
+// keep its place and then leaves the pairing to this kernel).  The loads run
+// in four rounds, each issued at once: the cell; its slots, neighbours and
+// type; the neighbours' types and new indices; the slots of the neighbours
+// that change (read at every lane, at an index in range, and used where the
+// neighbour changes).
 __global__ __launch_bounds__(AT) void link_kernel(AdaptArgs a) {
     const int64_t i = blockIdx.x * (int64_t)AT + threadIdx.x;
     if (i >= a.n_changing) return;
     const int64_t c = a.list[i];
     const int pc = a.plotter[c];
+    const int2 *es = reinterpret_cast<const int2 *>(a.eslot + 6 * c);  // 24 B, 8-B aligned
+    const int2 x01 = es[0], x23 = es[1], x45 = es[2];
+    const int32_t x[3][2] = {{x01.x, x01.y}, {x23.x, x23.y}, {x45.x, x45.y}};
+    const int32_t q[3] = {a.nbr[3 * c], a.nbr[3 * c + 1], a.nbr[3 * c + 2]};
+    int64_t o[3];
+    int ko[3];
+#pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const int32_t x0 = a.eslot[6 * c + 2 * k], x1 = a.eslot[6 * c + 2 * k + 1];
-        if (x0 < 0) continue;
-        const int32_t q = a.nbr[3 * c + k];
+        o[k] = q[k] >= 0 ? q[k] >> 2 : c;
+        ko[k] = q[k] >= 0 ? q[k] & 3 : 0;
+    }
+    int po[3];
+    int32_t kk[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        po[k] = a.plotter[o[k]];
+        kk[k] = a.kidx[o[k]];
+    }
+    // the neighbour's cell-edge on half s: the same end of the shared edge is
+    // half s across it when the two run in opposite senses (types of
+    // different signs), half 1 - s when they run in the same
+    int32_t ev[3][2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
         for (int s = 0; s < 2; ++s) {
-            const int32_t x = s ? x1 : x0;
-            if (s == 1 && x1 == x0) break;
-            int32_t y = -1;
-            if (q >= 0) {
-                const int64_t o = q >> 2;
-                const int ko = q & 3;
-                const int so = ((pc > 0) == (a.plotter[o] > 0)) ? 1 - s : s;
-                y = edge_slot(a, o, ko, so);
-                if (y >= 0) a.nbr_out[3 * (int64_t)(y >> 2) + (y & 3)] = x;
-            }
-            a.nbr_out[3 * (int64_t)(x >> 2) + (x & 3)] = y;
+            const int so = ((pc > 0) == (po[k] > 0)) ? 1 - s : s;
+            ev[k][s] = a.eslot[6 * o[k] + 2 * ko[k] + so];
+        }
+    asm volatile("" ::"v"(ev[0][0]), "v"(ev[0][1]), "v"(ev[1][0]), "v"(ev[1][1]), "v"(ev[2][0]),
+                 "v"(ev[2][1]));
+    int32_t y[3][2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+            y[k][s] = q[k] < 0 ? -1 : (kk[k] >= 0 ? (kk[k] << 2) | ko[k] : ev[k][s]);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        if (x[k][0] < 0) continue;
+        for (int s = 0; s < 2; ++s) {
+            const int32_t xs = x[k][s];
+            if (s == 1 && xs == x[k][0]) break;
+            const int32_t ys = y[k][s];
+            if (ys >= 0) a.nbr_out[3 * (int64_t)(ys >> 2) + (ys & 3)] = xs;
+            a.nbr_out[3 * (int64_t)(xs >> 2) + (xs & 3)] = ys;
         }
     }
 }
