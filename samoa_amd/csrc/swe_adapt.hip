@@ -121,11 +121,13 @@ __device__ __forceinline__ uint32_t integrity_word(uint32_t v0, uint32_t v1, uin
 }
 
 // mark_edges_integrity (Conformity.f90:1271-1317), in place, over tiles of
-// AT cells.  A workgroup iterates its tile to a fixpoint of its own in LDS (a
-// neighbour in the tile is read there, any other from memory every round)
-// before it writes the tile back, so chains of forced refinements inside a
-// tile settle in one sweep.  Every update is monotone, which keeps the global
-// fixpoint the same.
+// SWEEP_TILE cells, SWEEP_CPT per thread (the sweep is bound by each
+// workgroup's chain of memory round trips: more cells per workgroup, fewer
+// workgroups in turn).  A workgroup iterates its tile to a fixpoint of its
+// own in LDS (a neighbour in the tile is read there, any other from memory
+// every round) before it writes the tile back, so chains of forced
+// refinements inside a tile settle in one sweep.  Every update is monotone,
+// which keeps the global fixpoint the same.
 //
 // The first sweep of a remeshing starts from the requests: every word, its
 // own and an outside neighbour's, is request_word of the flags, an earlier
@@ -138,12 +140,15 @@ __device__ __forceinline__ uint32_t integrity_word(uint32_t v0, uint32_t v1, uin
 // whose stamp is older than the previous sweep (a stamp of an earlier call
 // included, compared modulo 2^32: stamp_is_old) has read every neighbour word
 // since it last changed and is at its fixpoint: it is skipped.  *changed gets the sweep if a word changed.
+constexpr int SWEEP_CPT = 2;
+constexpr int SWEEP_TILE = AT * SWEEP_CPT;
 __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t *depth,
                                                         const int32_t *nbr, uint32_t *slots,
                                                         uint32_t *stamp, uint32_t *changed,
                                                         uint32_t sweep, const int8_t *flags,
                                                         int dmin, int dmax) {
-    __shared__ uint32_t w[AT];
+    constexpr int CPT = SWEEP_CPT, TILE = SWEEP_TILE;
+    __shared__ uint32_t w[TILE];
     __shared__ uint32_t s_skip;
     const bool first = (sweep & 0xffffu) == 1u;
     if (!first) {
@@ -151,101 +156,139 @@ __global__ __launch_bounds__(AT) void conf_sweep_kernel(int64_t n, const int8_t 
         __syncthreads();
         if (s_skip) return;  // uniform across the workgroup
     }
-    const int64_t t0 = blockIdx.x * (int64_t)AT;
-    const int64_t c = t0 + threadIdx.x;
-    const bool live = c < n;
-    int32_t q[3] = {-1, -1, -1};
-    int d = 0;
-    uint32_t own = 0u;
-    if (live) {
-        q[0] = nbr[3 * c];
-        q[1] = nbr[3 * c + 1];
-        q[2] = nbr[3 * c + 2];
-        d = depth[c];
-        own = first ? request_word(flags[c], d, dmin, dmax) : ld_relaxed(&slots[c]);
+    const int64_t t0 = blockIdx.x * (int64_t)TILE;
+    // cell j of the thread is t0 + j AT + threadIdx.x, so that each load is
+    // coalesced across the workgroup
+    int64_t c[CPT];
+    bool live[CPT];
+    int32_t q[CPT][3];
+    int d[CPT];
+    uint32_t own[CPT], start[CPT];
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        c[j] = t0 + j * AT + threadIdx.x;
+        live[j] = c[j] < n;
+        const int64_t cc = live[j] ? c[j] : 0;
+        q[j][0] = live[j] ? nbr[3 * cc] : -1;
+        q[j][1] = live[j] ? nbr[3 * cc + 1] : -1;
+        q[j][2] = live[j] ? nbr[3 * cc + 2] : -1;
+        d[j] = live[j] ? depth[cc] : 0;
+        own[j] = !live[j] ? 0u : first ? request_word(flags[cc], d[j], dmin, dmax)
+                                       : ld_relaxed(&slots[cc]);
     }
-    const uint32_t start = own;
-    w[threadIdx.x] = own;
+#pragma unroll
+    for (int j = 0; j < CPT; ++j) {
+        start[j] = own[j];
+        w[j * AT + threadIdx.x] = own[j];
+    }
     // a neighbour outside the tile is read from memory: at its own index
     // (oi), every lane at some index (its own cell, for a neighbour in the
-    // tile or none), so that the three loads are issued together.  In the
-    // first sweep those words are the requests, fixed for the whole sweep,
-    // and are read once.
-    bool in_tile[3];
-    int64_t oi[3];
-    uint32_t fw[3] = {0u, 0u, 0u};
+    // tile or none), so that the loads are issued together.  In the first
+    // sweep those words are the requests, fixed for the whole sweep, and are
+    // read once.
+    bool in_tile[CPT][3];
+    int64_t oi[CPT][3];
+    uint32_t fw[CPT][3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const int64_t o = q[k] >> 2;
-        in_tile[k] = q[k] < 0 || (o >= t0 && o < t0 + AT);
-        oi[k] = in_tile[k] ? (live ? c : 0) : o;
-    }
-    if (first) {
-        int8_t fo[3], dp[3];
+    for (int j = 0; j < CPT; ++j)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            fo[k] = flags[oi[k]];
-            dp[k] = depth[oi[k]];
+            const int64_t o = q[j][k] >> 2;
+            in_tile[j][k] = q[j][k] < 0 || (o >= t0 && o < t0 + TILE);
+            oi[j][k] = in_tile[j][k] ? (live[j] ? c[j] : 0) : o;
+            fw[j][k] = 0u;
         }
+    if (first) {
+        int8_t fo[CPT][3], dp[CPT][3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) fw[k] = request_word(fo[k], dp[k], dmin, dmax);
+        for (int j = 0; j < CPT; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                fo[j][k] = flags[oi[j][k]];
+                dp[j][k] = depth[oi[j][k]];
+            }
+#pragma unroll
+        for (int j = 0; j < CPT; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) fw[j][k] = request_word(fo[j][k], dp[j][k], dmin, dmax);
     }
     __syncthreads();
     // each round settles at least one more cell of a chain; a tile that has
     // not settled after 32 rounds stamps itself for the next sweep
     bool settled = false;
     for (int round = 0; round < 32; ++round) {
-        uint32_t nw = own;
-        if (live) {
-            uint32_t ow[3] = {fw[0], fw[1], fw[2]};
-            if (!first) {
-                const uint32_t s0 = ld_relaxed(&slots[oi[0]]), s1 = ld_relaxed(&slots[oi[1]]),
-                               s2 = ld_relaxed(&slots[oi[2]]);
-                ow[0] = s0;
-                ow[1] = s1;
-                ow[2] = s2;
-            }
-            uint32_t v[3];
+        uint32_t ow[CPT][3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const uint32_t sk = (own >> (2 * k)) & 3u;
-                v[k] = sk;
-                if (q[k] >= 0) {
-                    const uint32_t wk = in_tile[k] ? w[(q[k] >> 2) - t0] : ow[k];
-                    const uint32_t tk = (wk >> (2 * (q[k] & 3))) & 3u;
-                    v[k] = ((sk | tk) & 1u) | (sk & tk & 2u);
+        for (int j = 0; j < CPT; ++j)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ow[j][k] = fw[j][k];
+        if (!first) {
+            uint32_t sl[CPT][3];
+#pragma unroll
+            for (int j = 0; j < CPT; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) sl[j][k] = ld_relaxed(&slots[oi[j][k]]);
+#pragma unroll
+            for (int j = 0; j < CPT; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) ow[j][k] = sl[j][k];
+        }
+        uint32_t nw[CPT];
+        bool moved = false;
+#pragma unroll
+        for (int j = 0; j < CPT; ++j) {
+            nw[j] = own[j];
+            if (live[j]) {
+                uint32_t v[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const uint32_t sk = (own[j] >> (2 * k)) & 3u;
+                    v[k] = sk;
+                    if (q[j][k] >= 0) {
+                        const uint32_t wk =
+                            in_tile[j][k] ? w[(q[j][k] >> 2) - t0] : ow[j][k];
+                        const uint32_t tk = (wk >> (2 * (q[j][k] & 3))) & 3u;
+                        v[k] = ((sk | tk) & 1u) | (sk & tk & 2u);
+                    }
                 }
+                nw[j] = integrity_word(v[0], v[1], v[2], d[j]);
             }
-            nw = integrity_word(v[0], v[1], v[2], d);
+            moved |= nw[j] != own[j];
         }
-        const bool moved = nw != own;
         __syncthreads();  // every read of this round is done
-        if (moved) {
-            w[threadIdx.x] = nw;
-            own = nw;
-        }
+#pragma unroll
+        for (int j = 0; j < CPT; ++j)
+            if (nw[j] != own[j]) {
+                w[j * AT + threadIdx.x] = nw[j];
+                own[j] = nw[j];
+            }
         if (!__syncthreads_or(moved)) {
             settled = true;
             break;
         }
     }
-    const bool diff = own != start;
-    if (live && (diff || first)) st_relaxed(&slots[c], own);
-    if (diff) {
+    bool any = false;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            if (q[k] < 0) continue;
-            const int64_t o = q[k] >> 2;
-            if (o < t0 || o >= t0 + AT) {
-                uint32_t *st = &stamp[o / AT];
-                if (ld_relaxed(st) != sweep) st_relaxed(st, sweep);
+    for (int j = 0; j < CPT; ++j) {
+        const bool diff = own[j] != start[j];
+        any |= diff;
+        if (live[j] && (diff || first)) st_relaxed(&slots[c[j]], own[j]);
+        if (diff) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                if (q[j][k] < 0) continue;
+                const int64_t o = q[j][k] >> 2;
+                if (o < t0 || o >= t0 + TILE) {
+                    uint32_t *st = &stamp[o / TILE];
+                    if (ld_relaxed(st) != sweep) st_relaxed(st, sweep);
+                }
             }
         }
     }
     if (!settled && threadIdx.x == 0) st_relaxed(&stamp[blockIdx.x], sweep);
     // one status write per workgroup, and only if no other has written it:
     // stores from every wave to one word contend across the whole device
-    if (__syncthreads_or(diff) && threadIdx.x == 0 && ld_relaxed(changed) != sweep)
+    if (__syncthreads_or(any) && threadIdx.x == 0 && ld_relaxed(changed) != sweep)
         st_relaxed(changed, sweep);
 }
 
@@ -918,12 +961,12 @@ static hipError_t wait_results(const int64_t *host, int64_t seq, hipStream_t s) 
 hipError_t adapt_plan(const AdaptArgs &a, hipStream_t s, int first_batch, bool offsets_valid,
                       bool prefix_ahead, int max_sweeps, uint32_t epoch, int64_t *host,
                       int64_t *host_dev, int64_t *seq, AdaptPlan *plan) {
-    const unsigned nb = blocks_for(a.n, AT);
+    const unsigned nb = blocks_for(a.n, SWEEP_TILE);  // conformity sweep tiles
     const int64_t tiles = adapt_scan_tiles(a.n);
     int64_t *res = a.bsum + tiles + 1;
     hipError_t e;
     if (!offsets_valid) {  // else the offsets of the unchanged mesh are in place
-        tree_size_kernel<<<nb, AT, 0, s>>>(a.n, a.depth, a.off);
+        tree_size_kernel<<<blocks_for(a.n, AT), AT, 0, s>>>(a.n, a.depth, a.off);
         if ((e = scan_i64(a.off, a.off, a.n, a.bsum, s)) != hipSuccess) return e;
         copy_total_kernel<<<1, 64, 0, s>>>(res, a.bsum, tiles);
     }
