@@ -498,10 +498,11 @@ __global__ __launch_bounds__(SCAN_T) void count_reduce_kernel(int64_t n, const i
 }
 
 // base[c] = exclusive prefix of the counts, rebuilt from the decision codes
-// (8 consecutive cells per thread, bsum holding the scanned tile sums),
-// kidx[c] = the new index of a cell that keeps its place, -1 for one that
-// changes, and list = the changing cells in curve order.  The prefixes go
-// through LDS so that the stores are coalesced.
+// (8 consecutive cells per thread, bsum holding the scanned tile sums), for
+// the cells that change only; kidx[c] = the new index of a cell that keeps
+// its place (its base), -1 for one that changes; list = the changing cells
+// in curve order.  The prefixes go through LDS so that the stores are
+// coalesced.
 __global__ __launch_bounds__(SCAN_T) void scan_codes_kernel(int64_t n, const int8_t *code,
                                                             const int64_t *bsum, int64_t *base,
                                                             int32_t *kidx, int32_t *list) {
@@ -542,10 +543,11 @@ __global__ __launch_bounds__(SCAN_T) void scan_codes_kernel(int64_t n, const int
         const int i = j * SCAN_T + threadIdx.x;
         if (tile + i < n) {
             const int64_t b = stage[i + i / SCAN_ITEMS];
-            base[tile + i] = b;
             // a kept cell counts exactly 1 (one new cell, no change); its
             // new index is below 2^29, and -1 marks the others
-            kidx[tile + i] = kstage[i + i / SCAN_ITEMS] ? (int32_t)(b & 0xffffffffLL) : -1;
+            const bool kept = kstage[i + i / SCAN_ITEMS] != 0;
+            kidx[tile + i] = kept ? (int32_t)(b & 0xffffffffLL) : -1;
+            if (!kept) base[tile + i] = b;
         }
     }
 }
@@ -1026,6 +1028,21 @@ hipError_t transfer_adjoint(int64_t n_old, const int8_t *code, const int64_t *ba
                             hipStream_t s) {
     transfer_adjoint_kernel<<<blocks_for(n_old, AT), AT, 0, s>>>(n_old, code, base, b_sampled, le,
                                                                 lu, lv, gb, le_o, lu_o, lv_o, gb_o);
+    return hipGetLastError();
+}
+
+// The full base of a remeshing's record: a kept cell's is its new index.
+__global__ __launch_bounds__(AT) void record_base_kernel(int64_t n, const int32_t *kidx,
+                                                         const int64_t *base, int64_t *out) {
+    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+    if (c >= n) return;
+    const int32_t k = kidx[c];
+    out[c] = k >= 0 ? (int64_t)k : base[c];
+}
+
+hipError_t record_base(int64_t n, const int32_t *kidx, const int64_t *base, int64_t *out,
+                       hipStream_t s) {
+    record_base_kernel<<<blocks_for(n, AT), AT, 0, s>>>(n, kidx, base, out);
     return hipGetLastError();
 }
 

@@ -1171,8 +1171,7 @@ int swe_adapt_record_device(swe_handle *h, int8_t *code, int64_t *base, int64_t 
         HIP_TRY(swe::identity_record(n, code, base, st));
     } else {
         HIP_TRY(hipMemcpyAsync(code, B.ad_code, (size_t)n, hipMemcpyDeviceToDevice, st));
-        HIP_TRY(hipMemcpyAsync(base, B.ad_base, sizeof(int64_t) * (size_t)n,
-                               hipMemcpyDeviceToDevice, st));
+        HIP_TRY(swe::record_base(n, B.ad_kidx, B.ad_base, base, st));
     }
     return SWE_OK;
 }

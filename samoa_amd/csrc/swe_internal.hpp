@@ -169,6 +169,10 @@ hipError_t transfer_adjoint(int64_t n_old, const int8_t *code, const int64_t *ba
                             hipStream_t s);
 // The record of a remeshing that changed nothing.
 hipError_t identity_record(int64_t n, int8_t *code, int64_t *base, hipStream_t s);
+// out[c] = kidx[c] where the cell is kept, else base[c] (base holds the
+// changing cells' prefixes only)
+hipError_t record_base(int64_t n, const int32_t *kidx, const int64_t *base, int64_t *out,
+                       hipStream_t s);
 // Writes the new mesh and state (outputs sized for n_new).
 // prefix_done: adapt_plan has launched the prefix pass already.
 hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s, bool prefix_done);
