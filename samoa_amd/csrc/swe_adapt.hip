@@ -425,39 +425,52 @@ __device__ __forceinline__ int64_t code_count(int k) {
 // tile.  It also checks the fixpoint the sweeps reached: status bit 2 if the
 // integrity rule would still change a word (more sweeps are needed, and
 // nothing else here is valid), bit 0 if a coarsening cell's sibling is not a
-// coarsening leaf (cannot happen at the fixpoint).
+// coarsening leaf (cannot happen at the fixpoint).  A sibling in the tile is
+// checked from LDS, where every cell leaves its depth and the two edge values
+// that check reads; one outside it from memory.
 __global__ __launch_bounds__(SCAN_T) void count_reduce_kernel(int64_t n, const int8_t *depth,
                                                               const int32_t *nbr,
                                                               const uint32_t *slots,
                                                               const int64_t *off, int8_t *code,
                                                               int64_t *bsum, uint32_t *status) {
     __shared__ int64_t lds[SCAN_T / 64 + 1];
+    // per cell: depth << 2 | edge 1 refines << 1 | edge 0 coarsens
+    __shared__ uint8_t s_info[SCAN_TILE];
     const int64_t tile = blockIdx.x * (int64_t)SCAN_TILE;
     int64_t sum = 0;
     bool moved = false, bad = false;
     // the sweeps are over: the words are read with plain loads, and every
     // cell's loads are issued before any is used (own word, depth and
-    // neighbours, then the neighbours' words)
+    // neighbours; then the neighbours' words and, where the cell's own word
+    // could coarsen edge 0, its tree offset), each at an index in range
     uint32_t own[SCAN_ITEMS], nw[SCAN_ITEMS][3];
     int32_t q[SCAN_ITEMS][3];
     int dep[SCAN_ITEMS];
+    int64_t ofs[SCAN_ITEMS];
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; ++j) {
         const int64_t c = tile + j * SCAN_T + threadIdx.x;
         const bool live = c < n;
-        own[j] = live ? slots[c] : 0u;
-        dep[j] = live ? depth[c] : 0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) q[j][k] = live ? nbr[3 * c + k] : -1;
+        const int64_t cc = live ? c : 0;
+        own[j] = slots[cc];
+        dep[j] = depth[cc];
+        const int32_t q0 = nbr[3 * cc], q1 = nbr[3 * cc + 1], q2 = nbr[3 * cc + 2];
+        q[j][0] = live ? q0 : -1;
+        q[j][1] = live ? q1 : -1;
+        q[j][2] = live ? q2 : -1;
     }
-#pragma unroll
-    for (int j = 0; j < SCAN_ITEMS; ++j)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) nw[j][k] = q[j][k] >= 0 ? slots[q[j][k] >> 2] : 0u;
 #pragma unroll
     for (int j = 0; j < SCAN_ITEMS; ++j) {
         const int64_t c = tile + j * SCAN_T + threadIdx.x;
-        if (c >= n) continue;
+        const int64_t cc = c < n ? c : 0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) nw[j][k] = slots[q[j][k] >= 0 ? q[j][k] >> 2 : cc];
+        ofs[j] = (own[j] & 2u) ? off[cc] : 0;
+    }
+    uint32_t vv[SCAN_ITEMS];
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const int64_t c = tile + j * SCAN_T + threadIdx.x;
         uint32_t v[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -465,22 +478,39 @@ __global__ __launch_bounds__(SCAN_T) void count_reduce_kernel(int64_t n, const i
             const uint32_t tk = (nw[j][k] >> (2 * (q[j][k] & 3))) & 3u;
             v[k] = q[j][k] < 0 ? sk : (((sk | tk) & 1u) | (sk & tk & 2u));
         }
-        const uint32_t v0 = v[0], v1 = v[1], v2 = v[2];
+        vv[j] = v[0] | (v[1] << 2) | (v[2] << 4);
+        if (c < n) {
+            moved |= integrity_word(v[0], v[1], v[2], dep[j]) != own[j];
+            s_info[j * SCAN_T + threadIdx.x] =
+                (uint8_t)((dep[j] << 2) | ((v[1] & 1u) << 1) | ((v[0] >> 1) & 1u));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < SCAN_ITEMS; ++j) {
+        const int64_t c = tile + j * SCAN_T + threadIdx.x;
+        if (c >= n) continue;
+        const uint32_t v0 = vv[j] & 3u, v1 = (vv[j] >> 2) & 3u, v2 = (vv[j] >> 4) & 3u;
         const int d = dep[j];
-        moved |= integrity_word(v0, v1, v2, d) != own[j];
         int k = CODE_KEEP;
         if (v1 & 1u) {
             k = CODE_SPLIT + ((int)(v2 & 1u) | ((int)(v0 & 1u) << 1));
         } else if (v0 & 2u) {
             // d > 0 here: the integrity rule clears the coarsen bits of root cells
             const int64_t parent = (int64_t)1 << (61 - d);
-            const bool first = off[c] % parent == 0;
+            const bool first = ofs[j] % parent == 0;
             const int64_t sib = first ? c + 1 : c - 1;
-            bool ok = sib >= 0 && sib < n && depth[sib] == d;
-            if (ok) {
-                const uint32_t so = slots[sib];
-                ok = (edge_value(slots, nbr, sib, so, 0) & 2u) &&
-                     !(edge_value(slots, nbr, sib, so, 1) & 1u);
+            bool ok;
+            if (sib >= tile && sib < tile + SCAN_TILE && sib < n) {
+                const uint32_t inf = s_info[sib - tile];
+                ok = (int)(inf >> 2) == d && (inf & 1u) && !(inf & 2u);
+            } else {
+                ok = sib >= 0 && sib < n && depth[sib] == d;
+                if (ok) {
+                    const uint32_t so = slots[sib];
+                    ok = (edge_value(slots, nbr, sib, so, 0) & 2u) &&
+                         !(edge_value(slots, nbr, sib, so, 1) & 1u);
+                }
             }
             bad |= !ok;
             k = first ? CODE_MERGE_FIRST : CODE_MERGE_SECOND;
