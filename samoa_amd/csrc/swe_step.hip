@@ -477,34 +477,43 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
 // a 4M-cell mesh (16k chunks) take four rounds of memory latency.
 constexpr int FIN_THREADS = 1024;
 
+// One workgroup per scenario.  Each thread reads FIN_PER partials of each
+// array at once (every load at an index in range, the ones past the end
+// unused), and the slot's time and dt are read at the start, so that the
+// chain of memory round trips is short for meshes up to FIN_PER * FIN_THREADS
+// chunks.
+constexpr int FIN_PER = 16;
 __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(StepArgs a) {
     __shared__ double s_min[FIN_THREADS / 64];
     __shared__ long long s_ref[FIN_THREADS / 64];
     const int s = blockIdx.x;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const Scen sc = a.scen ? a.scen[s] : a.scen0;
+    Slots *sl = &a.slots[s];
+    const double time0 = sl->time, dt0 = sl->dt;  // every thread: no branch to wait at
     const double huge = 1.7976931348623157e308;  // huge(1.0_SR)
-    double m0 = huge, m1 = huge, m2 = huge, m3 = huge;
+    double m = huge;
     long long r = 0;
     const double *pm = a.part_min + sc.chunk_off;
     const int32_t *pr = a.part_nref + sc.chunk_off;
     const int64_t nc = sc.nchunks;
-    int64_t i = t;
-    for (; i + 3 * FIN_THREADS < nc; i += 4 * FIN_THREADS) {
-        const double v0 = pm[i], v1 = pm[i + FIN_THREADS], v2 = pm[i + 2 * FIN_THREADS],
-                     v3 = pm[i + 3 * FIN_THREADS];
-        r += (long long)pr[i] + pr[i + FIN_THREADS] + pr[i + 2 * FIN_THREADS] +
-             pr[i + 3 * FIN_THREADS];
-        m0 = dmin2(m0, v0);
-        m1 = dmin2(m1, v1);
-        m2 = dmin2(m2, v2);
-        m3 = dmin2(m3, v3);
+    for (int64_t i0 = t; i0 < nc; i0 += (int64_t)FIN_PER * FIN_THREADS) {
+        double v[FIN_PER];
+        int32_t w[FIN_PER];
+#pragma unroll
+        for (int j = 0; j < FIN_PER; ++j) {
+            const int64_t i = i0 + (int64_t)j * FIN_THREADS;
+            const int64_t ii = i < nc ? i : i0;
+            v[j] = pm[ii];
+            w[j] = pr[ii];
+        }
+#pragma unroll
+        for (int j = 0; j < FIN_PER; ++j) {
+            const bool in = i0 + (int64_t)j * FIN_THREADS < nc;
+            m = dmin2(m, in ? v[j] : huge);
+            r += in ? w[j] : 0;
+        }
     }
-    for (; i < nc; i += FIN_THREADS) {
-        m0 = dmin2(m0, pm[i]);
-        r += pr[i];
-    }
-    double m = dmin2(dmin2(m0, m1), dmin2(m2, m3));
     m = wave_min(m);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) r += __shfl_xor(r, off, 64);
@@ -521,13 +530,12 @@ __global__ __launch_bounds__(FIN_THREADS) void finalize_kernel(StepArgs a) {
             m = dmin2(m, s_min[w]);
             r += s_ref[w];
         }
-        Slots *sl = &a.slots[s];
         sl->res_dtcfl = m;
         sl->res_nref = r;
         if (a.out_dtcfl) a.out_dtcfl[s] = m;
         if (a.out_nref) a.out_nref[s] = (int64_t)r;
         if (a.device_dt) {
-            sl->time = sl->time + sl->dt;
+            sl->time = time0 + dt0;
             double next = sc.courant * m;
             if (sc.dt_clamp > 0.0) next = dmin2(sc.dt_clamp, next);
             sl->dt = next;
