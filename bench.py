@@ -49,6 +49,13 @@ sys.path.insert(0, ROOT)
 METRIC = "cell-updates/s/GPU plain vs batched ensemble; adjoint/forward cost ratio"
 HBM_PEAK = 8.0e12           # MI355X HBM3E, bytes/s
 BYTES_PER_CELL = 56         # read eta, hu, hv, b; write eta, hu, hv (FP64)
+# how b is sampled: the tsunami raster (513 x 513 nodes) read bilinearly, as
+# the reference reads its ASAGI grid, -5000 m outside the nodes; each cell's
+# b is that raster averaged over min(16, raster_cap - depth) halvings of the
+# cell, as the reference's _ADAPT_INTEGRATE build does for gridded data, at
+# set-up and for every cell a remeshing splits
+BATHYMETRY = ("tsunami raster 513x513, bilinear (ASAGI-style, -5000 m outside), "
+              "cell-averaged as _ADAPT_INTEGRATE")
 PROFILE_TRAFFIC = os.path.join(ROOT, "profiles", "traffic.json")
 
 
@@ -394,7 +401,7 @@ def tsunami_frozen(args, torch):
             "workload": f"tsunami_frozen_uniform_d{d}",
             "cells": cells, "depth": d, "scaling_m": S, "solver": "aug_riemann",
             "boundary": "outflow", "dry_tolerance": 1e-3, "courant": courant,
-            "simulated_time_s": t_end,
+            "bathymetry": BATHYMETRY, "simulated_time_s": t_end,
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK, "unit": "GB/s",
@@ -823,6 +830,7 @@ def main():
             "cells_max": ad["cells_max"], "depths": [16, 26], "scaling_m": 1.0e6,
             "solver": "aug_riemann", "boundary": "outflow", "dry_tolerance": 1e-3,
             "courant": 0.45, "nadapt": args.nadapt, "adapt_tol_m": args.adapt_tol,
+            "bathymetry": BATHYMETRY,
             "simulated_time_s": ad["simulated_time_s"],
         },
         "roofline": {
