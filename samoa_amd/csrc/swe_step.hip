@@ -119,6 +119,11 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
         const int64_t xcd = v % 8, i = v / 8;
         if (i >= q + (xcd < r ? 1 : 0)) continue;  // uniform across the workgroup
         const int64_t chunk = xcd * q + (xcd < r ? xcd : r) + i;
+        // Raised wave priority from here to the edge loop: the chunk's
+        // descriptor and staging loads, and the edge list, are a chain of
+        // memory round trips and barriers; issued ahead of the other waves'
+        // solver arithmetic, they overlap it (3-5 % off the step).
+        __builtin_amdgcn_s_setprio(2);
         const int s = a.chunk_scen ? a.chunk_scen[chunk] : 0;
         const Scen sc = RAW ? a.scen0 : a.scen[s];
         // The state and topology arrays are addressed from per-scenario
@@ -215,6 +220,7 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
             if (own & (1 << k)) s_list[pre++] = (uint16_t)((t << 2) | k);
         __syncthreads();
 
+        __builtin_amdgcn_s_setprio(0);
         for (int e = t; e < total; e += CH) {
             if constexpr (TOL) {
                 const int ent = s_list[e];
