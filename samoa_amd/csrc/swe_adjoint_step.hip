@@ -57,6 +57,9 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
         const int64_t xcd = v % 8, i = v / 8;
         if (i >= q + (xcd < r ? 1 : 0)) continue;
         const int64_t chunk = xcd * q + (xcd < r ? xcd : r) + i;
+        // raised wave priority for the chunk's loads and edge list, as in the
+        // forward step kernel
+        __builtin_amdgcn_s_setprio(2);
         const int s = a.chunk_scen ? a.chunk_scen[chunk] : 0;
         const Scen sc = a.scen[s];
         const int64_t lbase = (chunk - sc.chunk_off) * CH;
@@ -141,6 +144,7 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
             if (own & (1 << k)) s_list[pre++] = (uint16_t)((t << 2) | k);
         __syncthreads();
 
+        __builtin_amdgcn_s_setprio(0);
         for (int e = t; e < total; e += CH) {
             const int ent = s_list[e];
             const int lc = ent >> 2, k = ent & 3;
