@@ -75,19 +75,21 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
         int8_t plot = 1, dep = 0;
         double dir_e = 0.0, dir_u = 0.0, dir_v = 0.0, dir_b = 0.0;
         if (active) {
+            // every load in one round: the state, topology, masks and
+            // adjoint inputs first, the depth (behind a branch) last
             const int64_t x = gs + lc_t;
             const double qe = a.eta[x], qu = a.hu[x], qv = a.hv[x], qb = a.b[x];
-            s_q[0][t] = qe;
-            s_q[1][t] = qu;
-            s_q[2][t] = qv;
-            s_q[3][t] = qb;
+            const int mk = a.masks[x];
+            const double le = a.leta[x], lu = a.lhu[x], lv = a.lhv[x];
             nb[0] = a.nbr[3 * (gt + lc_t)];
             nb[1] = a.nbr[3 * (gt + lc_t) + 1];
             nb[2] = a.nbr[3 * (gt + lc_t) + 2];
             plot = a.plotter[gt + lc_t];
             dep = (int8_t)(sc.depth >= 0 ? sc.depth : a.depth[gt + lc_t]);
-            const int mk = a.masks[x];
-            const double le = a.leta[x], lu = a.lhu[x], lv = a.lhv[x];
+            s_q[0][t] = qe;
+            s_q[1][t] = qu;
+            s_q[2][t] = qv;
+            s_q[3][t] = qb;
             double g2e = le, g2u = lu, g2v = lv;
             if (mk & 2) {  // drying fix: eta = min(b, 0), p = 0
                 g2e = g2u = g2v = 0.0;
