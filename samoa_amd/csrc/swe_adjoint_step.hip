@@ -191,8 +191,11 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
                 ou = a.hu[x];
                 ov = a.hv[x];
                 ob = a.b[x];
-                const int dl = sc.depth >= 0 ? sc.depth : a.depth[gt + o];
                 const int mk = a.masks[x];
+                // read with the rest, then selected (a read under the mask
+                // test would wait for the mask first)
+                const double l0 = a.leta[x], l1 = a.lhu[x], l2 = a.lhv[x];
+                const int dl = sc.depth >= 0 ? sc.depth : a.depth[gt + o];
                 // the neighbour's seed exactly as its own chunk forms it
                 // ((s * lam) after the limiter, times the edge length), so
                 // that results do not depend on the chunking
@@ -200,9 +203,9 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
                 const double scale = -dt / vol;
                 const double len_o = S * (ko == 1 ? s_leg[dl] : s_leg[dl + 1]);
                 const bool dry = (mk & 2) != 0;
-                so0 = (scale * (dry ? 0.0 : a.leta[x])) * len_o;
-                so1 = (scale * (dry ? 0.0 : a.lhu[x])) * len_o;
-                so2 = (scale * (dry ? 0.0 : a.lhv[x])) * len_o;
+                so0 = (scale * (dry ? 0.0 : l0)) * len_o;
+                so1 = (scale * (dry ? 0.0 : l1)) * len_o;
+                so2 = (scale * (dry ? 0.0 : l2)) * len_o;
             }
             const bool c_left = bnd || (a.forward ? (cc < o) : (cc > o));
             const double sx = c_left ? nx : -nx, sy = c_left ? ny : -ny;
