@@ -34,6 +34,7 @@ template <int SOLVER, int CH>
 __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
     __shared__ double s_q[4][CH];        // eta, hu, hv, b of the chunk at t_n
     __shared__ double s_seed[3][CH];     // s * lam^{n+1} after the limiter (per cell)
+    __shared__ double s_gb[SOLVER != 1 ? CH : 1];  // each cell's b gradient so far
     __shared__ double s_g[4][3][CH];     // transpose-flux contributions per (edge, cell)
     __shared__ int32_t s_nbr[3][CH];
     __shared__ int8_t s_plot[CH];
@@ -81,6 +82,11 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
             const double qe = a.eta[x], qu = a.hu[x], qv = a.hv[x], qb = a.b[x];
             const int mk = a.masks[x];
             const double le = a.leta[x], lu = a.lhu[x], lv = a.lhv[x];
+            // only this cell's thread adds to its b gradient: read now, with
+            // the other loads, not after the edge loop (the f-wave kernel, at
+            // 168 VGPRs and three waves per SIMD, reads it at the end: the
+            // staged read costs it a register and a wave)
+            if constexpr (SOLVER != 1) s_gb[t] = a.gb[x];
             nb[0] = a.nbr[3 * (gt + lc_t)];
             nb[1] = a.nbr[3 * (gt + lc_t) + 1];
             nb[2] = a.nbr[3 * (gt + lc_t) + 2];
@@ -259,7 +265,8 @@ __global__ __launch_bounds__(CH) void adjoint_step_kernel(AdjArgs a) {
             a.leta_out[x] = le;
             a.lhu_out[x] = lu;
             a.lhv_out[x] = lv;
-            a.gb[x] += lb;
+            if constexpr (SOLVER != 1) a.gb[x] = s_gb[t] + lb;
+            else a.gb[x] += lb;
         }
         __syncthreads();  // LDS is reused by the next chunk
     }
