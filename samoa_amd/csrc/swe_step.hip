@@ -153,10 +153,17 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
         double my_e = 0.0, my_u = 0.0, my_v = 0.0;  // TOL: this thread's cell state
         if (active) {
             if constexpr (TOL) {
+                // the topology loads go out with the state's, before the
+                // arithmetic that waits for the state
                 my_e = c_eta[t];
                 my_u = c_hu[t];
                 my_v = c_hv[t];
-                const tol::CellQ cq = tol::cell_q(my_e, my_u, my_v, c_b[t], drytol, g);
+                const double my_b = c_b[t];
+                nb[0] = c_nbr[3 * t];
+                nb[1] = c_nbr[3 * t + 1];
+                nb[2] = c_nbr[3 * t + 2];
+                plot = c_plot[t];
+                const tol::CellQ cq = tol::cell_q(my_e, my_u, my_v, my_b, drytol, g);
                 s_q[0][t] = cq.h;
                 s_q[1][t] = cq.wu;
                 s_q[2][t] = cq.wv;
@@ -167,11 +174,11 @@ __global__ __launch_bounds__(CH) __attribute__((amdgpu_waves_per_eu(4))) void eu
                 s_q[1][t] = c_hu[t];
                 s_q[2][t] = c_hv[t];
                 s_q[3][t] = c_b[t];
+                nb[0] = c_nbr[3 * t];
+                nb[1] = c_nbr[3 * t + 1];
+                nb[2] = c_nbr[3 * t + 2];
+                plot = c_plot[t];
             }
-            nb[0] = c_nbr[3 * t];
-            nb[1] = c_nbr[3 * t + 1];
-            nb[2] = c_nbr[3 * t + 2];
-            plot = c_plot[t];
             if (RAW) {  // a neighbour outside the mesh reads as the boundary
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
