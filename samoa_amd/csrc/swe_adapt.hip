@@ -726,9 +726,9 @@ __global__ __launch_bounds__(AT) void emit_keep_kernel(AdaptArgs a) {
 
 // The cells that split or merge (list[i], in curve order): their new cells,
 // and for every half of every old edge the new cell-edge that lies on it.
-// The loads run in four rounds, each issued at once: the cell (and, for a
-// merge, its sibling c + 1); its split pattern and the children's types;
-// the grandchildren's types.  The state stays in registers: as far as the
+// The loads run in four rounds, each issued at once: the list entry; the
+// cell (and, for a merge, its sibling c + 1); its split pattern and the
+// children's types; the grandchildren's types.  The state stays in registers: as far as the
 // compiler knows, the new cells' stores could alias the old arrays, and it
 // would read them again after every store.
 __global__ __launch_bounds__(AT) void emit_change_kernel(AdaptArgs a) {
