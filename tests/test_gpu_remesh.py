@@ -91,6 +91,49 @@ def test_random_requests_match_oracle(seed):
     op.close()
 
 
+def test_merging_pairs_across_count_tiles():
+    """The count pass checks a merging cell's sibling in LDS when both lie in
+    its tile of 2048 cells, and in memory when the pair straddles two tiles.
+    A uniform depth-12 mesh with one cell refined shifts the sibling pairs to
+    odd first indices, so that pairs straddle the tile boundaries; then every
+    cell requests coarsening.  Mesh and state equal the oracle's."""
+    tile = 2048  # cells per workgroup of the count pass
+    base = host.mesh_uniform(12)
+    flags = np.zeros(base.n, np.int8)
+    flags[0] = 1
+    cur = host.mesh_from_depths(po.adapt(base.depth, base.plotter, base.nbr, flags,
+                                         np.zeros((4, base.n)), dmin=1, dmax=13)[0])
+    dep = cur.depth.astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(np.int64(1) << (60 - dep))[:-1]])
+    first = off % (np.int64(1) << (61 - dep)) == 0
+    straddle = [t - 1 for t in range(tile, cur.n, tile)
+                if first[t - 1] and dep[t - 1] == dep[t]]
+    assert len(straddle) >= 2, straddle  # the case this test is for
+    rng = np.random.default_rng(5)
+    st = np.stack([5.0 + rng.random(cur.n), rng.normal(size=cur.n), rng.normal(size=cur.n),
+                   -rng.random(cur.n)])
+    op = swe.EulerTimestep(cur, scaling=1.0, dry_tolerance=1e-3, min_depth=1, max_depth=13)
+    op.upload(*st)
+    for r in range(2):
+        print("case: round", r, "cells", cur.n, "straddling pairs at", straddle, flush=True)
+        flags = np.full(cur.n, -1, np.int8)
+        d2, p2, s2, nb2, _, _ = po.adapt(cur.depth, cur.plotter, cur.nbr, flags, st, dmin=1,
+                                         dmax=13, coords=cur.coords)
+        n, _ = op.adapt(flags)
+        assert n == len(d2) and n < cur.n, r
+        d, p, nb = op.download_mesh()
+        assert np.array_equal(d, d2) and np.array_equal(p, p2) and np.array_equal(nb, nb2), r
+        assert np.array_equal(np.stack(op.download()), s2), r
+        cur = host.mesh_from_depths(d2)
+        st = s2
+        dep = cur.depth.astype(np.int64)
+        off = np.concatenate([[0], np.cumsum(np.int64(1) << (60 - dep))[:-1]])
+        first = off % (np.int64(1) << (61 - dep)) == 0
+        straddle = [t - 1 for t in range(tile, cur.n, tile)
+                    if first[t - 1] and dep[t - 1] == dep[t]]
+    op.close()
+
+
 @pytest.mark.parametrize("pattern", ["all", "random"])
 def test_tiny_meshes_down_to_the_two_root_cells(pattern):
     """Meshes smaller than one tile, from the two-cell mesh: four rounds that
