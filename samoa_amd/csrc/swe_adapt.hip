@@ -42,6 +42,11 @@ namespace swe {
 namespace {
 
 constexpr int AT = 256;           // threads per workgroup of the per-cell kernels
+// the emit kernels' workgroups, measured: the kept-cell emit streams over
+// every cell (larger workgroups, fewer of them), the changing-cell emit and
+// link run over the ~5 % that change (smaller ones, spread over more CUs)
+constexpr int KEEP_T = 512;
+constexpr int CHANGE_T = 128;
 constexpr int SCAN_T = 256;       // scan: threads per tile
 constexpr int SCAN_ITEMS = 8;     // scan: consecutive items per thread
 constexpr int SCAN_TILE = SCAN_T * SCAN_ITEMS;
@@ -693,8 +698,8 @@ __device__ __forceinline__ int32_t edge_slot(const AdaptArgs &a, int64_t c, int 
 // its pairing from link_kernel, which writes both sides of every edge of the
 // changing cells.  Writing all three entries makes the neighbour store one
 // 12-B store per lane, dense across the wave.
-__global__ __launch_bounds__(AT) void emit_keep_kernel(AdaptArgs a) {
-    const int64_t c = blockIdx.x * (int64_t)AT + threadIdx.x;
+__global__ __launch_bounds__(KEEP_T) void emit_keep_kernel(AdaptArgs a) {
+    const int64_t c = blockIdx.x * (int64_t)KEEP_T + threadIdx.x;
     if (c >= a.n) return;
     if (c == 0 && a.scen_dev) *a.scen_dev = a.scen_new;  // no host upload of the table
     // every load, the neighbours' new indices included, is issued before the
@@ -731,8 +736,8 @@ __global__ __launch_bounds__(AT) void emit_keep_kernel(AdaptArgs a) {
 // children's types; the grandchildren's types.  The state stays in registers: as far as the
 // compiler knows, the new cells' stores could alias the old arrays, and it
 // would read them again after every store.
-__global__ __launch_bounds__(AT) void emit_change_kernel(AdaptArgs a) {
-    const int64_t i = blockIdx.x * (int64_t)AT + threadIdx.x;
+__global__ __launch_bounds__(CHANGE_T) void emit_change_kernel(AdaptArgs a) {
+    const int64_t i = blockIdx.x * (int64_t)CHANGE_T + threadIdx.x;
     if (i >= a.n_changing) return;
     const int64_t c = a.list[i];
     const int64_t c1 = c + 1 < a.n ? c + 1 : c;
@@ -819,8 +824,8 @@ __global__ __launch_bounds__(AT) void emit_change_kernel(AdaptArgs a) {
 // type; the neighbours' types and new indices; the slots of the neighbours
 // that change (read at every lane, at an index in range, and used where the
 // neighbour changes).
-__global__ __launch_bounds__(AT) void link_kernel(AdaptArgs a) {
-    const int64_t i = blockIdx.x * (int64_t)AT + threadIdx.x;
+__global__ __launch_bounds__(CHANGE_T) void link_kernel(AdaptArgs a) {
+    const int64_t i = blockIdx.x * (int64_t)CHANGE_T + threadIdx.x;
     if (i >= a.n_changing) return;
     const int64_t c = a.list[i];
     const int pc = a.plotter[c];
@@ -1086,11 +1091,11 @@ hipError_t adapt_emit(const AdaptArgs &a, hipStream_t s, bool prefix_done) {
     if (tiles > 0 && !prefix_done)
         scan_codes_kernel<<<(unsigned)tiles, SCAN_T, 0, s>>>(a.n, a.code, a.bsum, a.base, a.kidx,
                                                              a.list);
-    emit_keep_kernel<<<blocks_for(a.n, AT), AT, 0, s>>>(a);
+    emit_keep_kernel<<<blocks_for(a.n, KEEP_T), KEEP_T, 0, s>>>(a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || a.n_changing == 0) return e;
-    emit_change_kernel<<<blocks_for(a.n_changing, AT), AT, 0, s>>>(a);
-    link_kernel<<<blocks_for(a.n_changing, AT), AT, 0, s>>>(a);
+    emit_change_kernel<<<blocks_for(a.n_changing, CHANGE_T), CHANGE_T, 0, s>>>(a);
+    link_kernel<<<blocks_for(a.n_changing, CHANGE_T), CHANGE_T, 0, s>>>(a);
     return hipGetLastError();
 }
 
